@@ -1,0 +1,243 @@
+/*
+ * starfish_hip.h -- C-ABI of libstarfish_hip.so, the MI355X (gfx950) implementation of
+ * the Starfish ES-PIC hot path: KineticMaterial particle move, Field2D charge scatter and
+ * moment sampling, and the Boltzmann-electron nonlinear Poisson solve, on uniform XY and
+ * RZ meshes.
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 (SF_OK) on success, < 0 on error, > 0 for
+ *     "completed with a warning" (e.g. SF_WARN_NOT_CONVERGED).  No exceptions or abort()
+ *     cross the ABI; sf_last_error() returns the message of the last failure on a context.
+ *   - Node arrays are contiguous FP64 in the reference's flattened order u = j*ni + i
+ *     (Mesh.IJtoN, core/domain/Mesh.java:893-897; Vec.deflate, core/common/Vec.java:165-183),
+ *     which is what Solver.MeshData and PoissonSolver hold (pic/PoissonSolver.java:123-125).
+ *   - Face arrays are [2*nj + 2*ni] in the reference's Face order: RIGHT[nj], TOP[ni],
+ *     LEFT[nj], BOTTOM[ni] (core/domain/Mesh.java:109-121).
+ *   - Host buffers are caller-owned and only read/written during the call; device buffers
+ *     are owned by the context.  One host thread per context; calls are synchronous.
+ *   - Paths in citations are relative to the reference's src/starfish/ directory.
+ */
+#ifndef STARFISH_HIP_H
+#define STARFISH_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SF_OK 0
+#define SF_WARN_NOT_CONVERGED 1
+#define SF_ERR_ARG (-1)
+#define SF_ERR_HIP (-2)
+#define SF_ERR_STATE (-3)
+#define SF_ERR_OOM (-4)
+#define SF_ERR_UNSUPPORTED (-5)
+#define SF_ERR_TIMEOUT (-6)
+
+/* domain types (core/domain/DomainModule.java DomainType) */
+#define SF_DOMAIN_XY 0
+#define SF_DOMAIN_RZ 1
+#define SF_DOMAIN_ZR 2
+
+/* mesh face boundary types (Mesh.DomainBoundaryType, core/domain/Mesh.java:146-160) */
+#define SF_FACE_OPEN (-1)
+#define SF_FACE_DIRICHLET 0
+#define SF_FACE_NEUMANN 1
+#define SF_FACE_PERIODIC 2
+#define SF_FACE_SYMMETRY 3
+#define SF_FACE_MESH 4
+#define SF_FACE_SINK 5
+#define SF_FACE_CIRCUIT 6
+
+/* node types (Mesh.NodeType, core/domain/Mesh.java:176-190) */
+#define SF_NODE_UNKNOWN (-2)
+#define SF_NODE_OPEN (-1)
+#define SF_NODE_DIRICHLET 0
+#define SF_NODE_INSULATOR 1
+#define SF_NODE_INTERNAL 2
+
+/* boundary (spline) types (Boundary.BoundaryType, core/boundaries/Boundary.java:120-131) */
+#define SF_BND_DIRICHLET 0
+#define SF_BND_OPEN 1
+#define SF_BND_SINK 2
+#define SF_BND_VIRTUAL 3
+#define SF_BND_INSULATOR 4
+
+typedef struct sf_ctx sf_ctx;
+typedef struct sf_geometry sf_geometry;
+
+/* UniformMesh metrics (core/domain/UniformMesh.java:114-161) */
+typedef struct {
+    int domain;          /* SF_DOMAIN_* */
+    int ni, nj;          /* number of nodes */
+    double x0[2];        /* origin */
+    double dh[2];        /* spacing */
+} sf_mesh_desc;
+
+/* ------------------------------------------------------------------------------------------
+ * Library / context
+ * ------------------------------------------------------------------------------------------ */
+const char* sf_version(void);
+/* Creates a context on HIP device `device` (device = -1: host-only context that supports the
+   geometry builder, the shared generator and nothing that touches a GPU); replaces the per-run state Starfish keeps in
+   KineticMaterial.MeshData and Solver.MeshData (core/materials/KineticMaterial.java:1314-1440,
+   core/solver/Solver.java:119-135). */
+int sf_ctx_create(int device, sf_ctx** out);
+int sf_ctx_destroy(sf_ctx* ctx);
+const char* sf_last_error(const sf_ctx* ctx);
+/* Time step used by push, injection rewind and sources (TimeModule dt, core/common/TimeModule.java). */
+int sf_ctx_set_dt(sf_ctx* ctx, double dt);
+/* Seeds the context's java.util.Random-compatible shared generator (Starfish.random,
+   core/common/Starfish.java:238); `raw_state` != 0 sets the 48-bit LCG state directly. */
+int sf_ctx_set_rng(sf_ctx* ctx, int64_t seed, int raw_state);
+int sf_ctx_get_rng_state(const sf_ctx* ctx, int64_t* state);
+/* Blocks until all queued device work of the context is done. */
+int sf_ctx_synchronize(sf_ctx* ctx);
+
+/* ------------------------------------------------------------------------------------------
+ * Host-side mesh geometry (what Starfish's Mesh.setBoundaries + Mesh.initNodes produce,
+ * core/domain/Mesh.java:364-402, 1204-1468, called from DomainModule.start,
+ * core/domain/DomainModule.java:267-279).  A Java host already has these arrays and passes
+ * them to sf_mesh_setup directly; non-Java hosts build them here.
+ * Boundaries are linear polylines (Spline.setPath "M ... L ...", core/boundaries/Spline.java:126-285).
+ * The Monte-Carlo interface-node volumes draw from the context's shared generator, as the
+ * reference draws from Starfish.rnd2() (Mesh.computeInterfaceNodeVol, Mesh.java:1053-1084).
+ * ------------------------------------------------------------------------------------------ */
+int sf_geometry_build(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_type, const double* face_value,
+                      int nbnd, const int* bnd_type, const double* bnd_value, const int* bnd_has_material,
+                      const int* bnd_npts, const double* bnd_pts, const int* bnd_reverse, sf_geometry** out);
+/* sizes: nseg (linear segments), nlist (total length of the node segment lists) */
+int sf_geometry_sizes(const sf_geometry* g, int* nseg, int* nlist);
+int sf_geometry_export(const sf_geometry* g, int* node_type, double* node_bc, double* node_vol, double* seg_xy,
+                       int* seg_btype, int* seg_absorb, int* node_seg_ptr, int* node_seg_idx);
+int sf_geometry_free(sf_geometry* g);
+
+/* ------------------------------------------------------------------------------------------
+ * Mesh upload (device-resident for the run).  node_type/node_bc/node_vol: [ni*nj];
+ * face_type/face_value: face layout; seg_*: [nseg] linear boundary segments with parent
+ * boundary type and "has a material" (default surface interaction absorbs,
+ * Material.performSurfaceInteraction, core/materials/Material.java:279-336); node segment lists
+ * in CSR form (Node.segments, Mesh.java:127-135).
+ * ------------------------------------------------------------------------------------------ */
+int sf_mesh_setup(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_type, const double* face_value,
+                  const int* node_type, const double* node_bc, const double* node_vol, int nseg,
+                  const double* seg_xy, const int* seg_btype, const int* seg_absorb, const int* node_seg_ptr,
+                  const int* node_seg_idx);
+
+/* ------------------------------------------------------------------------------------------
+ * Kinetic species (KineticMaterial, core/materials/KineticMaterial.java; Material keys
+ * charge/molwt/spwt/init nd_back, Material.java:62-80).
+ * vmax: expected maximum particle speed (m/s), sets the fixed-point scale of the velocity
+ * moment channels (0 selects 1e5 m/s).  capacity: maximum particles held on the device.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct {
+    double charge;       /* in units of e (Material "charge") */
+    double molwt;        /* in amu (Material "molwt") */
+    double spwt;         /* specific weight (KineticMaterial "spwt") */
+    double nd_back;      /* background density added after deposit (init nd_back) */
+    double vmax;         /* fixed-point velocity scale hint, m/s */
+    int64_t capacity;    /* particle capacity */
+    int track_ids;       /* 1: carry an int32 particle id through sort/compaction */
+} sf_species_desc;
+int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species);
+
+/* Adds particles from FP64 host arrays (positions x,y and velocities u,v,w; z / theta is not
+   kept).  mode 1 = KineticMaterial.addParticle(MeshData, Particle) semantics
+   (KineticMaterial.java:759-802: XtoL, clamp to ni-1/nj-1, half-step velocity rewind with the
+   current E, new id); mode 0 = load the state as given (restart-style, no rewind).
+   ids may be NULL (then ids continue the species counter). */
+int sf_particles_add(sf_ctx* ctx, int species, int64_t n, const double* x, const double* y, const double* u,
+                     const double* v, const double* w, const int* ids, int mode);
+int sf_particles_count(sf_ctx* ctx, int species, int64_t* n);
+/* Downloads particles: physical position (x,y), logical coordinates (lc0, lc1), velocities, ids.
+   Any pointer may be NULL.  Order is the device order (sorted by cell). */
+int sf_particles_download(sf_ctx* ctx, int species, double* x, double* y, double* lc0, double* lc1, double* u,
+                          double* v, double* w, int* ids);
+/* Device-side preload: n particles uniformly over the mesh cells whose four nodes are OPEN,
+   velocities drifting Maxwellian (drift vector + temperature in K, Utils.SampleMaxw3D's 1.2
+   widening kept, core/common/Utils.java:49-57), counter-based generator keyed by seed.
+   No rewind (a preloaded population starts at the half step). */
+int sf_particles_preload(sf_ctx* ctx, int species, int64_t n, const double* drift3, double temp, uint64_t seed);
+
+/* ------------------------------------------------------------------------------------------
+ * Hot path: one KineticMaterial.updateFields() for a species
+ * (KineticMaterial.java:117-197 + updateGasProperties 1553-1595 + Material.update nd_back
+ * 618-632): gather E, leapfrog push, XY drift / RZ rotation, boundary processing (segments,
+ * faces), deposit of n, nu, nv, nw and the moment sums, counting sort + compaction of dead
+ * particles.  Deposits are 64-bit fixed point, so densities are bit-identical run to run.
+ * The species' E field is the context's (sf_poisson_solve / sf_efield_set).
+ * ------------------------------------------------------------------------------------------ */
+int sf_push_deposit(sf_ctx* ctx, int species);
+/* which: 0 nd, 1 u, 2 v, 3 w (instantaneous, KineticMaterial.updateFields);
+   4 count-sum, 5 u-sum, 6 v-sum, 7 w-sum, 8 uu-sum, 9 vv-sum, 10 ww-sum, 11 mpc-sum
+   (KineticMaterial.updateSamples); 12 t, 13 t1, 14 t2, 15 t3, 16 nd-ave, 17 u-ave, 18 v-ave,
+   19 w-ave, 20 p, 21 mpc (KineticMaterial.computeFields, evaluated on request). */
+int sf_species_field(sf_ctx* ctx, int species, int which, double* out);
+/* KineticMaterial.clearSamples (KineticMaterial.java:1509-1529) */
+int sf_species_clear_samples(sf_ctx* ctx, int species);
+/* mover sums of the last push, multiplied by mass: mass, momentum[3], "energy" = mass*sum(mpw*|v|)
+   (KineticMaterial.java:247-259, 406-413) */
+int sf_species_stats(sf_ctx* ctx, int species, double* out5);
+
+/* ------------------------------------------------------------------------------------------
+ * Field solve: PoissonSolver (pic/PoissonSolver.java) with SolverModule's keys
+ * (core/solver/SolverModule.java:71-86).  The operator is the reference's finite-volume
+ * assembly (Solver.initCoefficients, core/solver/Solver.java:204-256, 451-726) applied
+ * matrix-free; Dirichlet and one-sided Neumann rows are eliminated into an equivalent
+ * symmetric system solved with Jacobi-PCG in FP32 and FP64 residual refinement inside the
+ * FP64 Newton iteration on the Boltzmann term (Solver.solveNonLinearNR, Solver.java:293-387).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int linear;          /* PoissonSolver "linear" */
+    double n0, Te0, phi0;/* Boltzmann reference density (m^-3), temperature (eV), potential (V) */
+    double eps_r;        /* relative permittivity */
+    int max_it;          /* max PCG iterations per linear solve (SolverModule "max_it") */
+    double rel_tol;      /* stop when ||F||_2 / ||F_ref||_2 <= rel_tol (FP64 residual) */
+    int nl_max_it;       /* max Newton iterations (SolverModule "nl_max_it") */
+    double nl_tol;       /* Newton also stops when ||y||_2/N < nl_tol (reference criterion) */
+} sf_poisson_desc;
+typedef struct {
+    int newton_iterations;
+    int pcg_iterations;      /* summed over Newton steps and refinements */
+    double rel_residual;     /* final ||F||_2 / ||F_ref||_2 */
+    double ref_norm;         /* reference-style norm ||F||_2 / N */
+    double ms;               /* device time of the solve */
+} sf_solve_info;
+int sf_poisson_setup(sf_ctx* ctx, const sf_poisson_desc* desc);
+/* SolverModule.updateRho (SolverModule.java:144-171): rho = sum over created species of q*nd */
+int sf_rho_update(sf_ctx* ctx);
+int sf_rho_set(sf_ctx* ctx, const double* rho);
+int sf_rho_get(sf_ctx* ctx, double* rho);
+/* PoissonSolver.update: solves for phi (device-resident, starting from the current phi).
+   phi_io may be NULL; if not NULL its content is uploaded as the initial guess and the
+   result is downloaded into it. */
+int sf_poisson_solve(sf_ctx* ctx, double* phi_io, sf_solve_info* info);
+int sf_phi_set(sf_ctx* ctx, const double* phi);
+int sf_phi_get(sf_ctx* ctx, double* phi);
+/* PotentialSolver.updateGradientField (pic/PotentialSolver.java:55-70): E = -grad(phi) */
+int sf_efield_update(sf_ctx* ctx);
+int sf_efield_set(sf_ctx* ctx, const double* efi, const double* efj);
+int sf_efield_get(sf_ctx* ctx, double* efi, double* efj);
+/* y = A x with the reference operator (diagnostic; x, y host [ni*nj] FP64) */
+int sf_operator_apply(sf_ctx* ctx, const double* x, double* y);
+
+/* ------------------------------------------------------------------------------------------
+ * Sources (sources/MaxwellianSource.java:44-125, core/source/Source.java:123-198): sampled on
+ * the host from the shared generator in the reference's order, then added with
+ * sf_particles_add mode 1.
+ * ------------------------------------------------------------------------------------------ */
+int sf_source_maxwellian(sf_ctx* ctx, int species, int npts, const double* pts, int reverse, double mdot,
+                         double v_drift, double temp, int* source);
+/* SourceModule.sampleSources (core/source/SourceModule.java:231-283): every source injects */
+int sf_sources_sample(sf_ctx* ctx, int64_t* injected);
+
+/* ------------------------------------------------------------------------------------------
+ * Timing of the last sf_push_deposit (device ms): push+deposit kernel, sort, normalise.
+ * ------------------------------------------------------------------------------------------ */
+int sf_last_timings(sf_ctx* ctx, double* out4);
+int sf_set_timing(sf_ctx* ctx, int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

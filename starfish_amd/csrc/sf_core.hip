@@ -1,0 +1,522 @@
+// Context, mesh, species and particle I/O entry points of the C-ABI.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+#include "sf_internal.h"
+
+using namespace sf;
+
+int sf_fail(sf_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+namespace {
+template <class T>
+int dalloc(sf_ctx* ctx, T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return sf_fail(ctx, SF_ERR_OOM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                                        hipGetErrorString(e));
+    return SF_OK;
+}
+template <class T>
+void dfree(T*& p) {
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+#define TRY(x)                 \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_ < 0) return rc_; \
+    } while (0)
+
+Species* get_species(sf_ctx* ctx, int s) {
+    if (!ctx || s < 0 || s >= (int)ctx->species.size()) return nullptr;
+    return ctx->species[s];
+}
+}  // namespace
+
+extern "C" {
+
+const char* sf_version(void) { return "starfish_hip 0.1 (gfx950)"; }
+
+int sf_ctx_create(int device, sf_ctx** out) {
+    if (!out) return SF_ERR_ARG;
+    *out = nullptr;
+    if (device == -1) {
+        // host-only context: geometry, generator and sources; every device entry point fails
+        sf_ctx* c = new sf_ctx();
+        c->device = -1;
+        c->rng.set_seed(0);
+        *out = c;
+        return SF_OK;
+    }
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return SF_ERR_HIP;
+    if (device < 0 || device >= ndev) return SF_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return SF_ERR_HIP;
+    sf_ctx* c = new sf_ctx();
+    c->device = device;
+    c->rng.set_seed(0);    // Starfish's shared generator is new Random(0)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SF_ERR_HIP;
+    }
+    for (auto& ev : c->ev) hipEventCreate(&ev);
+    *out = c;
+    return SF_OK;
+}
+
+static void free_species(Species* sp) {
+    for (int b = 0; b < 2; b++) {
+        dfree(sp->buf[b].cell);
+        dfree(sp->buf[b].fx);
+        dfree(sp->buf[b].fy);
+        dfree(sp->buf[b].vx);
+        dfree(sp->buf[b].vy);
+        dfree(sp->buf[b].vz);
+        dfree(sp->buf[b].id);
+    }
+    dfree(sp->key);
+    dfree(sp->acc);
+    dfree(sp->sums);
+    dfree(sp->mpc_sum);
+    dfree(sp->den);
+    dfree(sp->counts);
+    dfree(sp->offsets);
+    dfree(sp->cursor);
+    dfree(sp->blocksum);
+    dfree(sp->stats);
+    delete sp;
+}
+
+static void free_mesh(sf_ctx* c) {
+    dfree(c->d_face);
+    dfree(c->d_nseg_ptr);
+    dfree(c->d_nseg_idx);
+    dfree(c->d_seg);
+    dfree(c->d_hasseg);
+    dfree(c->d_node_vol);
+    dfree(c->d_phi);
+    dfree(c->d_rho);
+    dfree(c->d_efi);
+    dfree(c->d_efj);
+    dfree(c->d_ef32);
+}
+
+int sf_ctx_destroy(sf_ctx* ctx) {
+    if (!ctx) return SF_ERR_ARG;
+    if (ctx->device < 0) {
+        delete ctx;
+        return SF_OK;
+    }
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (Species* sp : ctx->species) free_species(sp);
+    if (ctx->poisson) poisson_free(ctx->poisson);
+    free_mesh(ctx);
+    for (auto& ev : ctx->ev) hipEventDestroy(ev);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SF_OK;
+}
+
+const char* sf_last_error(const sf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sf_ctx_set_dt(sf_ctx* ctx, double dt) {
+    SF_CHECK_CTX(ctx);
+    if (!(dt > 0) || !std::isfinite(dt)) return sf_fail(ctx, SF_ERR_ARG, "dt must be positive");
+    ctx->dt = dt;
+    return SF_OK;
+}
+
+int sf_ctx_set_rng(sf_ctx* ctx, int64_t seed, int raw_state) {
+    SF_CHECK_CTX(ctx);
+    if (raw_state) ctx->rng.seed = (uint64_t)seed & ((1ULL << 48) - 1);
+    else ctx->rng.set_seed(seed);
+    return SF_OK;
+}
+
+int sf_ctx_get_rng_state(const sf_ctx* ctx, int64_t* state) {
+    if (!ctx || !state) return SF_ERR_ARG;
+    *state = (int64_t)ctx->rng.seed;
+    return SF_OK;
+}
+
+int sf_ctx_synchronize(sf_ctx* ctx) {
+    SF_CHECK_CTX(ctx);
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SF_OK;
+}
+
+int sf_mesh_setup(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_type, const double* face_value,
+                  const int* node_type, const double* node_bc, const double* node_vol, int nseg,
+                  const double* seg_xy, const int* seg_btype, const int* seg_absorb, const int* node_seg_ptr,
+                  const int* node_seg_idx) {
+    SF_CHECK_CTX(ctx);
+    if (!mesh || !face_type || !node_type || !node_vol || !node_seg_ptr)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_setup: null argument");
+    if (mesh->ni < 2 || mesh->nj < 2 || mesh->ni > 65536 || mesh->nj > 65536)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_setup: mesh size %d x %d outside [2, 65536]", mesh->ni, mesh->nj);
+    if (!(mesh->dh[0] > 0 && mesh->dh[1] > 0)) return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_setup: spacing must be > 0");
+    if (mesh->domain != SF_DOMAIN_XY && mesh->domain != SF_DOMAIN_RZ && mesh->domain != SF_DOMAIN_ZR)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_setup: unknown domain type %d", mesh->domain);
+    if (nseg > 0 && (!seg_xy || !seg_btype || !seg_absorb || !node_seg_idx))
+        return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_setup: segment arrays missing");
+    if (ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "sf_mesh_setup: mesh already set");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    const int ni = mesh->ni, nj = mesh->nj;
+    const size_t nn = (size_t)ni * nj;
+    const int nface = 2 * ni + 2 * nj;
+    ctx->mdesc = *mesh;
+    MeshP& m = ctx->mp;
+    m.ni = ni;
+    m.nj = nj;
+    m.nci = ni - 1;
+    m.ncj = nj - 1;
+    m.domain = mesh->domain;
+    m.ntx = (m.nci + TILE_W - 1) / TILE_W;
+    m.nty = (m.ncj + TILE_H - 1) / TILE_H;
+    for (int d = 0; d < 2; d++) {
+        m.x0[d] = mesh->x0[d];
+        m.dh[d] = mesh->dh[d];
+        m.inv_dh[d] = 1.0 / mesh->dh[d];
+    }
+    ctx->h_face_type.assign(face_type, face_type + nface);
+    ctx->h_face_value.assign(nface, 0.0);
+    if (face_value) ctx->h_face_value.assign(face_value, face_value + nface);
+    ctx->h_node_type.assign(node_type, node_type + nn);
+    ctx->h_node_bc.assign(nn, 0.0);
+    if (node_bc) ctx->h_node_bc.assign(node_bc, node_bc + nn);
+    ctx->h_node_vol.assign(node_vol, node_vol + nn);
+    for (size_t u = 0; u < nn; u++)
+        if (!(node_vol[u] > 0) || !std::isfinite(node_vol[u]))
+            return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_setup: node volume %zu is not positive", u);
+    // device copies
+    std::vector<int8_t> face8(nface);
+    for (int k = 0; k < nface; k++) face8[k] = (int8_t)face_type[k];
+    // node segment lists restricted to segments the mover tests (DIRICHLET / SINK)
+    std::vector<int32_t> ptr(nn + 1, 0), idx;
+    std::vector<uint8_t> has(nn, 0);
+    for (size_t u = 0; u < nn; u++) {
+        for (int q = node_seg_ptr[u]; q < node_seg_ptr[u + 1]; q++) {
+            int s = node_seg_idx[q];
+            if (s < 0 || s >= nseg) return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_setup: segment index out of range");
+            if (seg_btype[s] == SF_BND_DIRICHLET || seg_btype[s] == SF_BND_SINK) idx.push_back(s);
+        }
+        ptr[u + 1] = (int32_t)idx.size();
+        has[u] = ptr[u + 1] > ptr[u];
+    }
+    std::vector<DevSeg> segs(nseg);
+    for (int s = 0; s < nseg; s++) {
+        DevSeg& g = segs[s];
+        g.x1[0] = seg_xy[4 * s];
+        g.x1[1] = seg_xy[4 * s + 1];
+        g.x2[0] = seg_xy[4 * s + 2];
+        g.x2[1] = seg_xy[4 * s + 3];
+        double dx = g.x2[0] - g.x1[0], dy = g.x2[1] - g.x1[1];
+        double len = std::sqrt(dx * dx + dy * dy);
+        if (!(len > 0)) return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_setup: zero-length segment %d", s);
+        dx /= len;
+        dy /= len;
+        g.nrm[0] = -dy;            // LinearSegment normal (LinearSegment.java:41-44)
+        g.nrm[1] = dx;
+        g.btype = seg_btype[s];
+        g.absorb = seg_absorb[s];
+    }
+    TRY(dalloc(ctx, &ctx->d_face, nface));
+    TRY(dalloc(ctx, &ctx->d_nseg_ptr, nn + 1));
+    TRY(dalloc(ctx, &ctx->d_nseg_idx, idx.size()));
+    TRY(dalloc(ctx, &ctx->d_seg, (size_t)nseg));
+    TRY(dalloc(ctx, &ctx->d_hasseg, nn));
+    TRY(dalloc(ctx, &ctx->d_node_vol, nn));
+    TRY(dalloc(ctx, &ctx->d_phi, nn));
+    TRY(dalloc(ctx, &ctx->d_rho, nn));
+    TRY(dalloc(ctx, &ctx->d_efi, nn));
+    TRY(dalloc(ctx, &ctx->d_efj, nn));
+    TRY(dalloc(ctx, &ctx->d_ef32, nn));
+    SF_HIP(ctx, hipMemcpy(ctx->d_face, face8.data(), nface, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(ctx->d_nseg_ptr, ptr.data(), (nn + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!idx.empty())
+        SF_HIP(ctx, hipMemcpy(ctx->d_nseg_idx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (nseg > 0) SF_HIP(ctx, hipMemcpy(ctx->d_seg, segs.data(), nseg * sizeof(DevSeg), hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(ctx->d_hasseg, has.data(), nn, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(ctx->d_node_vol, node_vol, nn * sizeof(double), hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemset(ctx->d_phi, 0, nn * sizeof(double)));
+    SF_HIP(ctx, hipMemset(ctx->d_rho, 0, nn * sizeof(double)));
+    SF_HIP(ctx, hipMemset(ctx->d_efi, 0, nn * sizeof(double)));
+    SF_HIP(ctx, hipMemset(ctx->d_efj, 0, nn * sizeof(double)));
+    SF_HIP(ctx, hipMemset(ctx->d_ef32, 0, nn * sizeof(float2)));
+    SF_HIP(ctx, hipDeviceSynchronize());
+    ctx->mesh_ready = true;
+    return SF_OK;
+}
+
+int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species) {
+    SF_CHECK_CTX(ctx);
+    if (!desc || !species) return sf_fail(ctx, SF_ERR_ARG, "sf_species_create: null argument");
+    if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "sf_species_create: mesh not set up");
+    if (!(desc->molwt > 0)) return sf_fail(ctx, SF_ERR_ARG, "sf_species_create: molwt must be > 0");
+    if (!(desc->spwt > 0)) return sf_fail(ctx, SF_ERR_ARG, "sf_species_create: spwt must be > 0");
+    if (desc->capacity < 1 || desc->capacity > (int64_t)2000000000)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_species_create: capacity must be in [1, 2e9]");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    Species* sp = new Species();
+    sp->desc = *desc;
+    sp->charge = desc->charge * QE;       // Material.java:64-65
+    sp->mass = desc->molwt * AMU;
+    sp->qm = sp->charge / sp->mass;       // Material.init (Material.java:711)
+    sp->capacity = desc->capacity;
+    double vmax = desc->vmax > 0 ? desc->vmax : 1e5;
+    // fixed-point scales: a particle contributes at most ~2^40 per channel
+    int kv = (int)std::floor(std::log2(std::ldexp(1.0, 40) / vmax));
+    int kuu = (int)std::floor(std::log2(std::ldexp(1.0, 40) / (vmax * vmax)));
+    sp->kscale[0] = std::ldexp(1.0, 40);
+    for (int c = 1; c <= 3; c++) sp->kscale[c] = std::ldexp(1.0, kv);
+    for (int c = 4; c <= 6; c++) sp->kscale[c] = std::ldexp(1.0, kuu);
+    const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj;
+    const size_t ncells = (size_t)ctx->mp.nci * ctx->mp.ncj;
+    const size_t cap = (size_t)desc->capacity;
+    int rc = 0;
+    for (int b = 0; b < 2 && !rc; b++) {
+        rc = dalloc(ctx, &sp->buf[b].cell, cap);
+        if (!rc) rc = dalloc(ctx, &sp->buf[b].fx, cap);
+        if (!rc) rc = dalloc(ctx, &sp->buf[b].fy, cap);
+        if (!rc) rc = dalloc(ctx, &sp->buf[b].vx, cap);
+        if (!rc) rc = dalloc(ctx, &sp->buf[b].vy, cap);
+        if (!rc) rc = dalloc(ctx, &sp->buf[b].vz, cap);
+        if (!rc && desc->track_ids) rc = dalloc(ctx, &sp->buf[b].id, cap);
+    }
+    if (!rc) rc = dalloc(ctx, &sp->key, cap);
+    if (!rc) rc = dalloc(ctx, &sp->acc, NCH * nn);
+    if (!rc) rc = dalloc(ctx, &sp->sums, NCH * nn);
+    if (!rc) rc = dalloc(ctx, &sp->mpc_sum, ncells);
+    if (!rc) rc = dalloc(ctx, &sp->den, 4 * nn);
+    if (!rc) rc = dalloc(ctx, &sp->counts, ncells);
+    if (!rc) rc = dalloc(ctx, &sp->offsets, ncells + 1);
+    if (!rc) rc = dalloc(ctx, &sp->cursor, ncells);
+    if (!rc) rc = dalloc(ctx, &sp->blocksum, ncells / 4096 + 2);
+    if (!rc) rc = dalloc(ctx, &sp->stats, 8);
+    if (rc) {
+        free_species(sp);
+        return rc;
+    }
+    hipMemset(sp->acc, 0, NCH * nn * sizeof(unsigned long long));
+    hipMemset(sp->sums, 0, NCH * nn * sizeof(long long));
+    hipMemset(sp->mpc_sum, 0, ncells * sizeof(long long));
+    hipMemset(sp->counts, 0, ncells * sizeof(int32_t));
+    hipMemset(sp->offsets, 0, (ncells + 1) * sizeof(int32_t));
+    // Material.init: nd starts at nd_back, u = v = w = 0
+    std::vector<double> init(4 * nn, 0.0);
+    for (size_t u = 0; u < nn; u++) init[u] = desc->nd_back;
+    hipMemcpy(sp->den, init.data(), 4 * nn * sizeof(double), hipMemcpyHostToDevice);
+    SF_HIP(ctx, hipDeviceSynchronize());
+    ctx->species.push_back(sp);
+    *species = (int)ctx->species.size() - 1;
+    return SF_OK;
+}
+
+int sf_particles_add(sf_ctx* ctx, int species, int64_t n, const double* x, const double* y, const double* u,
+                     const double* v, const double* w, const int* ids, int mode) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp) return sf_fail(ctx, SF_ERR_ARG, "unknown species %d", species);
+    if (n < 0 || (n > 0 && (!x || !y || !u || !v || !w))) return sf_fail(ctx, SF_ERR_ARG, "sf_particles_add: bad arrays");
+    if (mode != 0 && mode != 1) return sf_fail(ctx, SF_ERR_ARG, "sf_particles_add: mode must be 0 or 1");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    if (mode == 1) TRY(ef32_refresh(ctx));
+    return add_particles_device(ctx, sp, n, x, y, u, v, w, ids, mode);
+}
+
+int sf_particles_count(sf_ctx* ctx, int species, int64_t* n) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp || !n) return sf_fail(ctx, SF_ERR_ARG, "sf_particles_count: bad argument");
+    *n = sp->np;
+    return SF_OK;
+}
+
+int sf_particles_download(sf_ctx* ctx, int species, double* x, double* y, double* lc0, double* lc1, double* u,
+                          double* v, double* w, int* ids) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp) return sf_fail(ctx, SF_ERR_ARG, "unknown species %d", species);
+    if (ids && !sp->desc.track_ids) return sf_fail(ctx, SF_ERR_STATE, "species does not track ids");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t n = sp->np;
+    if (n == 0) return SF_OK;
+    const PState& s = sp->buf[sp->cur];
+    std::vector<int32_t> cell(n), id;
+    std::vector<float> fx(n), fy(n), vx(n), vy(n), vz(n);
+    SF_HIP(ctx, hipMemcpy(cell.data(), s.cell, n * 4, hipMemcpyDeviceToHost));
+    SF_HIP(ctx, hipMemcpy(fx.data(), s.fx, n * 4, hipMemcpyDeviceToHost));
+    SF_HIP(ctx, hipMemcpy(fy.data(), s.fy, n * 4, hipMemcpyDeviceToHost));
+    SF_HIP(ctx, hipMemcpy(vx.data(), s.vx, n * 4, hipMemcpyDeviceToHost));
+    SF_HIP(ctx, hipMemcpy(vy.data(), s.vy, n * 4, hipMemcpyDeviceToHost));
+    SF_HIP(ctx, hipMemcpy(vz.data(), s.vz, n * 4, hipMemcpyDeviceToHost));
+    if (ids) {
+        id.resize(n);
+        SF_HIP(ctx, hipMemcpy(id.data(), s.id, n * 4, hipMemcpyDeviceToHost));
+    }
+    const MeshP& m = ctx->mp;
+    for (int64_t k = 0; k < n; k++) {
+        double l0 = (cell[k] & 0xFFFF) + (double)fx[k];
+        double l1 = ((cell[k] >> 16) & 0xFFFF) + (double)fy[k];
+        if (x) x[k] = m.x0[0] + l0 * m.dh[0];
+        if (y) y[k] = m.x0[1] + l1 * m.dh[1];
+        if (lc0) lc0[k] = l0;
+        if (lc1) lc1[k] = l1;
+        if (u) u[k] = vx[k];
+        if (v) v[k] = vy[k];
+        if (w) w[k] = vz[k];
+        if (ids) ids[k] = id[k];
+    }
+    return SF_OK;
+}
+
+int sf_particles_preload(sf_ctx* ctx, int species, int64_t n, const double* drift3, double temp, uint64_t seed) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp || !drift3 || n < 0 || !(temp >= 0)) return sf_fail(ctx, SF_ERR_ARG, "sf_particles_preload: bad argument");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    TRY(preload(ctx, sp, n, drift3, temp, seed));
+    return sort_species(ctx, sp);
+}
+
+int sf_push_deposit(sf_ctx* ctx, int species) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp) return sf_fail(ctx, SF_ERR_ARG, "unknown species %d", species);
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    TRY(ef32_refresh(ctx));
+    return push_deposit(ctx, sp);
+}
+
+int sf_species_clear_samples(sf_ctx* ctx, int species) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp) return sf_fail(ctx, SF_ERR_ARG, "unknown species %d", species);
+    const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj, ncells = (size_t)ctx->mp.nci * ctx->mp.ncj;
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipMemsetAsync(sp->sums, 0, NCH * nn * sizeof(long long), ctx->stream));
+    SF_HIP(ctx, hipMemsetAsync(sp->mpc_sum, 0, ncells * sizeof(long long), ctx->stream));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    sp->num_samples = 0;
+    return SF_OK;
+}
+
+int sf_species_stats(sf_ctx* ctx, int species, double* out5) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp || !out5) return sf_fail(ctx, SF_ERR_ARG, "sf_species_stats: bad argument");
+    for (int k = 0; k < 5; k++) out5[k] = sp->stats_host[k];
+    return SF_OK;
+}
+
+int sf_species_field(sf_ctx* ctx, int species, int which, double* out) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp || !out) return sf_fail(ctx, SF_ERR_ARG, "sf_species_field: bad argument");
+    if (which < 0 || which > 21) return sf_fail(ctx, SF_ERR_ARG, "sf_species_field: unknown field %d", which);
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    return species_field(ctx, sp, which, out);
+}
+
+int sf_last_timings(sf_ctx* ctx, double* out4) {
+    SF_CHECK_CTX(ctx);
+    if (!out4) return SF_ERR_ARG;
+    for (int k = 0; k < 4; k++) out4[k] = ctx->last_ms[k];
+    return SF_OK;
+}
+
+int sf_set_timing(sf_ctx* ctx, int enabled) {
+    SF_CHECK_CTX(ctx);
+    ctx->timing = enabled != 0;
+    return SF_OK;
+}
+
+}  // extern "C"
+
+namespace sf {
+
+// Downloads instantaneous fields, moment sums, and the KineticMaterial.computeFields
+// derived fields (KineticMaterial.java:1606-1671), evaluated in FP64 on the host.
+int species_field(sf_ctx* ctx, Species* sp, int which, double* out) {
+    const MeshP& m = ctx->mp;
+    const size_t nn = (size_t)m.ni * m.nj;
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (which <= 3) {
+        SF_HIP(ctx, hipMemcpy(out, sp->den + which * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
+        return SF_OK;
+    }
+    std::vector<long long> sums(NCH * nn);
+    SF_HIP(ctx, hipMemcpy(sums.data(), sp->sums, NCH * nn * sizeof(long long), hipMemcpyDeviceToHost));
+    const double spwt = sp->desc.spwt;
+    auto sumf = [&](int ch, size_t u) { return (double)sums[ch * nn + u] / sp->kscale[ch] * spwt; };
+    if (which >= 4 && which <= 10) {
+        // count-sum, u-sum, v-sum, w-sum, uu-sum, vv-sum, ww-sum
+        static const int chmap[7] = {0, 1, 2, 3, 4, 5, 6};
+        int ch = chmap[which - 4];
+        for (size_t u = 0; u < nn; u++) out[u] = sumf(ch, u);
+        return SF_OK;
+    }
+    const size_t ncells = (size_t)m.nci * m.ncj;
+    std::vector<long long> mpc(ncells);
+    SF_HIP(ctx, hipMemcpy(mpc.data(), sp->mpc_sum, ncells * sizeof(long long), hipMemcpyDeviceToHost));
+    if (which == 11 || which == 21) {
+        // mpc-sum / mpc are cell data stored on node (i,j) of the cell
+        double f = which == 21 ? (sp->num_samples > 0 ? 1.0 / sp->num_samples : 0.0) : 1.0;
+        for (size_t u = 0; u < nn; u++) out[u] = 0;
+        for (int j = 0; j < m.ncj; j++)
+            for (int i = 0; i < m.nci; i++) out[(size_t)j * m.ni + i] = (double)mpc[(size_t)j * m.nci + i] * f;
+        return SF_OK;
+    }
+    const double inv_ns = sp->num_samples > 0 ? 1.0 / sp->num_samples : 0.0;
+    for (size_t u = 0; u < nn; u++) {
+        double count = sumf(0, u);
+        double T = 0, T1 = 0, T2 = 0, T3 = 0;
+        double uu_m = 0, vv_m = 0, ww_m = 0;
+        if (count > 0) {
+            double uu = sumf(1, u) / count, vv = sumf(2, u) / count, ww = sumf(3, u) / count;
+            double uu2 = sumf(4, u) / count - uu * uu;
+            double vv2 = sumf(5, u) / count - vv * vv;
+            double ww2 = sumf(6, u) / count - ww * ww;
+            double f = sp->mass / K_B;
+            T = (uu2 + vv2 + ww2) * f / 3.0;
+            T1 = uu2 * f;
+            T2 = vv2 * f;
+            T3 = ww2 * f;
+            uu_m = uu;
+            vv_m = vv;
+            ww_m = ww;
+        }
+        double nd_ave = count * inv_ns / ctx->h_node_vol[u];
+        switch (which) {
+            case 12: out[u] = T; break;
+            case 13: out[u] = T1; break;
+            case 14: out[u] = T2; break;
+            case 15: out[u] = T3; break;
+            case 16: out[u] = nd_ave; break;
+            case 17: out[u] = uu_m; break;
+            case 18: out[u] = vv_m; break;
+            case 19: out[u] = ww_m; break;
+            case 20: out[u] = nd_ave * K_B * T; break;
+            default: out[u] = 0;
+        }
+    }
+    return SF_OK;
+}
+
+}  // namespace sf

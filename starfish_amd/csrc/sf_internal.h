@@ -1,0 +1,202 @@
+// Internal declarations of libstarfish_hip.so (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/starfish_hip.h"
+
+namespace sf {
+
+// Physical constants (core/common/Constants.java:20-33).
+constexpr double EPS0 = 8.85418782e-12;
+constexpr double K_B = 1.3806503e-23;
+constexpr double QE = 1.60217646e-19;
+constexpr double AMU = 1.66053886e-27;
+constexpr double FLT_EPS = 1e-7;
+constexpr double PI = 3.14159265358979323846;
+
+// Tile of cells owned by one workgroup of the push/deposit pass.
+constexpr int TILE_W = 32;                      // cells along i
+constexpr int TILE_H = 8;                       // cells along j
+constexpr int TILE_NW = TILE_W + 3;             // nodes along i incl. one-cell halo on both sides
+constexpr int TILE_NH = TILE_H + 3;
+constexpr int TILE_NODES = TILE_NW * TILE_NH;
+constexpr int TILE_CW = TILE_W + 2;             // cells incl. halo
+constexpr int TILE_CH = TILE_H + 2;
+constexpr int TILE_CELLS = TILE_CW * TILE_CH;
+constexpr int NCH = 7;                          // deposit channels: n, nu, nv, nw, nuu, nvv, nww
+constexpr int PUSH_THREADS = 256;
+constexpr int CHUNK = 1024;                     // particles staged in LDS per round
+constexpr int CHUNK_STRIDE = 37;                // odd stride of the lane->particle permutation
+
+// Particle state, structure of arrays, one buffer.
+struct PState {
+    int32_t* cell;   // packed (j << 16) | i of the cell, -1 never stored
+    float* fx;       // offset inside the cell along i (lc0 = i + fx)
+    float* fy;       // offset along j
+    float* vx;
+    float* vy;
+    float* vz;
+    int32_t* id;     // optional
+};
+
+// Mesh parameters passed by value to kernels.
+struct MeshP {
+    int ni, nj, nci, ncj;
+    int domain;
+    int ntx, nty;     // tiles
+    double x0[2], dh[2], inv_dh[2];
+};
+
+// Segment record for boundary processing (LinearSegment.java).
+struct DevSeg {
+    double x1[2], x2[2];
+    double nrm[2];
+    int btype;
+    int absorb;
+};
+
+// Device pointers of the mesh.
+struct MeshD {
+    const int8_t* face;        // [2nj+2ni] face types (RIGHT, TOP, LEFT, BOTTOM)
+    const int32_t* nseg_ptr;   // [ni*nj+1] node segment lists (DIRICHLET/SINK only)
+    const int32_t* nseg_idx;
+    const DevSeg* seg;
+    const uint8_t* node_hasseg;// [ni*nj] 1 if the node lists a DIRICHLET/SINK segment
+    const float2* ef;          // [ni*nj] E field (i, j components)
+};
+
+// Per-species push constants.
+struct PushC {
+    double qm_dt;          // q/m * dt
+    double dt;
+    double kscale[NCH];    // fixed-point scales 2^k per channel
+    double spwt;
+};
+
+struct Species {
+    sf_species_desc desc;
+    double charge, mass, qm;
+    int64_t capacity = 0;
+    int64_t np = 0, nsorted = 0;
+    int cur = 0;
+    PState buf[2]{};
+    int32_t* key = nullptr;        // new cell key per particle (-1 dead)
+    unsigned long long* acc = nullptr;   // [NCH][nn] int64 deposit accumulators
+    long long* sums = nullptr;     // [NCH][nn] running moment sums (int64)
+    long long* mpc_sum = nullptr;  // [ncells]
+    double* den = nullptr;         // [4][nn] nd, u, v, w
+    int32_t* counts = nullptr;     // [ncells] histogram of new cells
+    int32_t* offsets = nullptr;    // [ncells+1] exclusive scan (cell ranges of the sorted array)
+    int32_t* cursor = nullptr;     // [ncells]
+    int32_t* blocksum = nullptr;   // scan scratch
+    long long* stats = nullptr;    // [8] int64 fixed-point mover sums
+    int id_counter = 0;
+    int num_samples = 0;
+    double kscale[NCH];
+    double stats_host[5] = {0, 0, 0, 0, 0};
+};
+
+// Host-side java.util.Random (JDK-specified 48-bit LCG), the shared Starfish generator
+// (core/common/Starfish.java:238-295).
+struct JavaRandom {
+    uint64_t seed = 0;
+    void set_seed(int64_t s) { seed = ((uint64_t)s ^ 0x5DEECE66DULL) & ((1ULL << 48) - 1); }
+    int32_t next(int bits) {
+        seed = (seed * 0x5DEECE66DULL + 0xBULL) & ((1ULL << 48) - 1);
+        return (int32_t)(uint32_t)(seed >> (48 - bits));
+    }
+    double next_double() { return (double)(((int64_t)next(26) << 27) + next(27)) * 0x1.0p-53; }
+    double rnd2() { return -1.0 + 2 * next_double(); }
+};
+
+struct HostSource {
+    int species;
+    std::vector<double> seg;       // x1,y1,x2,y2 per linear segment
+    std::vector<double> cum;
+    double area = 0, mdot = 0, v_drift = 0, v_th = 0, rem = 0;
+};
+
+struct PoissonState;
+void poisson_free(PoissonState* p);
+
+}  // namespace sf
+
+struct sf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double dt = 1e-7;
+    sf::JavaRandom rng;
+    bool mesh_ready = false;
+    sf_mesh_desc mdesc{};
+    sf::MeshP mp{};
+    // host copies
+    std::vector<int> h_face_type;
+    std::vector<double> h_face_value;
+    std::vector<int> h_node_type;
+    std::vector<double> h_node_bc, h_node_vol;
+    // device mesh
+    int8_t* d_face = nullptr;
+    int32_t* d_nseg_ptr = nullptr;
+    int32_t* d_nseg_idx = nullptr;
+    sf::DevSeg* d_seg = nullptr;
+    uint8_t* d_hasseg = nullptr;
+    double* d_node_vol = nullptr;
+    // fields
+    double* d_phi = nullptr;
+    double* d_rho = nullptr;
+    double* d_efi = nullptr;
+    double* d_efj = nullptr;
+    float2* d_ef32 = nullptr;
+    std::vector<sf::Species*> species;
+    std::vector<sf::HostSource> sources;
+    sf::PoissonState* poisson = nullptr;
+    bool timing = false;
+    hipEvent_t ev[6]{};
+    double last_ms[4] = {0, 0, 0, 0};
+    sf::MeshD meshd() const {
+        sf::MeshD m;
+        m.face = d_face;
+        m.nseg_ptr = d_nseg_ptr;
+        m.nseg_idx = d_nseg_idx;
+        m.seg = d_seg;
+        m.node_hasseg = d_hasseg;
+        m.ef = d_ef32;
+        return m;
+    }
+};
+
+// error helpers
+int sf_fail(sf_ctx* ctx, int code, const char* fmt, ...);
+#define SF_HIP(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return sf_fail((ctx), SF_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                                    \
+    } while (0)
+#define SF_CHECK_CTX(ctx)                          \
+    do {                                           \
+        if (!(ctx)) return SF_ERR_ARG;             \
+    } while (0)
+
+namespace sf {
+int species_alloc_buffers(sf_ctx* ctx, Species* sp);
+int push_deposit(sf_ctx* ctx, Species* sp);
+int sort_species(sf_ctx* ctx, Species* sp);
+int add_particles_device(sf_ctx* ctx, Species* sp, int64_t n, const double* x, const double* y, const double* u,
+                         const double* v, const double* w, const int* ids, int mode);
+int species_field(sf_ctx* ctx, Species* sp, int which, double* out);
+int efield_update(sf_ctx* ctx);
+int ef32_refresh(sf_ctx* ctx);
+int rho_update(sf_ctx* ctx);
+int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* d);
+int poisson_solve(sf_ctx* ctx, sf_solve_info* info);
+int operator_apply(sf_ctx* ctx, const double* x, double* y);
+int preload(sf_ctx* ctx, Species* sp, int64_t n, const double* drift, double temp, uint64_t seed);
+}  // namespace sf

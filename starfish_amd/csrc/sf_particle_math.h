@@ -1,0 +1,243 @@
+// Device-side particle mechanics shared by the push kernels: field gather, leapfrog push,
+// XY drift / RZ rotation and boundary processing.  Computation is FP64; particle state is
+// stored as a packed cell index plus FP32 offsets and FP32 velocities.
+#pragma once
+#include "sf_internal.h"
+
+namespace sf {
+
+// Field2D.gather / gather_safe (core/domain/Field2D.java:300-390) on the FP32 E field.
+// The reference evaluates the bilinear form and, when the Java array access goes out of
+// bounds, falls back to gather_safe (clamped corner, zero weight beyond the edge).
+template <class Load>
+__device__ __forceinline__ void gather_e(const MeshP& m, double fi, double fj, Load load, double& ex, double& ey) {
+    int i = (int)fi, j = (int)fj;
+    double di = fi - i, dj = fj - j;
+    if (i < 0 || j < 0 || i + 1 >= m.ni || j + 1 >= m.nj) {
+        if (i < 0) { i = 0; di = 0; }
+        if (j < 0) { j = 0; dj = 0; }
+        if (i >= m.ni - 1) { i = m.ni - 1; di = 0; }
+        if (j >= m.nj - 1) { j = m.nj - 1; dj = 0; }
+        float2 e00 = load(i, j);
+        double w00 = (1 - di) * (1 - dj);
+        ex = w00 * e00.x;
+        ey = w00 * e00.y;
+        if (di > 0) {
+            float2 e = load(i + 1, j);
+            ex += di * (1 - dj) * e.x;
+            ey += di * (1 - dj) * e.y;
+        }
+        if (di > 0 && dj > 0) {
+            float2 e = load(i + 1, j + 1);
+            ex += di * dj * e.x;
+            ey += di * dj * e.y;
+        }
+        if (dj > 0) {
+            float2 e = load(i, j + 1);
+            ex += (1 - di) * dj * e.x;
+            ey += (1 - di) * dj * e.y;
+        }
+        return;
+    }
+    float2 e00 = load(i, j), e10 = load(i + 1, j), e11 = load(i + 1, j + 1), e01 = load(i, j + 1);
+    ex = (1 - di) * (1 - dj) * e00.x;
+    ex += di * (1 - dj) * e10.x;
+    ex += di * dj * e11.x;
+    ex += (1 - di) * dj * e01.x;
+    ey = (1 - di) * (1 - dj) * e00.y;
+    ey += di * (1 - dj) * e10.y;
+    ey += di * dj * e11.y;
+    ey += (1 - di) * dj * e01.y;
+}
+
+__device__ __forceinline__ int face_type(const MeshP& m, const MeshD& md, int face, int idx) {
+    // face layout RIGHT[nj], TOP[ni], LEFT[nj], BOTTOM[ni]
+    int off = 0;
+    if (face == 1) off = m.nj;
+    else if (face == 2) off = m.nj + m.ni;
+    else if (face == 3) off = 2 * m.nj + m.ni;
+    return (int)md.face[off + idx];
+}
+
+// LinearSegment.intersect(p3,p4) (core/boundaries/LinearSegment.java:113-160); returns t along
+// the particle path in t1 and along the segment in t0, or -1.
+__device__ __forceinline__ void seg_intersect(const DevSeg& s, const double* p3, const double* p4, double& t0,
+                                              double& t1) {
+    const double* p1 = s.x1;
+    const double* p2 = s.x2;
+    double X1 = p1[0], X2 = p2[0], X3 = p3[0], X4 = p4[0];
+    double Y1 = p1[1], Y2 = p2[1], Y3 = p3[1], Y4 = p4[1];
+    double den = (X1 - X2) * (Y3 - Y4) - (Y1 - Y2) * (X3 - X4);
+    if (den == 0) { t0 = -1; t1 = -1; return; }
+    double xp0 = ((X1 * Y2 - Y1 * X2) * (X3 - X4) - (X1 - X2) * (X3 * Y4 - Y3 * X4)) / den;
+    double xp1 = ((X1 * Y2 - Y1 * X2) * (Y3 - Y4) - (Y1 - Y2) * (X3 * Y4 - Y3 * X4)) / den;
+    if (fabs(p2[0] - p1[0]) > 1e-6) t0 = (xp0 - p1[0]) / (p2[0] - p1[0]);
+    else t0 = (xp1 - p1[1]) / (p2[1] - p1[1]);
+    if (t0 < -FLT_EPS || t0 > (1 + FLT_EPS)) { t0 = -1; t1 = -1; return; }
+    if (fabs(p4[0] - p3[0]) > 1e-6) t1 = (xp0 - p3[0]) / (p4[0] - p3[0]);
+    else t1 = (xp1 - p3[1]) / (p4[1] - p3[1]);
+    if (t1 < -FLT_EPS || t1 > (1 + FLT_EPS)) { t0 = -1; t1 = -1; return; }
+    if (t0 < 0) t0 = 0;
+    if (t1 < 0) t1 = 0;
+    if (t0 > 1) t0 = 1;
+    if (t1 > 1) t1 = 1;
+}
+
+// KineticMaterial.ProcessBoundary (core/materials/KineticMaterial.java:471-750) in logical
+// coordinates.  lc/v are updated in place; dtr receives the remaining time.  Returns alive.
+__device__ __noinline__ bool process_boundary(const MeshP& m, const MeshD& md, double& lc0, double& lc1, double o0,
+                                              double o1, double& v0, double& v1, double& v2, double& dtr) {
+    const int ni = m.ni, nj = m.nj;
+    double dt0 = dtr;
+    dtr = 0;
+    int i_min = (int)fmin(lc0, o0), i_max = (int)fmax(lc0, o0);
+    int j_min = (int)fmin(lc1, o1), j_max = (int)fmax(lc1, o1);
+    if (i_min < 0) i_min = 0;
+    if (j_min < 0) j_min = 0;
+    if (i_max >= ni) i_max = ni - 1;
+    if (j_max >= nj) j_max = nj - 1;
+    bool any = false;
+    for (int i = i_min; i <= i_max && !any; i++)
+        for (int j = j_min; j <= j_max; j++)
+            if (md.node_hasseg[j * ni + i]) { any = true; break; }
+    if (any) {
+        double oldp[2] = {m.x0[0] + o0 * m.dh[0], m.x0[1] + o1 * m.dh[1]};
+        double newp[2] = {m.x0[0] + lc0 * m.dh[0], m.x0[1] + lc1 * m.dh[1]};
+        double tp_min = 2.0;
+        int seg_min = -1;
+        double vmag = sqrt(v0 * v0 + v1 * v1);
+        for (int i = i_min; i <= i_max; i++)
+            for (int j = j_min; j <= j_max; j++) {
+                int n = j * ni + i;
+                for (int q = md.nseg_ptr[n]; q < md.nseg_ptr[n + 1]; q++) {
+                    int s = md.nseg_idx[q];
+                    const DevSeg sg = md.seg[s];
+                    double t0, t1;
+                    seg_intersect(sg, oldp, newp, t0, t1);
+                    if (t1 > 0) {
+                        double acos_ = (sg.nrm[0] * v0 + sg.nrm[1] * v1) / vmag;
+                        if (t1 < FLT_EPS && (acos_ > 0 || sg.btype == SF_BND_VIRTUAL)) continue;
+                        if (t1 < tp_min) { tp_min = t1; seg_min = s; }
+                    }
+                }
+            }
+        if (seg_min >= 0) {
+            tp_min *= 0.9999;
+            double px = oldp[0] + tp_min * (newp[0] - oldp[0]);
+            double py = oldp[1] + tp_min * (newp[1] - oldp[1]);
+            lc0 = (px - m.x0[0]) * m.inv_dh[0];
+            lc1 = (py - m.x0[1]) * m.inv_dh[1];
+            dtr = dt0 * (1 - tp_min);
+            if (lc0 < 0 && lc0 > -FLT_EPS) lc0 = 0;
+            if (lc1 < 0 && lc1 > -FLT_EPS) lc1 = 0;
+            const DevSeg sg = md.seg[seg_min];
+            // default surface interaction of a boundary with a material absorbs; SINK always does
+            if (sg.absorb || sg.btype == SF_BND_SINK) return false;
+        }
+    }
+    if (lc0 < 0 || lc1 < 0 || lc0 >= ni - 1 || lc1 >= nj - 1) {
+        double t_right = 99, t_top = 99, t_left = 99, t_bottom = 99;
+        if (lc0 >= ni - 1) t_right = (ni - 1.0 - o0) / (lc0 - o0);
+        if (lc1 >= nj - 1) t_top = (nj - 1.0 - o1) / (lc1 - o1);
+        if (lc0 < 0) t_left = o0 / (o0 - lc0);
+        if (lc1 < 0) t_bottom = o1 / (o1 - lc1);
+        int face = 0;  // RIGHT
+        double t = t_right;
+        if (t_top < t) { face = 1; t = t_top; }
+        if (t_left < t) { face = 2; t = t_left; }
+        if (t_bottom < t) { face = 3; t = t_bottom; }
+        lc0 = o0 + t * (lc0 - o0);
+        lc1 = o1 + t * (lc1 - o1);
+        if (lc0 < 0) lc0 = 0;
+        else if (lc0 > ni - 1) lc0 = ni - 1;
+        if (lc1 < 0) lc1 = 0;
+        else if (lc1 > nj - 1) lc1 = nj - 1;
+        dtr = dt0 * (1 - t);
+        int i = (int)lc0, j = (int)lc1;
+        if (face == 1) j++;
+        if (face == 0) i++;
+        if (i < 0) i = 0;
+        if (j < 0) j = 0;
+        if (i >= ni - 1) i = ni - 1;
+        if (j >= nj - 1) j = nj - 1;
+        int type = (face == 2 || face == 0) ? face_type(m, md, face, j) : face_type(m, md, face, i);
+        switch (type) {
+            case SF_FACE_SYMMETRY:
+                // Vec.mirror about the inward face normal flips the normal velocity component
+                if (face == 0 || face == 2) v0 = -v0;
+                else v1 = -v1;
+                return true;
+            case SF_FACE_PERIODIC:
+                // the reference shifts the physical position; here the logical coordinate
+                if (face == 2) lc0 += (ni - 1);
+                else if (face == 0) lc0 -= (ni - 1);
+                else if (face == 3) lc1 += (nj - 1);
+                else lc1 -= (nj - 1);
+                return true;
+            default:
+                return false;  // OPEN, DIRICHLET, NEUMANN, MESH without neighbour, SINK, CIRCUIT (ions)
+        }
+    }
+    return true;
+}
+
+// One particle through ParticleMover.run (KineticMaterial.java:299-422), B = 0 branch.
+// Inputs: logical coordinate and velocity; outputs updated in place.  Returns alive.
+template <class Load>
+__device__ __forceinline__ bool push_particle(const MeshP& m, const MeshD& md, const PushC& pc, Load load,
+                                              double& lc0, double& lc1, double& v0, double& v1, double& v2) {
+    double ex, ey;
+    gather_e(m, lc0, lc1, load, ex, ey);
+    v0 += pc.qm_dt * ex;
+    v1 += pc.qm_dt * ey;
+    double dtr = pc.dt;
+    int bounces = 0;
+    bool alive = true;
+    while (dtr > 0 && bounces++ < 10) {
+        double o0 = lc0, o1 = lc1;
+        if (m.domain == SF_DOMAIN_RZ) {
+            // rotateToRZ (KineticMaterial.java:424-442): r after the planar move, rotated by w*dt
+            double xr = m.x0[0] + lc0 * m.dh[0] + v0 * dtr;
+            lc1 += v1 * dtr * m.inv_dh[1];
+            double A = v2 * dtr;
+            double R = sqrt(A * A + xr * xr);
+            double c = xr / R, s = A / R;
+            double nv0 = c * v0 + s * v2;
+            double nv2 = -s * v0 + c * v2;
+            v0 = nv0;
+            v2 = nv2;
+            lc0 = (R - m.x0[0]) * m.inv_dh[0];
+        } else if (m.domain == SF_DOMAIN_ZR) {
+            // rotateToZR (KineticMaterial.java:444-462)
+            double yr = m.x0[1] + lc1 * m.dh[1] + v1 * dtr;
+            lc0 += v0 * dtr * m.inv_dh[0];
+            double A = v2 * dtr;
+            double R = sqrt(A * A + yr * yr);
+            double c = yr / R, s = A / R;
+            double nv1 = c * v1 + s * v2;
+            double nv2 = -s * v1 + c * v2;
+            v1 = nv1;
+            v2 = nv2;
+            lc1 = (R - m.x0[1]) * m.inv_dh[1];
+        } else {
+            lc0 += v0 * dtr * m.inv_dh[0];
+            lc1 += v1 * dtr * m.inv_dh[1];
+        }
+        alive = process_boundary(m, md, lc0, lc1, o0, o1, v0, v1, v2, dtr);
+        if (!alive) break;
+    }
+    return alive;
+}
+
+__device__ __forceinline__ int cell_i(int32_t c) { return c & 0xFFFF; }
+__device__ __forceinline__ int cell_j(int32_t c) { return (c >> 16) & 0xFFFF; }
+__device__ __forceinline__ int32_t pack_cell(int i, int j) { return (j << 16) | i; }
+
+// Exact round-to-nearest of a double below 2^51 in magnitude to int64 (1.5*2^52 trick):
+// one FMA and one integer subtraction, order independent.
+constexpr double QMAGIC = 6755399441055744.0;
+__device__ __forceinline__ long long quant(double w, double val) {
+    return (long long)(__double_as_longlong(fma(w, val, QMAGIC)) - __double_as_longlong(QMAGIC));
+}
+
+}  // namespace sf

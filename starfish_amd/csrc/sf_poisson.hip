@@ -1,0 +1,899 @@
+// Boltzmann-electron nonlinear Poisson solve (pic/PoissonSolver.java, core/solver/Solver.java)
+// on gfx950: matrix-free operator, FP64 Newton, FP32 Jacobi-PCG with FP64 residual refinement.
+//
+// Reference operator.  Solver.initCoefficients assembles, on a uniform mesh:
+//   * Dirichlet / internal nodes: identity rows                         (Solver.java:463-468)
+//   * nodes on a non-MESH mesh face: the gradient row Gi (left/right faces, which take
+//     precedence at corners) or Gj (bottom/top), i.e. (phi_in - phi_face)/dh = 0
+//                                                                        (Solver.java:486-499)
+//   * every other node: the finite-volume Laplacian sum_f R_f (grad phi . n) l_f / V with
+//     V = nodeVol (2*pi*A*r in RZ), which on a uniform mesh is the 5-point stencil
+//     XY: (phi_E - 2 phi + phi_W)/dx^2 + (phi_N - 2 phi + phi_S)/dy^2
+//     RZ: [r_{i+1/2}(phi_E - phi) - r_{i-1/2}(phi - phi_W)]/(2 pi r_i dr^2)
+//         + (phi_N - 2 phi + phi_S)/(2 pi dz^2)                          (Solver.java:503-552)
+// Eliminating the identity rows (values into the right-hand side) and the one-sided rows
+// (phi_face = phi_inside) and multiplying each remaining row by its control volume W_i gives
+// a symmetric positive-definite system L x + W b(x) = f with the same discrete solution.
+// Node classes are derived on the host; a free node stores, per neighbour, whether that
+// neighbour is free, fixed (value known) or a face node tied back to the node itself.
+#include <cmath>
+#include <cstring>
+
+#include "sf_internal.h"
+
+namespace sf {
+
+enum { NB_SELF = 0, NB_FREE = 1, NB_FIXED = 2 };
+constexpr uint8_t CLS_NOT_FREE = 0xFF;
+
+struct PoissonState {
+    sf_poisson_desc desc{};
+    double eps = EPS0, C = 0;     // C = e/eps
+    int nfree = 0;
+    // device
+    uint8_t* cls = nullptr;       // [nn] neighbour classes (2 bits each: E, W, N, S) or CLS_NOT_FREE
+    double* W = nullptr;          // [nn] row weight (control volume)
+    double* bfix = nullptr;       // [nn] sum of c' * phi_fixed over fixed neighbours
+    int32_t* src = nullptr;       // [nn] node whose value a node takes (itself if free, -1 if fixed)
+    double* fixval = nullptr;     // [nn] value of fixed nodes
+    double* x = nullptr;          // [nn] FP64 potential on free nodes (solution vector)
+    double* r = nullptr;          // [nn] FP64 Newton residual
+    double* y = nullptr;          // [nn] FP64 accumulated correction
+    double* rr = nullptr;         // [nn] FP64 refinement residual
+    float* D = nullptr;           // [nn] FP32 Jacobian diagonal
+    double* P = nullptr;          // [nn] FP64 W * b'(x)
+    float *d = nullptr, *res = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
+    double* partial = nullptr;    // [nblk * 4]
+    double* scal = nullptr;       // device scalars
+    unsigned int* ticket = nullptr;
+    int* istate = nullptr;        // [0] done flag, [1] iterations
+    int nblk_tile = 0, nblk_elem = 0;
+    double* host_scal = nullptr;  // pinned
+    int* host_istate = nullptr;
+    hipEvent_t e0{}, e1{};
+};
+
+// device scalars layout
+enum { S_RZ = 0, S_RZ_OLD, S_PQ, S_RR, S_TOL2, S_NORM2, S_R0, S_NSCAL = 8 };
+
+void poisson_free(PoissonState* p) {
+    if (!p) return;
+    void* ptrs[] = {p->cls, p->W,  p->bfix, p->src, p->fixval, p->x,  p->r,       p->y,    p->rr,
+                    p->D,   p->P,  p->d,    p->res, p->z,      p->p0, p->p1,      p->q,    p->partial,
+                    p->scal, p->ticket, p->istate};
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    if (p->host_scal) hipHostFree(p->host_scal);
+    if (p->host_istate) hipHostFree(p->host_istate);
+    if (p->e0) hipEventDestroy(p->e0);
+    if (p->e1) hipEventDestroy(p->e1);
+    delete p;
+}
+
+// symmetric stencil coefficients c' of a free node (row scaled by its control volume)
+struct Stencil {
+    double cE, cW, cN, cS;
+};
+__host__ __device__ __forceinline__ Stencil stencil(const MeshP& m, int i, int j) {
+    Stencil s;
+    if (m.domain == SF_DOMAIN_RZ) {
+        double dr = m.dh[0], dz = m.dh[1];
+        double ri = m.x0[0] + i * dr;
+        s.cE = (ri + 0.5 * dr) * dz / dr;
+        s.cW = (ri - 0.5 * dr) * dz / dr;
+        s.cN = ri * dr / dz;
+        s.cS = s.cN;
+    } else if (m.domain == SF_DOMAIN_ZR) {
+        double dz = m.dh[0], dr = m.dh[1];
+        double rj = m.x0[1] + j * dr;
+        s.cN = (rj + 0.5 * dr) * dz / dr;
+        s.cS = (rj - 0.5 * dr) * dz / dr;
+        s.cE = rj * dr / dz;
+        s.cW = s.cE;
+    } else {
+        s.cE = s.cW = m.dh[1] / m.dh[0];
+        s.cN = s.cS = m.dh[0] / m.dh[1];
+    }
+    return s;
+}
+// control volume that scales a free row (Mesh.nodeVol for interior nodes, 2*pi factor kept)
+__host__ __device__ __forceinline__ double row_weight(const MeshP& m, int i, int j) {
+    if (m.domain == SF_DOMAIN_RZ) return 2 * PI * m.dh[0] * m.dh[1] * (m.x0[0] + i * m.dh[0]);
+    if (m.domain == SF_DOMAIN_ZR) return 2 * PI * m.dh[0] * m.dh[1] * (m.x0[1] + j * m.dh[1]);
+    return m.dh[0] * m.dh[1];
+}
+// reference row scale: the assembled RZ row carries R_f but is divided by 2*pi*A*r, so the
+// symmetric coefficients above (without 2*pi) equal W * A_ref with W = 2*pi*A*r.
+
+__device__ __forceinline__ int nbc(uint8_t c, int dir) { return (c >> (2 * dir)) & 3; }
+
+// L x restricted to free neighbours (FP64): diag*x_i - sum_free c' x_nb
+__device__ __forceinline__ double lred_apply64(const MeshP& m, uint8_t c, int i, int j, size_t u, const double* x,
+                                               double& diagL) {
+    Stencil s = stencil(m, i, j);
+    double xi = x[u];
+    double acc = 0;
+    diagL = 0;
+    int e = nbc(c, 0), w = nbc(c, 1), n = nbc(c, 2), so = nbc(c, 3);
+    if (e != NB_SELF) { diagL += s.cE; if (e == NB_FREE) acc -= s.cE * x[u + 1]; }
+    if (w != NB_SELF) { diagL += s.cW; if (w == NB_FREE) acc -= s.cW * x[u - 1]; }
+    if (n != NB_SELF) { diagL += s.cN; if (n == NB_FREE) acc -= s.cN * x[u + m.ni]; }
+    if (so != NB_SELF) { diagL += s.cS; if (so == NB_FREE) acc -= s.cS * x[u - m.ni]; }
+    return acc + diagL * xi;
+}
+
+// Newton residual r = f + bfix - L x - W b(x), Jacobian diagonal D = diag(L) + W b'(x);
+// also ||r||^2 partials.  f = W * rho / eps.
+__global__ void newton_residual(MeshP m, const uint8_t* __restrict__ cls, const double* __restrict__ W,
+                                const double* __restrict__ bfix, const double* __restrict__ rho,
+                                const double* __restrict__ x, double* __restrict__ r, float* __restrict__ D,
+                                double* __restrict__ P, double inv_eps, double Cn0, double phi0, double inv_kTe,
+                                int linear, double* __restrict__ partial, unsigned int* ticket,
+                                double* __restrict__ scal, int slot) {
+    __shared__ double sh[16];
+    const int64_t nn = (int64_t)m.ni * m.nj;
+    double acc = 0;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        uint8_t c = cls[u];
+        if (c == CLS_NOT_FREE) { r[u] = 0; D[u] = 1.0f; P[u] = 0; continue; }
+        int i = (int)(u % m.ni), j = (int)(u / m.ni);
+        double diagL;
+        double Lx = lred_apply64(m, c, i, j, u, x, diagL);
+        double Wu = W[u];
+        double bx = 0, bp = 0;
+        if (!linear) {
+            bx = Cn0 * exp((x[u] - phi0) * inv_kTe);    // PoissonSolver.java:229-244
+            bp = bx * inv_kTe;
+        }
+        double ru = Wu * rho[u] * inv_eps + bfix[u] - Lx - Wu * bx;
+        r[u] = ru;
+        P[u] = Wu * bp;
+        D[u] = (float)(diagL + Wu * bp);
+        acc += ru * ru;
+    }
+    // block reduction, last block finalises
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w];
+        partial[blockIdx.x] = s;
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            double tot = 0;
+            for (unsigned int b = 0; b < gridDim.x; b++) tot += ((volatile double*)partial)[b];
+            scal[slot] = tot;
+            *ticket = 0;
+        }
+    }
+}
+
+// ||f + bfix - W b(0)||^2 for the fixed reference norm
+__global__ void ref_norm_kernel(MeshP m, const uint8_t* __restrict__ cls, const double* __restrict__ W,
+                                const double* __restrict__ bfix, const double* __restrict__ rho, double inv_eps,
+                                double Cn0, double phi0, double inv_kTe, int linear, double* __restrict__ partial,
+                                unsigned int* ticket, double* __restrict__ scal, int slot) {
+    __shared__ double sh[16];
+    const int64_t nn = (int64_t)m.ni * m.nj;
+    double acc = 0;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        if (cls[u] == CLS_NOT_FREE) continue;
+        double b0 = linear ? 0.0 : Cn0 * exp((0.0 - phi0) * inv_kTe);
+        double v = W[u] * rho[u] * inv_eps + bfix[u] - W[u] * b0;
+        acc += v * v;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w];
+        partial[blockIdx.x] = s;
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            double tot = 0;
+            for (unsigned int b = 0; b < gridDim.x; b++) tot += ((volatile double*)partial)[b];
+            scal[slot] = tot;
+            *ticket = 0;
+        }
+    }
+}
+
+// refinement residual rr = r - (L + P) y (FP64) and ||rr||^2; also initialises the FP32 PCG:
+// d = 0, res = rr, z = rr / D, rz partial
+__global__ void refine_residual(MeshP m, const uint8_t* __restrict__ cls, const double* __restrict__ r,
+                                const double* __restrict__ y, const double* __restrict__ P,
+                                const float* __restrict__ D, double* __restrict__ rr, float* __restrict__ d,
+                                float* __restrict__ res, float* __restrict__ z, float* __restrict__ p0,
+                                double* __restrict__ partial, unsigned int* ticket, double* __restrict__ scal,
+                                int* istate) {
+    __shared__ double sh[2][16];
+    const int64_t nn = (int64_t)m.ni * m.nj;
+    double a_rr = 0, a_rz = 0;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        uint8_t c = cls[u];
+        float zero = 0.0f;
+        if (c == CLS_NOT_FREE) {
+            rr[u] = 0; d[u] = zero; res[u] = zero; z[u] = zero; p0[u] = zero;
+            continue;
+        }
+        int i = (int)(u % m.ni), j = (int)(u / m.ni);
+        double diagL;
+        double Jy = lred_apply64(m, c, i, j, u, y, diagL) + P[u] * y[u];
+        double v = r[u] - Jy;
+        rr[u] = v;
+        float rf = (float)v;
+        float zf = rf / D[u];
+        d[u] = zero;
+        res[u] = rf;
+        z[u] = zf;
+        p0[u] = zf;
+        a_rr += v * v;
+        a_rz += (double)rf * (double)zf;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a_rr += __shfl_xor(a_rr, o);
+        a_rz += __shfl_xor(a_rz, o);
+    }
+    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = a_rr; sh[1][threadIdx.x >> 6] = a_rz; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) { s0 += sh[0][w]; s1 += sh[1][w]; }
+        partial[2 * blockIdx.x] = s0;
+        partial[2 * blockIdx.x + 1] = s1;
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            double t0 = 0, t1 = 0;
+            for (unsigned int b = 0; b < gridDim.x; b++) {
+                t0 += ((volatile double*)partial)[2 * b];
+                t1 += ((volatile double*)partial)[2 * b + 1];
+            }
+            scal[S_NORM2] = t0;
+            scal[S_RR] = t0;
+            scal[S_RZ] = t1;
+            istate[0] = (t0 <= scal[S_TOL2]) ? 1 : 0;
+            istate[2] = 0;   // first iteration uses p = z
+            *ticket = 0;
+        }
+    }
+}
+
+// PCG step 1: p_new = z + beta p_old (beta = rz/rz_old; p = z on the first iteration);
+// q = J p_new; partial p.q
+constexpr int PT_W = 64, PT_H = 4;   // interior nodes per block (256 threads, one node each)
+__global__ void __launch_bounds__(256) pcg_spmv(MeshP m, const uint8_t* __restrict__ cls,
+                                                const float* __restrict__ D, const float* __restrict__ z,
+                                                const float* __restrict__ pold, float* __restrict__ pnew,
+                                                float* __restrict__ q, double* __restrict__ partial,
+                                                unsigned int* ticket, double* __restrict__ scal, int* istate) {
+    if (istate[0]) return;   // converged: the remaining queued iterations are no-ops
+    __shared__ float ps[(PT_H + 2) * (PT_W + 2)];
+    __shared__ double sh[4];
+    const int bx = blockIdx.x % ((m.ni + PT_W - 1) / PT_W), by = blockIdx.x / ((m.ni + PT_W - 1) / PT_W);
+    const int i0 = bx * PT_W - 1, j0 = by * PT_H - 1;
+    const bool first = istate[2] == 0;
+    const float beta = first ? 0.0f : (float)(scal[S_RZ] / scal[S_RZ_OLD]);
+    for (int k = threadIdx.x; k < (PT_H + 2) * (PT_W + 2); k += blockDim.x) {
+        int i = i0 + k % (PT_W + 2), j = j0 + k / (PT_W + 2);
+        float v = 0.0f;
+        if (i >= 0 && j >= 0 && i < m.ni && j < m.nj) {
+            size_t u = (size_t)j * m.ni + i;
+            v = first ? z[u] : z[u] + beta * pold[u];
+        }
+        ps[k] = v;
+    }
+    __syncthreads();
+    const int li = threadIdx.x % PT_W, lj = threadIdx.x / PT_W;
+    const int i = i0 + 1 + li, j = j0 + 1 + lj;
+    double acc = 0;
+    if (i < m.ni && j < m.nj) {
+        size_t u = (size_t)j * m.ni + i;
+        uint8_t c = cls[u];
+        int l = (lj + 1) * (PT_W + 2) + (li + 1);
+        float pv = ps[l];
+        float qv = 0.0f;
+        if (c != CLS_NOT_FREE) {
+            Stencil s = stencil(m, i, j);
+            qv = D[u] * pv;
+            if (nbc(c, 0) == NB_FREE) qv -= (float)s.cE * ps[l + 1];
+            if (nbc(c, 1) == NB_FREE) qv -= (float)s.cW * ps[l - 1];
+            if (nbc(c, 2) == NB_FREE) qv -= (float)s.cN * ps[l + PT_W + 2];
+            if (nbc(c, 3) == NB_FREE) qv -= (float)s.cS * ps[l - PT_W - 2];
+        }
+        pnew[u] = pv;
+        q[u] = qv;
+        acc = (double)pv * (double)qv;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            double tot = 0;
+            for (unsigned int b = 0; b < gridDim.x; b++) tot += ((volatile double*)partial)[b];
+            scal[S_PQ] = tot;
+            *ticket = 0;
+        }
+    }
+}
+
+// PCG step 2: alpha = rz / pq; d += alpha p; res -= alpha q; z = res / D; partials rz, rr.
+// The last block sets rz_old, rz, the done flag and the iteration count.
+__global__ void __launch_bounds__(256) pcg_update(int64_t nn, const float* __restrict__ D, const float* __restrict__ p,
+                                                  const float* __restrict__ q, float* __restrict__ d,
+                                                  float* __restrict__ res, float* __restrict__ z,
+                                                  double* __restrict__ partial, unsigned int* ticket,
+                                                  double* __restrict__ scal, int* istate, int max_it) {
+    if (istate[0]) return;
+    __shared__ double sh[2][4];
+    const float alpha = (float)(scal[S_RZ] / scal[S_PQ]);
+    double a_rz = 0, a_rr = 0;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        float pv = p[u], qv = q[u];
+        float dv = d[u] + alpha * pv;
+        float rv = res[u] - alpha * qv;
+        float zv = rv / D[u];
+        d[u] = dv;
+        res[u] = rv;
+        z[u] = zv;
+        a_rz += (double)rv * (double)zv;
+        a_rr += (double)rv * (double)rv;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a_rz += __shfl_xor(a_rz, o);
+        a_rr += __shfl_xor(a_rr, o);
+    }
+    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = a_rz; sh[1][threadIdx.x >> 6] = a_rr; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+        partial[2 * blockIdx.x + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            double t0 = 0, t1 = 0;
+            for (unsigned int b = 0; b < gridDim.x; b++) {
+                t0 += ((volatile double*)partial)[2 * b];
+                t1 += ((volatile double*)partial)[2 * b + 1];
+            }
+            scal[S_RZ_OLD] = scal[S_RZ];
+            scal[S_RZ] = t0;
+            scal[S_RR] = t1;
+            istate[1] += 1;
+            istate[2] = 1;
+            if (t1 <= scal[S_TOL2] || istate[1] >= max_it || !(t1 == t1)) istate[0] = 1;
+            *ticket = 0;
+        }
+    }
+}
+
+// y += d (FP64 accumulation of the FP32 correction)
+__global__ void accumulate_y(int64_t nn, const float* __restrict__ d, double* __restrict__ y) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
+        y[u] += (double)d[u];
+}
+// x += y (Newton update, Solver.java:366-368 with the sign of the reduced system); y = 0
+__global__ void newton_update(int64_t nn, const uint8_t* __restrict__ cls, double* __restrict__ x,
+                              double* __restrict__ y, double* __restrict__ partial, unsigned int* ticket,
+                              double* __restrict__ scal, int slot) {
+    __shared__ double sh[16];
+    double acc = 0;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        double v = y[u];
+        if (cls[u] != CLS_NOT_FREE) x[u] += v;
+        acc += v * v;
+        y[u] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w];
+        partial[blockIdx.x] = s;
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            double tot = 0;
+            for (unsigned int b = 0; b < gridDim.x; b++) tot += ((volatile double*)partial)[b];
+            scal[slot] = tot;
+            *ticket = 0;
+        }
+    }
+}
+
+// full potential from the free-node solution: fixed nodes take their value, face nodes the
+// value of the node their one-sided row ties them to
+__global__ void expand_phi(int64_t nn, const int32_t* __restrict__ src, const double* __restrict__ fixval,
+                           const double* __restrict__ x, double* __restrict__ phi) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        int32_t s = src[u];
+        phi[u] = s < 0 ? fixval[u] : x[s];
+    }
+}
+__global__ void gather_free(int64_t nn, const uint8_t* __restrict__ cls, const double* __restrict__ phi,
+                            double* __restrict__ x) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
+        x[u] = cls[u] == CLS_NOT_FREE ? 0.0 : phi[u];
+}
+
+// E = -grad(phi): Solver.evaluateGradient with the assembled Gi/Gj rows, which on a uniform
+// mesh are central differences inside and one-sided differences on the mesh faces.
+__global__ void efield_kernel(MeshP m, const double* __restrict__ phi, double* __restrict__ efi,
+                              double* __restrict__ efj, float2* __restrict__ ef32) {
+    const int64_t nn = (int64_t)m.ni * m.nj;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        int i = (int)(u % m.ni), j = (int)(u / m.ni);
+        double gi, gj;
+        if (i == 0) gi = (phi[u + 1] - phi[u]) * m.inv_dh[0];
+        else if (i == m.ni - 1) gi = (phi[u] - phi[u - 1]) * m.inv_dh[0];
+        else gi = (phi[u + 1] - phi[u - 1]) * 0.5 * m.inv_dh[0];
+        if (j == 0) gj = (phi[u + m.ni] - phi[u]) * m.inv_dh[1];
+        else if (j == m.nj - 1) gj = (phi[u] - phi[u - m.ni]) * m.inv_dh[1];
+        else gj = (phi[u + m.ni] - phi[u - m.ni]) * 0.5 * m.inv_dh[1];
+        efi[u] = -gi;
+        efj[u] = -gj;
+        ef32[u] = make_float2((float)(-gi), (float)(-gj));
+    }
+}
+
+// y = A_ref x on the full grid (reference rows), for the operator parity check
+__global__ void apply_reference_operator(MeshP m, const uint8_t* __restrict__ rowtype, const double* __restrict__ x,
+                                         double* __restrict__ y) {
+    const int64_t nn = (int64_t)m.ni * m.nj;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        int i = (int)(u % m.ni), j = (int)(u / m.ni);
+        uint8_t t = rowtype[u];
+        double v;
+        switch (t) {
+            case 0: v = x[u]; break;                                           // identity
+            case 1: v = (x[u + 1] - x[u]) * m.inv_dh[0]; break;                  // Gi, left face
+            case 2: v = (x[u] - x[u - 1]) * m.inv_dh[0]; break;                  // Gi, right face
+            case 3: v = (x[u + m.ni] - x[u]) * m.inv_dh[1]; break;               // Gj, bottom face
+            case 4: v = (x[u] - x[u - m.ni]) * m.inv_dh[1]; break;               // Gj, top face
+            default: {
+                Stencil s = stencil(m, i, j);
+                double Wu = row_weight(m, i, j);
+                v = (s.cE * (x[u + 1] - x[u]) + s.cW * (x[u - 1] - x[u]) + s.cN * (x[u + m.ni] - x[u]) +
+                     s.cS * (x[u - m.ni] - x[u])) / Wu;
+            }
+        }
+        y[u] = v;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// host
+// ----------------------------------------------------------------------------------------
+static int grid_elem(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 1024) b = 1024;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+// row types of the reference assembly: 0 identity, 1..4 one-sided gradient rows, 5 Laplacian
+static int reference_row_types(sf_ctx* ctx, std::vector<uint8_t>& rt) {
+    const MeshP& m = ctx->mp;
+    const int ni = m.ni, nj = m.nj;
+    rt.assign((size_t)ni * nj, 5);
+    auto face = [&](int f, int idx) {
+        int off = f == 0 ? 0 : (f == 1 ? nj : (f == 2 ? nj + ni : 2 * nj + ni));
+        return ctx->h_face_type[off + idx];
+    };
+    for (int j = 0; j < nj; j++)
+        for (int i = 0; i < ni; i++) {
+            size_t u = (size_t)j * ni + i;
+            int nt = ctx->h_node_type[u];
+            if (nt == SF_NODE_DIRICHLET || nt == SF_NODE_INTERNAL) { rt[u] = 0; continue; }
+            if (nt == SF_NODE_INSULATOR)
+                return sf_fail(ctx, SF_ERR_UNSUPPORTED, "insulator nodes are not supported by the GPU solver");
+            if (i == 0 && face(2, j) != SF_FACE_MESH) { rt[u] = 1; continue; }
+            if (i == ni - 1 && face(0, j) != SF_FACE_MESH) { rt[u] = 2; continue; }
+            if (j == 0 && face(3, i) != SF_FACE_MESH) { rt[u] = 3; continue; }
+            if (j == nj - 1 && face(1, i) != SF_FACE_MESH) { rt[u] = 4; continue; }
+            if (i == 0 || j == 0 || i == ni - 1 || j == nj - 1)
+                return sf_fail(ctx, SF_ERR_UNSUPPORTED, "MESH faces (multi-domain) are not supported");
+        }
+    return SF_OK;
+}
+
+int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* dsc) {
+    if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
+    const MeshP& m = ctx->mp;
+    const int ni = m.ni, nj = m.nj;
+    const size_t nn = (size_t)ni * nj;
+    std::vector<uint8_t> rt;
+    int rc = reference_row_types(ctx, rt);
+    if (rc) return rc;
+    // resolve the value source of every node: free nodes are their own source, fixed nodes
+    // have none (-1), one-sided rows follow their tie (chains at corners)
+    std::vector<int32_t> src(nn);
+    for (size_t u = 0; u < nn; u++) src[u] = rt[u] == 0 ? -1 : (int32_t)u;
+    auto tie = [&](size_t u) -> int64_t {
+        switch (rt[u]) {
+            case 1: return (int64_t)u + 1;
+            case 2: return (int64_t)u - 1;
+            case 3: return (int64_t)u + ni;
+            case 4: return (int64_t)u - ni;
+        }
+        return -2;
+    };
+    std::vector<double> fixval(nn, 0.0);
+    for (size_t u = 0; u < nn; u++)
+        if (rt[u] == 0) fixval[u] = ctx->h_node_bc[u];
+    for (size_t u = 0; u < nn; u++) {
+        if (rt[u] == 0 || rt[u] == 5) continue;
+        size_t v = u;
+        int guard = 0;
+        while (rt[v] >= 1 && rt[v] <= 4 && guard++ < 8) v = (size_t)tie(v);
+        if (rt[v] >= 1 && rt[v] <= 4) return sf_fail(ctx, SF_ERR_UNSUPPORTED, "cyclic face rows at node %zu", u);
+        if (rt[v] == 0) { src[u] = -1; fixval[u] = ctx->h_node_bc[v]; }
+        else src[u] = (int32_t)v;
+    }
+    std::vector<uint8_t> cls(nn, CLS_NOT_FREE);
+    std::vector<double> W(nn, 0.0), bfix(nn, 0.0);
+    int nfree = 0;
+    for (int j = 0; j < nj; j++)
+        for (int i = 0; i < ni; i++) {
+            size_t u = (size_t)j * ni + i;
+            if (rt[u] != 5) continue;
+            nfree++;
+            Stencil s = stencil(m, i, j);
+            const size_t nb[4] = {u + 1, u - 1, u + ni, u - ni};
+            const double cc[4] = {s.cE, s.cW, s.cN, s.cS};
+            uint8_t c = 0;
+            for (int d = 0; d < 4; d++) {
+                size_t v = nb[d];
+                int kind;
+                if (rt[v] == 5) kind = NB_FREE;
+                else if (src[v] < 0) { kind = NB_FIXED; bfix[u] += cc[d] * fixval[v]; }
+                else if ((size_t)src[v] == u) kind = NB_SELF;
+                else return sf_fail(ctx, SF_ERR_UNSUPPORTED, "node %zu couples to a face row of another node", u);
+                c |= (uint8_t)(kind << (2 * d));
+            }
+            cls[u] = c;
+            W[u] = row_weight(m, i, j);
+        }
+    PoissonState* P = ctx->poisson;
+    if (!P) {
+        P = new PoissonState();
+        ctx->poisson = P;
+        auto A = [&](void** p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 8); };
+        hipError_t e = hipSuccess;
+        int nb_tile = ((ni + PT_W - 1) / PT_W) * ((nj + PT_H - 1) / PT_H);
+        P->nblk_tile = nb_tile;
+        P->nblk_elem = grid_elem((int64_t)nn);
+        size_t npart = (size_t)std::max(nb_tile, P->nblk_elem) * 4 + 16;
+        e = A((void**)&P->cls, nn);
+        if (!e) e = A((void**)&P->W, nn * 8);
+        if (!e) e = A((void**)&P->bfix, nn * 8);
+        if (!e) e = A((void**)&P->src, nn * 4);
+        if (!e) e = A((void**)&P->fixval, nn * 8);
+        if (!e) e = A((void**)&P->x, nn * 8);
+        if (!e) e = A((void**)&P->r, nn * 8);
+        if (!e) e = A((void**)&P->y, nn * 8);
+        if (!e) e = A((void**)&P->rr, nn * 8);
+        if (!e) e = A((void**)&P->D, nn * 4);
+        if (!e) e = A((void**)&P->P, nn * 8);
+        if (!e) e = A((void**)&P->d, nn * 4);
+        if (!e) e = A((void**)&P->res, nn * 4);
+        if (!e) e = A((void**)&P->z, nn * 4);
+        if (!e) e = A((void**)&P->p0, nn * 4);
+        if (!e) e = A((void**)&P->p1, nn * 4);
+        if (!e) e = A((void**)&P->q, nn * 4);
+        if (!e) e = A((void**)&P->partial, npart * 8);
+        if (!e) e = A((void**)&P->scal, S_NSCAL * 8 * 2);
+        if (!e) e = A((void**)&P->ticket, 16);
+        if (!e) e = A((void**)&P->istate, 16);
+        if (!e) e = hipHostMalloc((void**)&P->host_scal, S_NSCAL * 8 * 2);
+        if (!e) e = hipHostMalloc((void**)&P->host_istate, 16);
+        if (!e) e = hipEventCreate(&P->e0);
+        if (!e) e = hipEventCreate(&P->e1);
+        if (e) return sf_fail(ctx, SF_ERR_OOM, "poisson allocation failed: %s", hipGetErrorString(e));
+        hipMemset(P->ticket, 0, 16);
+        hipMemset(P->istate, 0, 16);
+        hipMemset(P->y, 0, nn * 8);
+        hipMemset(P->x, 0, nn * 8);
+        hipMemset(P->d, 0, nn * 4);
+        hipMemset(P->res, 0, nn * 4);
+        hipMemset(P->z, 0, nn * 4);
+        hipMemset(P->p0, 0, nn * 4);
+        hipMemset(P->p1, 0, nn * 4);
+        hipMemset(P->q, 0, nn * 4);
+    }
+    P->desc = *dsc;
+    P->eps = EPS0 * (dsc->eps_r > 0 ? dsc->eps_r : 1.0);
+    P->C = QE / P->eps;
+    P->nfree = nfree;
+    SF_HIP(ctx, hipMemcpy(P->cls, cls.data(), nn, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(P->W, W.data(), nn * 8, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(P->bfix, bfix.data(), nn * 8, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(P->src, src.data(), nn * 4, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(P->fixval, fixval.data(), nn * 8, hipMemcpyHostToDevice));
+    // phi starts with the Dirichlet values on fixed nodes (Node.bc_value)
+    std::vector<double> phi(nn, 0.0);
+    for (size_t u = 0; u < nn; u++) phi[u] = src[u] < 0 ? fixval[u] : 0.0;
+    SF_HIP(ctx, hipMemcpy(ctx->d_phi, phi.data(), nn * 8, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipDeviceSynchronize());
+    return SF_OK;
+}
+
+int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
+    PoissonState* P = ctx->poisson;
+    if (!P) return sf_fail(ctx, SF_ERR_STATE, "poisson solver not set up");
+    const MeshP& m = ctx->mp;
+    const int64_t nn = (int64_t)m.ni * m.nj;
+    const sf_poisson_desc& D = P->desc;
+    hipStream_t st = ctx->stream;
+    const int ge = P->nblk_elem;
+    const double inv_eps = 1.0 / P->eps;
+    const double Cn0 = D.linear ? 0.0 : P->C * D.n0;
+    const double inv_kTe = D.linear ? 0.0 : 1.0 / D.Te0;
+    const double rel_tol = D.rel_tol > 0 ? D.rel_tol : 1e-8;
+    const int max_it = D.max_it > 0 ? D.max_it : 5000;
+    const int nl_max = D.linear ? 1 : (D.nl_max_it > 0 ? D.nl_max_it : 50);
+    SF_HIP(ctx, hipEventRecord(P->e0, st));
+    hipLaunchKernelGGL(gather_free, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, (const double*)ctx->d_phi,
+                       P->x);
+    hipLaunchKernelGGL(ref_norm_kernel, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
+                       (const double*)P->bfix, (const double*)ctx->d_rho, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
+                       P->partial, P->ticket, P->scal, (int)S_R0);
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
+    SF_HIP(ctx, hipStreamSynchronize(st));
+    double ref2 = P->host_scal[S_R0];
+    double ref = std::sqrt(ref2);
+    if (!(ref > 0)) ref = 1.0;
+    int newton = 0, pcg_total = 0;
+    double rel = 0, rnorm = 0;
+    bool converged = false;
+    const int batch = 32;
+    for (int it = 0; it <= nl_max; it++) {
+        hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+                           (const double*)P->W, (const double*)P->bfix, (const double*)ctx->d_rho,
+                           (const double*)P->x, P->r, P->D, P->P, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
+                           P->partial, P->ticket, P->scal, (int)S_NORM2);
+        SF_HIP(ctx, hipGetLastError());
+        SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
+        SF_HIP(ctx, hipStreamSynchronize(st));
+        rnorm = std::sqrt(P->host_scal[S_NORM2]);
+        rel = rnorm / ref;
+        if (!std::isfinite(rel)) return sf_fail(ctx, SF_ERR_STATE, "Newton residual is not finite (diverged)");
+        if (rel <= rel_tol) { converged = true; break; }
+        if (it == nl_max) break;
+        newton++;
+        // inexact Newton: solve J y = r to max(0.01*||r||, 0.5*tol*ref) with FP32 PCG + FP64 refinement
+        double lin_target = std::max(D.linear ? 0.5 * rel_tol * ref : 1e-2 * rnorm, 0.5 * rel_tol * ref);
+        // inner FP32 tolerance: a relative reduction FP32 can deliver
+        int refine = 0;
+        while (true) {
+            hipLaunchKernelGGL(refine_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+                               (const double*)P->r, (const double*)P->y, (const double*)P->P, (const float*)P->D,
+                               P->rr, P->d, P->res, P->z, P->p0, P->partial, P->ticket, P->scal, P->istate);
+            SF_HIP(ctx, hipGetLastError());
+            SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
+            SF_HIP(ctx, hipStreamSynchronize(st));
+            double rr = std::sqrt(P->host_scal[S_NORM2]);
+            if (rr <= lin_target || refine >= 8 || pcg_total >= max_it * nl_max) break;
+            double inner_tol = std::max(lin_target, 1e-5 * rr);
+            double tol2 = inner_tol * inner_tol;
+            SF_HIP(ctx, hipMemcpyAsync(P->scal + S_TOL2, &tol2, 8, hipMemcpyHostToDevice, st));
+            int zero[3] = {0, 0, 0};
+            SF_HIP(ctx, hipMemcpyAsync(P->istate, zero, 12, hipMemcpyHostToDevice, st));
+            int done = 0, its = 0;
+            float* pa = P->p0;
+            float* pb = P->p1;
+            while (!done) {
+                for (int b = 0; b < batch; b++) {
+                    hipLaunchKernelGGL(pcg_spmv, dim3(P->nblk_tile), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+                                       (const float*)P->D, (const float*)P->z, (const float*)pa, pb, P->q,
+                                       P->partial, P->ticket, P->scal, P->istate);
+                    hipLaunchKernelGGL(pcg_update, dim3(ge), dim3(256), 0, st, nn, (const float*)P->D,
+                                       (const float*)pb, (const float*)P->q, P->d, P->res, P->z, P->partial,
+                                       P->ticket, P->scal, P->istate, max_it);
+                    std::swap(pa, pb);
+                }
+                SF_HIP(ctx, hipGetLastError());
+                SF_HIP(ctx, hipMemcpyAsync(P->host_istate, P->istate, 12, hipMemcpyDeviceToHost, st));
+                SF_HIP(ctx, hipStreamSynchronize(st));
+                done = P->host_istate[0];
+                its = P->host_istate[1];
+            }
+            pcg_total += its;
+            hipLaunchKernelGGL(accumulate_y, dim3(ge), dim3(256), 0, st, nn, (const float*)P->d, P->y);
+            refine++;
+        }
+        hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y,
+                           P->partial, P->ticket, P->scal, (int)S_RR);
+        SF_HIP(ctx, hipGetLastError());
+        if (D.linear) {
+            // one linear solve; evaluate the final residual on the next pass of the loop
+            continue;
+        }
+    }
+    hipLaunchKernelGGL(expand_phi, dim3(ge), dim3(256), 0, st, nn, (const int32_t*)P->src, (const double*)P->fixval,
+                       (const double*)P->x, ctx->d_phi);
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipEventRecord(P->e1, st));
+    SF_HIP(ctx, hipEventSynchronize(P->e1));
+    float ms = 0;
+    hipEventElapsedTime(&ms, P->e0, P->e1);
+    if (info) {
+        info->newton_iterations = newton;
+        info->pcg_iterations = pcg_total;
+        info->rel_residual = rel;
+        info->ref_norm = rnorm / (double)nn;
+        info->ms = ms;
+    }
+    return converged ? SF_OK : SF_WARN_NOT_CONVERGED;
+}
+
+int efield_update(sf_ctx* ctx) {
+    const int64_t nn = (int64_t)ctx->mp.ni * ctx->mp.nj;
+    hipLaunchKernelGGL(efield_kernel, dim3(grid_elem(nn)), dim3(256), 0, ctx->stream, ctx->mp,
+                       (const double*)ctx->d_phi, ctx->d_efi, ctx->d_efj, ctx->d_ef32);
+    SF_HIP(ctx, hipGetLastError());
+    return SF_OK;
+}
+
+// rho = sum_s q_s nd_s (SolverModule.updateRho)
+__global__ void rho_accumulate(int64_t nn, const double* __restrict__ den, double q, double* __restrict__ rho,
+                               int first) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
+        rho[u] = (first ? 0.0 : rho[u]) + den[u] * q;
+}
+
+int rho_update(sf_ctx* ctx) {
+    const int64_t nn = (int64_t)ctx->mp.ni * ctx->mp.nj;
+    bool first = true;
+    for (Species* sp : ctx->species) {
+        if (sp->charge == 0) continue;
+        hipLaunchKernelGGL(rho_accumulate, dim3(grid_elem(nn)), dim3(256), 0, ctx->stream, nn, (const double*)sp->den,
+                           sp->charge, ctx->d_rho, first ? 1 : 0);
+        first = false;
+    }
+    if (first) SF_HIP(ctx, hipMemsetAsync(ctx->d_rho, 0, nn * 8, ctx->stream));
+    SF_HIP(ctx, hipGetLastError());
+    return SF_OK;
+}
+
+int operator_apply(sf_ctx* ctx, const double* x, double* y) {
+    std::vector<uint8_t> rt;
+    int rc = reference_row_types(ctx, rt);
+    if (rc) return rc;
+    const int64_t nn = (int64_t)ctx->mp.ni * ctx->mp.nj;
+    uint8_t* d_rt = nullptr;
+    double *d_x = nullptr, *d_y = nullptr;
+    SF_HIP(ctx, hipMalloc((void**)&d_rt, nn));
+    SF_HIP(ctx, hipMalloc((void**)&d_x, nn * 8));
+    SF_HIP(ctx, hipMalloc((void**)&d_y, nn * 8));
+    SF_HIP(ctx, hipMemcpy(d_rt, rt.data(), nn, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(d_x, x, nn * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(apply_reference_operator, dim3(grid_elem(nn)), dim3(256), 0, ctx->stream, ctx->mp,
+                       (const uint8_t*)d_rt, (const double*)d_x, d_y);
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SF_HIP(ctx, hipMemcpy(y, d_y, nn * 8, hipMemcpyDeviceToHost));
+    hipFree(d_rt);
+    hipFree(d_x);
+    hipFree(d_y);
+    return SF_OK;
+}
+
+}  // namespace sf
+
+using namespace sf;
+
+extern "C" {
+
+int sf_poisson_setup(sf_ctx* ctx, const sf_poisson_desc* desc) {
+    SF_CHECK_CTX(ctx);
+    if (!desc) return sf_fail(ctx, SF_ERR_ARG, "sf_poisson_setup: null desc");
+    if (!desc->linear && !(desc->Te0 > 0)) return sf_fail(ctx, SF_ERR_ARG, "Te0 must be > 0 in nonlinear mode");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    return poisson_setup(ctx, desc);
+}
+
+int sf_rho_update(sf_ctx* ctx) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    return rho_update(ctx);
+}
+
+int sf_rho_set(sf_ctx* ctx, const double* rho) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready || !rho) return sf_fail(ctx, SF_ERR_ARG, "sf_rho_set: bad state or argument");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SF_HIP(ctx, hipMemcpy(ctx->d_rho, rho, (size_t)ctx->mp.ni * ctx->mp.nj * 8, hipMemcpyHostToDevice));
+    return SF_OK;
+}
+
+int sf_rho_get(sf_ctx* ctx, double* rho) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready || !rho) return sf_fail(ctx, SF_ERR_ARG, "sf_rho_get: bad state or argument");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SF_HIP(ctx, hipMemcpy(rho, ctx->d_rho, (size_t)ctx->mp.ni * ctx->mp.nj * 8, hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+int sf_poisson_solve(sf_ctx* ctx, double* phi_io, sf_solve_info* info) {
+    SF_CHECK_CTX(ctx);
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj;
+    if (phi_io) {
+        SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        SF_HIP(ctx, hipMemcpy(ctx->d_phi, phi_io, nn * 8, hipMemcpyHostToDevice));
+    }
+    int rc = poisson_solve(ctx, info);
+    if (rc < 0) return rc;
+    if (phi_io) SF_HIP(ctx, hipMemcpy(phi_io, ctx->d_phi, nn * 8, hipMemcpyDeviceToHost));
+    return rc;
+}
+
+int sf_phi_set(sf_ctx* ctx, const double* phi) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready || !phi) return sf_fail(ctx, SF_ERR_ARG, "sf_phi_set: bad state or argument");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SF_HIP(ctx, hipMemcpy(ctx->d_phi, phi, (size_t)ctx->mp.ni * ctx->mp.nj * 8, hipMemcpyHostToDevice));
+    return SF_OK;
+}
+
+int sf_phi_get(sf_ctx* ctx, double* phi) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready || !phi) return sf_fail(ctx, SF_ERR_ARG, "sf_phi_get: bad state or argument");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SF_HIP(ctx, hipMemcpy(phi, ctx->d_phi, (size_t)ctx->mp.ni * ctx->mp.nj * 8, hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+int sf_efield_update(sf_ctx* ctx) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    return efield_update(ctx);
+}
+
+int sf_efield_set(sf_ctx* ctx, const double* efi, const double* efj) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready || !efi || !efj) return sf_fail(ctx, SF_ERR_ARG, "sf_efield_set: bad state or argument");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj;
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SF_HIP(ctx, hipMemcpy(ctx->d_efi, efi, nn * 8, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(ctx->d_efj, efj, nn * 8, hipMemcpyHostToDevice));
+    return ef32_refresh(ctx);
+}
+
+int sf_efield_get(sf_ctx* ctx, double* efi, double* efj) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj;
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (efi) SF_HIP(ctx, hipMemcpy(efi, ctx->d_efi, nn * 8, hipMemcpyDeviceToHost));
+    if (efj) SF_HIP(ctx, hipMemcpy(efj, ctx->d_efj, nn * 8, hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+int sf_operator_apply(sf_ctx* ctx, const double* x, double* y) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready || !x || !y) return sf_fail(ctx, SF_ERR_ARG, "sf_operator_apply: bad state or argument");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    return operator_apply(ctx, x, y);
+}
+
+}  // extern "C"

@@ -1,0 +1,851 @@
+// Push + deposit + per-cell counting sort of one kinetic species on gfx950.
+//
+// One step of KineticMaterial.updateFields() (core/materials/KineticMaterial.java:117-197,
+// 1553-1595) runs as:
+//   pass A  (push_deposit_tile / push_deposit_tail): gather E, push, boundary processing and
+//           deposit of the 7 moment channels into LDS int64 accumulators of a 32x8-cell tile,
+//           LDS histogram of the new cells; writes the packed new cell per particle.
+//   scan    (scan_*): exclusive scan of the cell histogram -> cell ranges, cursors, mpc-sum.
+//   pass B  (push_scatter): the identical push recomputed from the same inputs, particle
+//           written to its slot of the new cell (counting sort + compaction of the dead).
+//   normalise (normalize_nodes): int64 channels -> nd, u, v, w; running sums += channels.
+// Particles are read coalesced in cell-sorted order.  The deposit and the sums are 64-bit
+// fixed point, so every density and moment is independent of the order of the additions.
+#include "sf_particle_math.h"
+
+namespace sf {
+
+// ----------------------------------------------------------------------------------------
+// deposit helpers
+// ----------------------------------------------------------------------------------------
+
+// Corner weights of Field2D.scatter (core/domain/Field2D.java:244-295) for a particle at
+// cell (i,j) + offset (di,dj); RZ uses Ruyten's eq. 4.2 for the radial weight.
+__device__ __forceinline__ void scatter_weights(const MeshP& m, int i, int j, double di, double dj, double* w) {
+    if (m.domain == SF_DOMAIN_RZ) {
+        double fi = i + di;
+        double rp = m.x0[0] + (i + 1) * m.dh[0], rm = m.x0[0] + i * m.dh[0], r = m.x0[0] + fi * m.dh[0];
+        di = 1 - (0.5 * (rp - r) * (2 * rp + 3 * rm - r) / (rp * rp - rm * rm));
+    } else if (m.domain == SF_DOMAIN_ZR) {
+        double fj = j + dj;
+        double rp = m.x0[1] + (j + 1) * m.dh[1], rm = m.x0[1] + j * m.dh[1], r = m.x0[1] + fj * m.dh[1];
+        dj = 1 - (0.5 * (rp - r) * (2 * rp + 3 * rm - r) / (rp * rp - rm * rm));
+    }
+    w[0] = (1 - di) * (1 - dj);   // (i, j)
+    w[1] = di * (1 - dj);         // (i+1, j)
+    w[2] = di * dj;               // (i+1, j+1)
+    w[3] = (1 - di) * dj;         // (i, j+1)
+}
+
+__device__ __forceinline__ void channel_values(const PushC& pc, double v0, double v1, double v2, double* val) {
+    val[0] = pc.kscale[0];
+    val[1] = v0 * pc.kscale[1];
+    val[2] = v1 * pc.kscale[2];
+    val[3] = v2 * pc.kscale[3];
+    val[4] = v0 * v0 * pc.kscale[4];
+    val[5] = v1 * v1 * pc.kscale[5];
+    val[6] = v2 * v2 * pc.kscale[6];
+}
+
+// Global-memory deposit (used for the unsorted tail and for particles that left the tile).
+__device__ __forceinline__ void deposit_global(const MeshP& m, unsigned long long* acc, size_t nn, int i, int j,
+                                               double di, double dj, const double* val) {
+    double w[4];
+    scatter_weights(m, i, j, di, dj, w);
+    const int ci[4] = {0, 1, 1, 0}, cj[4] = {0, 0, 1, 1};
+    for (int c = 0; c < 4; c++) {
+        size_t u = (size_t)(j + cj[c]) * m.ni + (i + ci[c]);
+        for (int ch = 0; ch < NCH; ch++) {
+            long long q = quant(w[c], val[ch]);
+            if (q != 0) atomicAdd(acc + ch * nn + u, (unsigned long long)q);
+        }
+    }
+}
+
+// Load of the FP32 particle state, conversion to logical coordinates.
+struct PIn {
+    int ci, cj;
+    double lc0, lc1, v0, v1, v2;
+};
+__device__ __forceinline__ PIn load_particle(const PState& s, int64_t p) {
+    PIn q;
+    int32_t c = s.cell[p];
+    q.ci = cell_i(c);
+    q.cj = cell_j(c);
+    q.lc0 = q.ci + (double)s.fx[p];
+    q.lc1 = q.cj + (double)s.fy[p];
+    q.v0 = s.vx[p];
+    q.v1 = s.vy[p];
+    q.v2 = s.vz[p];
+    return q;
+}
+
+// New cell of an alive particle (alive particles satisfy 0 <= lc <= n-1).
+__device__ __forceinline__ void new_cell(const MeshP& m, double lc0, double lc1, int& ci, int& cj) {
+    ci = (int)lc0;
+    cj = (int)lc1;
+    if (ci > m.nci - 1) ci = m.nci - 1;
+    if (cj > m.ncj - 1) cj = m.ncj - 1;
+    if (ci < 0) ci = 0;
+    if (cj < 0) cj = 0;
+}
+
+struct StatAcc {
+    long long n, p0, p1, p2, e;
+};
+
+__device__ __forceinline__ void stat_add(StatAcc& s, const PushC& pc, double v0, double v1, double v2) {
+    s.n += 1;
+    s.p0 += quant(1.0, v0 * pc.kscale[1]);
+    s.p1 += quant(1.0, v1 * pc.kscale[1]);
+    s.p2 += quant(1.0, v2 * pc.kscale[1]);
+    s.e += quant(1.0, sqrt(v0 * v0 + v1 * v1 + v2 * v2) * pc.kscale[1]);
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ void stat_flush(StatAcc s, long long* stats) {
+    s.n = wave_sum_ll(s.n);
+    s.p0 = wave_sum_ll(s.p0);
+    s.p1 = wave_sum_ll(s.p1);
+    s.p2 = wave_sum_ll(s.p2);
+    s.e = wave_sum_ll(s.e);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd((unsigned long long*)&stats[0], (unsigned long long)s.n);
+        atomicAdd((unsigned long long*)&stats[1], (unsigned long long)s.p0);
+        atomicAdd((unsigned long long*)&stats[2], (unsigned long long)s.p1);
+        atomicAdd((unsigned long long*)&stats[3], (unsigned long long)s.p2);
+        atomicAdd((unsigned long long*)&stats[4], (unsigned long long)s.e);
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// pass A, sorted particles: one workgroup per tile of TILE_W x TILE_H cells
+// ----------------------------------------------------------------------------------------
+struct TileLds {
+    unsigned long long acc[NCH][TILE_NODES];
+    int hist[TILE_CELLS];
+    float2 ef[TILE_NODES];
+    int32_t s_cell[CHUNK];
+    float s_fx[CHUNK], s_fy[CHUNK], s_vx[CHUNK], s_vy[CHUNK], s_vz[CHUNK];
+    int32_t s_key[CHUNK];
+    int seg_start[TILE_H], seg_pre[TILE_H + 1];
+};
+
+__global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD md, PushC pc, PState s,
+                                                                   const int32_t* __restrict__ offsets,
+                                                                   int32_t* __restrict__ keys,
+                                                                   unsigned long long* __restrict__ acc,
+                                                                   int32_t* __restrict__ counts, long long* stats) {
+    __shared__ TileLds L;
+    const int t = threadIdx.x;
+    const int tx = blockIdx.x % m.ntx, ty = blockIdx.x / m.ntx;
+    const int ti0 = tx * TILE_W, tj0 = ty * TILE_H;      // first cell of the tile
+    const int n_i0 = ti0 - 1, n_j0 = tj0 - 1;            // first node of the LDS tile (halo)
+    const int c_i0 = ti0 - 1, c_j0 = tj0 - 1;            // first cell of the LDS histogram
+    const size_t nn = (size_t)m.ni * m.nj;
+
+    for (int k = t; k < NCH * TILE_NODES; k += PUSH_THREADS) (&L.acc[0][0])[k] = 0ull;
+    for (int k = t; k < TILE_CELLS; k += PUSH_THREADS) L.hist[k] = 0;
+    for (int k = t; k < TILE_NODES; k += PUSH_THREADS) {
+        int i = n_i0 + k % TILE_NW, j = n_j0 + k / TILE_NW;
+        i = min(max(i, 0), m.ni - 1);
+        j = min(max(j, 0), m.nj - 1);
+        L.ef[k] = md.ef[(size_t)j * m.ni + i];
+    }
+    if (t == 0) {
+        int pre = 0;
+        for (int r = 0; r < TILE_H; r++) {
+            int row = tj0 + r;
+            int st = 0, len = 0;
+            if (row < m.ncj) {
+                int k0 = row * m.nci + ti0;
+                int k1 = row * m.nci + min(ti0 + TILE_W, m.nci);
+                st = offsets[k0];
+                len = offsets[k1] - st;
+            }
+            L.seg_start[r] = st;
+            L.seg_pre[r] = pre;
+            pre += len;
+        }
+        L.seg_pre[TILE_H] = pre;
+    }
+    __syncthreads();
+    const int ntile = L.seg_pre[TILE_H];
+
+    auto load_e = [&](int i, int j) -> float2 {
+        int li = i - n_i0, lj = j - n_j0;
+        if (li >= 0 && li < TILE_NW && lj >= 0 && lj < TILE_NH) return L.ef[lj * TILE_NW + li];
+        return md.ef[(size_t)j * m.ni + i];
+    };
+    StatAcc st{0, 0, 0, 0, 0};
+
+    for (int base = 0; base < ntile; base += CHUNK) {
+        const int cn = min(CHUNK, ntile - base);
+        // stage the chunk (coalesced loads)
+        for (int k = t; k < cn; k += PUSH_THREADS) {
+            int v = base + k;
+            int r = 0;
+            while (r < TILE_H - 1 && v >= L.seg_pre[r + 1]) r++;
+            int64_t g = (int64_t)L.seg_start[r] + (v - L.seg_pre[r]);
+            L.s_cell[k] = s.cell[g];
+            L.s_fx[k] = s.fx[g];
+            L.s_fy[k] = s.fy[g];
+            L.s_vx[k] = s.vx[g];
+            L.s_vy[k] = s.vy[g];
+            L.s_vz[k] = s.vz[g];
+        }
+        __syncthreads();
+        // process in a cell-strided order so that the lanes of a wave hit different cells
+        for (int kk = 0; kk < CHUNK / PUSH_THREADS; kk++) {
+            int k = ((t + kk * PUSH_THREADS) * CHUNK_STRIDE) & (CHUNK - 1);
+            if (k >= cn) continue;
+            int32_t c = L.s_cell[k];
+            double lc0 = cell_i(c) + (double)L.s_fx[k];
+            double lc1 = cell_j(c) + (double)L.s_fy[k];
+            double v0 = L.s_vx[k], v1 = L.s_vy[k], v2 = L.s_vz[k];
+            bool alive = push_particle(m, md, pc, load_e, lc0, lc1, v0, v1, v2);
+            int32_t key = -1;
+            if (alive) {
+                int ci, cj;
+                new_cell(m, lc0, lc1, ci, cj);
+                key = pack_cell(ci, cj);
+                float fxn = (float)(lc0 - ci), fyn = (float)(lc1 - cj);
+                float vxn = (float)v0, vyn = (float)v1, vzn = (float)v2;
+                stat_add(st, pc, vxn, vyn, vzn);
+                // histogram of the new cell
+                int hi = ci - c_i0, hj = cj - c_j0;
+                if (hi >= 0 && hi < TILE_CW && hj >= 0 && hj < TILE_CH) atomicAdd(&L.hist[hj * TILE_CW + hi], 1);
+                else atomicAdd(&counts[cj * m.nci + ci], 1);
+                // deposit from the stored (FP32) state; Field2D.scatter skips lc >= n-1
+                double di = (double)fxn, dj = (double)fyn;
+                if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
+                    double val[NCH];
+                    channel_values(pc, vxn, vyn, vzn, val);
+                    int li = ci - n_i0, lj = cj - n_j0;
+                    if (li >= 0 && li + 1 < TILE_NW && lj >= 0 && lj + 1 < TILE_NH) {
+                        double w[4];
+                        scatter_weights(m, ci, cj, di, dj, w);
+                        const int off[4] = {0, 1, TILE_NW + 1, TILE_NW};
+                        int l0 = lj * TILE_NW + li;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+#pragma unroll
+                            for (int ch = 0; ch < NCH; ch++) {
+                                long long v = quant(w[q], val[ch]);
+                                atomicAdd(&L.acc[ch][l0 + off[q]], (unsigned long long)v);
+                            }
+                        }
+                    } else {
+                        deposit_global(m, acc, nn, ci, cj, di, dj, val);
+                    }
+                }
+            }
+            L.s_key[k] = key;
+        }
+        __syncthreads();
+        for (int k = t; k < cn; k += PUSH_THREADS) {
+            int v = base + k;
+            int r = 0;
+            while (r < TILE_H - 1 && v >= L.seg_pre[r + 1]) r++;
+            int64_t g = (int64_t)L.seg_start[r] + (v - L.seg_pre[r]);
+            keys[g] = L.s_key[k];
+        }
+        __syncthreads();
+    }
+    stat_flush(st, stats);
+    // flush LDS accumulators (nonzero entries) with 64-bit integer atomics
+    for (int k = t; k < TILE_NODES; k += PUSH_THREADS) {
+        int i = n_i0 + k % TILE_NW, j = n_j0 + k / TILE_NW;
+        if (i < 0 || j < 0 || i >= m.ni || j >= m.nj) continue;
+        size_t u = (size_t)j * m.ni + i;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) {
+            unsigned long long v = L.acc[ch][k];
+            if (v != 0ull) atomicAdd(acc + ch * nn + u, v);
+        }
+    }
+    for (int k = t; k < TILE_CELLS; k += PUSH_THREADS) {
+        int h = L.hist[k];
+        if (h == 0) continue;
+        int ci = c_i0 + k % TILE_CW, cj = c_j0 + k / TILE_CW;
+        atomicAdd(&counts[cj * m.nci + ci], h);
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// pass A, unsorted tail (particles added since the last sort): global atomics
+// ----------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, PushC pc, PState s, int64_t p0,
+                                                         int64_t p1, int32_t* __restrict__ keys,
+                                                         unsigned long long* __restrict__ acc,
+                                                         int32_t* __restrict__ counts, long long* stats) {
+    const size_t nn = (size_t)m.ni * m.nj;
+    auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
+    StatAcc st{0, 0, 0, 0, 0};
+    for (int64_t p = p0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < p1;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        PIn q = load_particle(s, p);
+        bool alive = push_particle(m, md, pc, load_e, q.lc0, q.lc1, q.v0, q.v1, q.v2);
+        int32_t key = -1;
+        if (alive) {
+            int ci, cj;
+            new_cell(m, q.lc0, q.lc1, ci, cj);
+            key = pack_cell(ci, cj);
+            float fxn = (float)(q.lc0 - ci), fyn = (float)(q.lc1 - cj);
+            float vxn = (float)q.v0, vyn = (float)q.v1, vzn = (float)q.v2;
+            stat_add(st, pc, vxn, vyn, vzn);
+            atomicAdd(&counts[cj * m.nci + ci], 1);
+            double di = fxn, dj = fyn;
+            if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
+                double val[NCH];
+                channel_values(pc, vxn, vyn, vzn, val);
+                deposit_global(m, acc, nn, ci, cj, di, dj, val);
+            }
+        }
+        keys[p] = key;
+    }
+    stat_flush(st, stats);
+}
+
+// ----------------------------------------------------------------------------------------
+// counting-sort scan over cells: counts -> offsets (exclusive, ncells+1), cursor = offsets,
+// mpc-sum += counts, counts = 0
+// ----------------------------------------------------------------------------------------
+constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int& total) {
+    // wave inclusive scan
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        int nw = blockDim.x >> 6;
+        int w = lane < nw ? sh[lane] : 0;
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(w, o);
+            if (lane >= o) w += y;
+        }
+        if (lane < nw) sh[lane] = w;
+    }
+    __syncthreads();
+    int wave_prefix = wid > 0 ? sh[wid - 1] : 0;
+    total = sh[(blockDim.x >> 6) - 1];
+    __syncthreads();
+    return wave_prefix + x - v;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) scan_reduce(const int32_t* __restrict__ counts, int n,
+                                                            int32_t* __restrict__ blocksum) {
+    __shared__ int sh[32];
+    int base = blockIdx.x * SCAN_TILE;
+    int s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int idx = base + threadIdx.x * SCAN_ITEMS + k;
+        if (idx < n) s += counts[idx];
+    }
+    int total;
+    block_exclusive_scan(s, sh, total);
+    if (threadIdx.x == 0) blocksum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) scan_blocks(int32_t* __restrict__ blocksum, int nb) {
+    __shared__ int sh[32];
+    int carry = 0;
+    for (int base = 0; base < nb; base += SCAN_THREADS) {
+        int idx = base + threadIdx.x;
+        int v = idx < nb ? blocksum[idx] : 0;
+        int total;
+        int ex = block_exclusive_scan(v, sh, total);
+        if (idx < nb) blocksum[idx] = ex + carry;
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) blocksum[nb] = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) scan_write(int32_t* __restrict__ counts, int n,
+                                                           const int32_t* __restrict__ blocksum,
+                                                           int32_t* __restrict__ offsets, int32_t* __restrict__ cursor,
+                                                           long long* __restrict__ mpc_sum, int nb) {
+    __shared__ int sh[32];
+    int base = blockIdx.x * SCAN_TILE;
+    int v[SCAN_ITEMS];
+    int s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int idx = base + threadIdx.x * SCAN_ITEMS + k;
+        v[k] = idx < n ? counts[idx] : 0;
+        s += v[k];
+    }
+    int total;
+    int ex = block_exclusive_scan(s, sh, total) + blocksum[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int idx = base + threadIdx.x * SCAN_ITEMS + k;
+        if (idx < n) {
+            offsets[idx] = ex;
+            cursor[idx] = ex;
+            if (mpc_sum) mpc_sum[idx] += v[k];
+            counts[idx] = 0;
+        }
+        ex += v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) offsets[n] = blocksum[nb];
+}
+
+// ----------------------------------------------------------------------------------------
+// pass B: recompute the push, write each particle to its slot in the new cell order
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ int64_t wave_rank(bool valid, int key, int32_t* cursor) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long active = __ballot(valid);
+    int64_t dst = -1;
+    while (active) {
+        int leader = __ffsll((long long)active) - 1;
+        int lk = __shfl(key, leader);
+        unsigned long long mm = __ballot(valid && key == lk);
+        int b = 0;
+        if (lane == leader) b = atomicAdd(&cursor[lk], __popcll(mm));
+        b = __shfl(b, leader);
+        if (valid && key == lk) dst = b + __popcll(mm & ((1ull << lane) - 1ull));
+        active &= ~mm;
+    }
+    return dst;
+}
+
+template <bool IDS>
+__global__ void __launch_bounds__(256) push_scatter(MeshP m, MeshD md, PushC pc, PState s, PState d, int64_t np,
+                                                    const int32_t* __restrict__ keys, int32_t* __restrict__ cursor) {
+    auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nloop = (np + stride - 1) / stride * stride;   // uniform trip count for the wave ops
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nloop; p += stride) {
+        bool valid = p < np;
+        int32_t key = valid ? keys[p] : -1;
+        valid = valid && key >= 0;
+        int ci = 0, cj = 0, lkey = 0;
+        float fxn = 0, fyn = 0, vxn = 0, vyn = 0, vzn = 0;
+        int32_t idv = 0;
+        if (valid) {
+            PIn q = load_particle(s, p);
+            push_particle(m, md, pc, load_e, q.lc0, q.lc1, q.v0, q.v1, q.v2);
+            ci = cell_i(key);
+            cj = cell_j(key);
+            lkey = cj * m.nci + ci;
+            fxn = (float)(q.lc0 - ci);
+            fyn = (float)(q.lc1 - cj);
+            vxn = (float)q.v0;
+            vyn = (float)q.v1;
+            vzn = (float)q.v2;
+            if (IDS) idv = s.id[p];
+        }
+        int64_t dst = wave_rank(valid, lkey, cursor);
+        if (valid) {
+            d.cell[dst] = key;
+            d.fx[dst] = fxn;
+            d.fy[dst] = fyn;
+            d.vx[dst] = vxn;
+            d.vy[dst] = vyn;
+            d.vz[dst] = vzn;
+            if (IDS) d.id[dst] = idv;
+        }
+    }
+}
+
+// sort without a push (after particles were loaded): key = current cell
+__global__ void __launch_bounds__(256) count_current(MeshP m, PState s, int64_t np, int32_t* __restrict__ keys,
+                                                     int32_t* __restrict__ counts) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int32_t c = s.cell[p];
+        keys[p] = c;
+        atomicAdd(&counts[cell_j(c) * m.nci + cell_i(c)], 1);
+    }
+}
+
+template <bool IDS>
+__global__ void __launch_bounds__(256) copy_scatter(MeshP m, PState s, PState d, int64_t np,
+                                                    const int32_t* __restrict__ keys, int32_t* __restrict__ cursor) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nloop = (np + stride - 1) / stride * stride;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nloop; p += stride) {
+        bool valid = p < np;
+        int32_t key = valid ? keys[p] : -1;
+        valid = valid && key >= 0;
+        int lkey = valid ? cell_j(key) * m.nci + cell_i(key) : 0;
+        int64_t dst = wave_rank(valid, lkey, cursor);
+        if (valid) {
+            d.cell[dst] = key;
+            d.fx[dst] = s.fx[p];
+            d.fy[dst] = s.fy[p];
+            d.vx[dst] = s.vx[p];
+            d.vy[dst] = s.vy[p];
+            d.vz[dst] = s.vz[p];
+            if (IDS) d.id[dst] = s.id[p];
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// node-wise normalisation (KineticMaterial.updateFields tail, Field2D.divideByField and
+// scaleByVol, Material.update nd_back) and sample accumulation (updateSamples)
+// ----------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) normalize_nodes(int64_t nn, unsigned long long* __restrict__ acc,
+                                                       long long* __restrict__ sums, const double* __restrict__ vol,
+                                                       double* __restrict__ den, double inv_kw, double inv_kv,
+                                                       double spwt, double nd_back, int samples) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        long long a[NCH];
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) {
+            a[ch] = (long long)acc[ch * nn + u];
+            acc[ch * nn + u] = 0ull;
+        }
+        if (samples) {
+#pragma unroll
+            for (int ch = 0; ch < NCH; ch++) sums[ch * nn + u] += a[ch];
+        }
+        double wsum = (double)a[0] * inv_kw;
+        double nd = wsum * spwt / vol[u];
+        double uu = 0, vv = 0, ww = 0;
+        if (a[0] != 0) {
+            uu = ((double)a[1] * inv_kv) / wsum;
+            vv = ((double)a[2] * inv_kv) / wsum;
+            ww = ((double)a[3] * inv_kv) / wsum;
+        }
+        den[u] = nd + nd_back;
+        den[nn + u] = uu;
+        den[2 * nn + u] = vv;
+        den[3 * nn + u] = ww;
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// particle addition (KineticMaterial.addParticle) and device preload
+// ----------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) add_particles_kernel(MeshP m, MeshD md, PState d, int64_t base, int64_t n,
+                                                            const double* __restrict__ in, double qm_half_dt,
+                                                            int rewind, const int32_t* __restrict__ ids,
+                                                            int id0, int with_ids) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        double x = in[k], y = in[n + k], u = in[2 * n + k], v = in[3 * n + k], w = in[4 * n + k];
+        double lc0 = (x - m.x0[0]) * m.inv_dh[0];
+        double lc1 = (y - m.x0[1]) * m.inv_dh[1];
+        if (rewind) {
+            if (lc0 >= m.ni) lc0 = m.ni - 1;
+            if (lc1 >= m.nj) lc1 = m.nj - 1;
+            double ex, ey;
+            auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
+            gather_e(m, lc0, lc1, load_e, ex, ey);
+            u += qm_half_dt * ex;        // part.dt = -dt/2 (KineticMaterial.java:777-791)
+            v += qm_half_dt * ey;
+        }
+        int ci = (int)lc0, cj = (int)lc1;
+        ci = min(max(ci, 0), m.nci - 1);
+        cj = min(max(cj, 0), m.ncj - 1);
+        int64_t p = base + k;
+        d.cell[p] = pack_cell(ci, cj);
+        d.fx[p] = (float)(lc0 - ci);
+        d.fy[p] = (float)(lc1 - cj);
+        d.vx[p] = (float)u;
+        d.vy[p] = (float)v;
+        d.vz[p] = (float)w;
+        if (with_ids) d.id[p] = ids ? ids[k] : id0 + (int)k;
+    }
+}
+
+// counter-based generator for the preload: splitmix64 finaliser over (seed, particle, draw)
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t seed, uint64_t p, uint32_t k) {
+    uint64_t h = mix64(seed ^ mix64(p * 0x100000001B3ULL + k));
+    return (double)(h >> 11) * 0x1.0p-53;
+}
+
+__global__ void __launch_bounds__(256) preload_kernel(MeshP m, PState d, int64_t base, int64_t n,
+                                                      const int32_t* __restrict__ open_cells,
+                                                      const double* __restrict__ cdf, int nopen, double dx_,
+                                                      double dy_, double dz_, double vth, uint64_t seed,
+                                                      int with_ids, int id0) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        double u = u01(seed, k, 0) * cdf[nopen];
+        int lo = 0, hi = nopen - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (cdf[mid + 1] <= u) lo = mid + 1;
+            else hi = mid;
+        }
+        int32_t c = open_cells[lo];
+        int ci = c & 0xFFFF, cj = c >> 16;
+        double fx = u01(seed, k, 1), fy = u01(seed, k, 2);
+        if (m.domain == SF_DOMAIN_RZ) {
+            // uniform in volume: r^2 uniform between the cell faces
+            double r1 = m.x0[0] + ci * m.dh[0], r2 = r1 + m.dh[0];
+            double r = sqrt(r1 * r1 + fx * (r2 * r2 - r1 * r1));
+            fx = (r - r1) * m.inv_dh[0];
+        }
+        double vel[3];
+        for (int a = 0; a < 3; a++) {     // Utils.SampleMaxw1D with M = 12, widened by 1.2
+            double s = 0;
+            for (int q = 0; q < 12; q++) s += u01(seed, k, 3 + a * 12 + q);
+            vel[a] = (vth * 1.2) * sqrt(0.5) * (s - 6.0);
+        }
+        int64_t p = base + k;
+        d.cell[p] = pack_cell(ci, cj);
+        d.fx[p] = (float)fx;
+        d.fy[p] = (float)fy;
+        d.vx[p] = (float)(vel[0] + dx_);
+        d.vy[p] = (float)(vel[1] + dy_);
+        d.vz[p] = (float)(vel[2] + dz_);
+        if (with_ids) d.id[p] = id0 + (int)k;
+    }
+}
+
+__global__ void ef32_kernel(int64_t nn, const double* __restrict__ efi, const double* __restrict__ efj,
+                            float2* __restrict__ ef) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
+        ef[u] = make_float2((float)efi[u], (float)efj[u]);
+}
+
+// ----------------------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------------------
+static int grid_for(int64_t n, int threads, int max_blocks = 65536) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > max_blocks) b = max_blocks;
+    return (int)b;
+}
+
+static PushC make_pushc(sf_ctx* ctx, Species* sp) {
+    PushC pc;
+    pc.qm_dt = sp->qm * ctx->dt;
+    pc.dt = ctx->dt;
+    for (int c = 0; c < NCH; c++) pc.kscale[c] = sp->kscale[c];
+    pc.spwt = sp->desc.spwt;
+    return pc;
+}
+
+static int run_scan(sf_ctx* ctx, Species* sp, bool samples) {
+    const int ncells = ctx->mp.nci * ctx->mp.ncj;
+    const int nb = (ncells + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, ctx->stream, sp->counts, ncells, sp->blocksum);
+    hipLaunchKernelGGL(scan_blocks, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, sp->blocksum, nb);
+    hipLaunchKernelGGL(scan_write, dim3(nb), dim3(SCAN_THREADS), 0, ctx->stream, sp->counts, ncells, sp->blocksum,
+                       sp->offsets, sp->cursor, samples ? sp->mpc_sum : (long long*)nullptr, nb);
+    SF_HIP(ctx, hipGetLastError());
+    return SF_OK;
+}
+
+static int read_total(sf_ctx* ctx, Species* sp, int64_t* total) {
+    const int ncells = ctx->mp.nci * ctx->mp.ncj;
+    int32_t t = 0;
+    SF_HIP(ctx, hipMemcpyAsync(&t, sp->offsets + ncells, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *total = t;
+    return SF_OK;
+}
+
+int sort_species(sf_ctx* ctx, Species* sp) {
+    if (sp->np == 0) { sp->nsorted = 0; return SF_OK; }
+    PState s = sp->buf[sp->cur], d = sp->buf[1 - sp->cur];
+    hipLaunchKernelGGL(count_current, dim3(grid_for(sp->np, 256)), dim3(256), 0, ctx->stream, ctx->mp, s, sp->np,
+                       sp->key, sp->counts);
+    int rc = run_scan(ctx, sp, false);
+    if (rc) return rc;
+    if (sp->desc.track_ids)
+        hipLaunchKernelGGL(copy_scatter<true>, dim3(grid_for(sp->np, 256)), dim3(256), 0, ctx->stream, ctx->mp, s,
+                           d, sp->np, sp->key, sp->cursor);
+    else
+        hipLaunchKernelGGL(copy_scatter<false>, dim3(grid_for(sp->np, 256)), dim3(256), 0, ctx->stream, ctx->mp, s,
+                           d, sp->np, sp->key, sp->cursor);
+    SF_HIP(ctx, hipGetLastError());
+    int64_t total;
+    rc = read_total(ctx, sp, &total);
+    if (rc) return rc;
+    if (total != sp->np) return sf_fail(ctx, SF_ERR_STATE, "sort lost particles (%lld of %lld)", (long long)total,
+                                        (long long)sp->np);
+    sp->cur = 1 - sp->cur;
+    sp->nsorted = sp->np;
+    return SF_OK;
+}
+
+int push_deposit(sf_ctx* ctx, Species* sp) {
+    if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
+    const MeshP& m = ctx->mp;
+    MeshD md = ctx->meshd();
+    PushC pc = make_pushc(ctx, sp);
+    PState s = sp->buf[sp->cur], d = sp->buf[1 - sp->cur];
+    const int64_t nn = (int64_t)m.ni * m.nj;
+    SF_HIP(ctx, hipMemsetAsync(sp->stats, 0, 8 * sizeof(long long), ctx->stream));
+    if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if (sp->nsorted > 0)
+        hipLaunchKernelGGL(push_deposit_tile, dim3(m.ntx * m.nty), dim3(PUSH_THREADS), 0, ctx->stream, m, md, pc, s,
+                           sp->offsets, sp->key, sp->acc, sp->counts, sp->stats);
+    if (sp->np > sp->nsorted)
+        hipLaunchKernelGGL(push_deposit_tail, dim3(grid_for(sp->np - sp->nsorted, 256, 4096)), dim3(256), 0,
+                           ctx->stream, m, md, pc, s, sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
+    SF_HIP(ctx, hipGetLastError());
+    if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    int rc = run_scan(ctx, sp, true);
+    if (rc) return rc;
+    if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    if (sp->np > 0) {
+        if (sp->desc.track_ids)
+            hipLaunchKernelGGL(push_scatter<true>, dim3(grid_for(sp->np, 256, 1 << 20)), dim3(256), 0, ctx->stream, m,
+                               md, pc, s, d, sp->np, sp->key, sp->cursor);
+        else
+            hipLaunchKernelGGL(push_scatter<false>, dim3(grid_for(sp->np, 256, 1 << 20)), dim3(256), 0, ctx->stream, m,
+                               md, pc, s, d, sp->np, sp->key, sp->cursor);
+        SF_HIP(ctx, hipGetLastError());
+    }
+    if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    hipLaunchKernelGGL(normalize_nodes, dim3(grid_for(nn, 256, 8192)), dim3(256), 0, ctx->stream, nn, sp->acc,
+                       sp->sums, (const double*)ctx->d_node_vol, sp->den, 1.0 / sp->kscale[0], 1.0 / sp->kscale[1],
+                       sp->desc.spwt, sp->desc.nd_back, 1);
+    SF_HIP(ctx, hipGetLastError());
+    if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    sp->num_samples++;
+    int64_t total;
+    rc = read_total(ctx, sp, &total);
+    if (rc) return rc;
+    // mover sums (KineticMaterial.java:247-259)
+    long long st[8];
+    SF_HIP(ctx, hipMemcpy(st, sp->stats, 8 * sizeof(long long), hipMemcpyDeviceToHost));
+    double inv_kv = 1.0 / sp->kscale[1];
+    sp->stats_host[0] = (double)st[0] * sp->desc.spwt * sp->mass;
+    sp->stats_host[1] = (double)st[1] * inv_kv * sp->desc.spwt * sp->mass;
+    sp->stats_host[2] = (double)st[2] * inv_kv * sp->desc.spwt * sp->mass;
+    sp->stats_host[3] = (double)st[3] * inv_kv * sp->desc.spwt * sp->mass;
+    sp->stats_host[4] = (double)st[4] * inv_kv * sp->desc.spwt * sp->mass;
+    if (ctx->timing) {
+        float a, b, c, e;
+        hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+        hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
+        hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
+        hipEventElapsedTime(&e, ctx->ev[3], ctx->ev[4]);
+        ctx->last_ms[0] = a;
+        ctx->last_ms[1] = b;
+        ctx->last_ms[2] = c;
+        ctx->last_ms[3] = e;
+    }
+    sp->np = total;
+    sp->nsorted = total;
+    sp->cur = 1 - sp->cur;
+    return SF_OK;
+}
+
+int add_particles_device(sf_ctx* ctx, Species* sp, int64_t n, const double* x, const double* y, const double* u,
+                         const double* v, const double* w, const int* ids, int mode) {
+    if (n <= 0) return SF_OK;
+    if (sp->np + n > sp->capacity)
+        return sf_fail(ctx, SF_ERR_OOM, "species capacity %lld exceeded (have %lld, adding %lld)",
+                       (long long)sp->capacity, (long long)sp->np, (long long)n);
+    // MeshData.addParticle drops non-finite velocities (KineticMaterial.java:1356-1361); also
+    // only particles inside the mesh are accepted (DomainModule.getMesh, DomainModule.java:106-117)
+    std::vector<double> h;
+    h.reserve(5 * n);
+    std::vector<int> hid;
+    std::vector<double> cols[5];
+    const MeshP& m = ctx->mp;
+    const double xd0 = m.x0[0] + (m.ni - 1) * m.dh[0], xd1 = m.x0[1] + (m.nj - 1) * m.dh[1];
+    int64_t kept = 0;
+    for (int c = 0; c < 5; c++) cols[c].reserve(n);
+    for (int64_t k = 0; k < n; k++) {
+        double vx = u[k], vy = v[k], vz = w[k];
+        if (!(std::isfinite(vx) && std::isfinite(vy) && std::isfinite(vz))) continue;
+        double lc0 = (x[k] - m.x0[0]) / m.dh[0], lc1 = (y[k] - m.x0[1]) / m.dh[1];
+        bool strict = x[k] >= m.x0[0] && x[k] < xd0 && y[k] >= m.x0[1] && y[k] < xd1;
+        bool loose = !(lc0 < -FLT_EPS || lc1 < -FLT_EPS || lc0 > (m.ni - 1 + FLT_EPS) || lc1 > (m.nj - 1 + FLT_EPS));
+        if (!(strict || loose)) continue;
+        cols[0].push_back(x[k]);
+        cols[1].push_back(y[k]);
+        cols[2].push_back(vx);
+        cols[3].push_back(vy);
+        cols[4].push_back(vz);
+        if (ids) hid.push_back(ids[k]);
+        kept++;
+    }
+    if (kept == 0) return SF_OK;
+    for (int c = 0; c < 5; c++) h.insert(h.end(), cols[c].begin(), cols[c].end());
+    double* d_in = nullptr;
+    int32_t* d_ids = nullptr;
+    SF_HIP(ctx, hipMallocAsync((void**)&d_in, 5 * kept * sizeof(double), ctx->stream));
+    SF_HIP(ctx, hipMemcpyAsync(d_in, h.data(), 5 * kept * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (ids && sp->desc.track_ids) {
+        SF_HIP(ctx, hipMallocAsync((void**)&d_ids, kept * sizeof(int32_t), ctx->stream));
+        SF_HIP(ctx, hipMemcpyAsync(d_ids, hid.data(), kept * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    }
+    hipLaunchKernelGGL(add_particles_kernel, dim3(grid_for(kept, 256, 4096)), dim3(256), 0, ctx->stream, ctx->mp,
+                       ctx->meshd(), sp->buf[sp->cur], sp->np, kept, (const double*)d_in, -0.5 * ctx->dt * sp->qm,
+                       mode, (const int32_t*)d_ids, sp->id_counter, sp->desc.track_ids);
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipFreeAsync(d_in, ctx->stream));
+    if (d_ids) SF_HIP(ctx, hipFreeAsync(d_ids, ctx->stream));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!(ids && sp->desc.track_ids)) sp->id_counter += (int)kept;
+    sp->np += kept;
+    return SF_OK;
+}
+
+int preload(sf_ctx* ctx, Species* sp, int64_t n, const double* drift, double temp, uint64_t seed) {
+    if (n <= 0) return SF_OK;
+    if (sp->np + n > sp->capacity) return sf_fail(ctx, SF_ERR_OOM, "species capacity exceeded by preload");
+    const MeshP& m = ctx->mp;
+    std::vector<int32_t> open;
+    std::vector<double> cdf(1, 0.0);
+    for (int j = 0; j < m.ncj; j++)
+        for (int i = 0; i < m.nci; i++) {
+            auto nt = [&](int a, int b) { return ctx->h_node_type[(size_t)b * m.ni + a]; };
+            if (nt(i, j) == SF_NODE_OPEN && nt(i + 1, j) == SF_NODE_OPEN && nt(i + 1, j + 1) == SF_NODE_OPEN &&
+                nt(i, j + 1) == SF_NODE_OPEN) {
+                open.push_back((j << 16) | i);
+                double wgt = 1.0;
+                if (m.domain == SF_DOMAIN_RZ) {
+                    double r1 = m.x0[0] + i * m.dh[0], r2 = r1 + m.dh[0];
+                    wgt = r2 * r2 - r1 * r1;
+                }
+                cdf.push_back(cdf.back() + wgt);
+            }
+        }
+    if (open.empty()) return sf_fail(ctx, SF_ERR_STATE, "preload: no open cells");
+    int32_t* d_open = nullptr;
+    double* d_cdf = nullptr;
+    SF_HIP(ctx, hipMalloc((void**)&d_open, open.size() * sizeof(int32_t)));
+    SF_HIP(ctx, hipMalloc((void**)&d_cdf, cdf.size() * sizeof(double)));
+    SF_HIP(ctx, hipMemcpy(d_open, open.data(), open.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(d_cdf, cdf.data(), cdf.size() * sizeof(double), hipMemcpyHostToDevice));
+    double vth = sqrt(2 * K_B * temp / sp->mass);   // Utils.computeVth
+    hipLaunchKernelGGL(preload_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, ctx->stream, ctx->mp,
+                       sp->buf[sp->cur], sp->np, n, (const int32_t*)d_open, (const double*)d_cdf, (int)open.size(),
+                       drift[0], drift[1], drift[2], vth, seed, sp->desc.track_ids, sp->id_counter);
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(d_open);
+    hipFree(d_cdf);
+    sp->id_counter += (int)n;
+    sp->np += n;
+    return SF_OK;
+}
+
+int ef32_refresh(sf_ctx* ctx) {
+    int64_t nn = (int64_t)ctx->mp.ni * ctx->mp.nj;
+    hipLaunchKernelGGL(ef32_kernel, dim3(grid_for(nn, 256, 8192)), dim3(256), 0, ctx->stream, nn,
+                       (const double*)ctx->d_efi, (const double*)ctx->d_efj, ctx->d_ef32);
+    SF_HIP(ctx, hipGetLastError());
+    return SF_OK;
+}
+
+}  // namespace sf

@@ -1,0 +1,63 @@
+"""Shared set-up for the parity tests: geometry of the benchmark decks on a host-only context,
+the oracle (checker) over the same geometry, and seeded particle sets."""
+import ctypes as C
+
+import numpy as np
+
+from starfish_amd import deck as D
+from starfish_amd.plugin import Context, Geometry
+
+AMU = D.AMU
+QE = D.QE
+
+
+def geometry(deck, seed=None):
+    """Builds the mesh geometry on a host-only context; returns (geo, rng_state_after)."""
+    ctx = Context(-1)
+    ctx.call("sf_ctx_set_rng", deck.seed if seed is None else seed, 0)
+    g = Geometry(ctx, deck)
+    st = C.c_int64()
+    ctx.call("sf_ctx_get_rng_state", C.byref(st))
+    ctx.close()
+    return g, st.value
+
+
+def oracle_for(deck, geo, rng_state=None, nthreads=1):
+    import oracle as O
+    o = O.Oracle.from_geometry(geo, seed=deck.seed, nthreads=nthreads, dt=deck.dt)
+    if rng_state is not None:
+        O.set_rng_state(o, rng_state)
+    for m in deck.materials:
+        if m.kind.lower() in ("kinetic", "kinetic_hip"):
+            o.add_species(m.charge, m.molwt, m.spwt, m.nd_back)
+    if deck.solver is not None:
+        o.set_solver(deck.solver)
+    return o
+
+
+def open_cell_particles(deck, geo, n, seed, drift=(7000.0, 0.0, 0.0), temp=1000.0, molwt=16.0):
+    """n particles in open cells, stored values exactly representable in the GPU layout
+    (FP32 in-cell offsets and FP32 velocities)."""
+    rng = np.random.default_rng(seed)
+    m = deck.mesh
+    nt = geo.node_type.reshape(m.nj, m.ni)
+    openc = (nt[:-1, :-1] == -1) & (nt[1:, :-1] == -1) & (nt[:-1, 1:] == -1) & (nt[1:, 1:] == -1)
+    cj, ci = np.nonzero(openc)
+    k = rng.integers(0, len(ci), n)
+    fx = rng.random(n).astype(np.float32).astype(np.float64)
+    fy = rng.random(n).astype(np.float32).astype(np.float64)
+    lc0 = ci[k] + fx
+    lc1 = cj[k] + fy
+    x = m.x0[0] + lc0 * m.dh[0]
+    y = m.x0[1] + lc1 * m.dh[1]
+    vth = np.sqrt(2 * 1.3806503e-23 * temp / (molwt * AMU))
+    v = rng.normal(size=(n, 3)) * vth + np.array(drift)
+    v = v.astype(np.float32).astype(np.float64)
+    return x, y, v, lc0, lc1
+
+
+def rel_err(a, b, scale=None):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    s = scale if scale is not None else max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / s
