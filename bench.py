@@ -1,0 +1,181 @@
+#!/usr/bin/env python
+"""Benchmark of the Starfish ES-PIC hot path on one MI355X.
+
+Workload (default): O+ flow past a -100 V cylinder on a 1024x512-node XY mesh (dx = 1e-3 m),
+2e7 macroparticles preloaded as a drifting Maxwellian (7000 m/s, 1000 K) plus left-edge
+injection at 1e20 m^-2 s^-1, Boltzmann electrons (n0 = 1e10 m^-3, kTe = 2 eV), dt = 1e-7 s.
+A step is one Starfish main-loop iteration: source injection, push + deposit + counting sort,
+rho, Newton-PCG potential solve to a 1e-8 relative residual, E = -grad(phi).
+
+Output: one JSON line.  `value` is particles pushed and deposited per second of device time of
+the particle phase (push/deposit pass, cell scan, push/scatter pass, normalisation), summed over
+the timed steps with HIP events on the library's stream.  The full-step rate (all phases, host
+clock) and the potential-solve cost are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+BYTES_PUSH_DEPOSIT = 28.0   # pass A: read 24 B of state, write the 4 B new-cell key
+BYTES_PUSH_SCATTER = 52.0   # pass B: read 24 B state + 4 B key, write 24 B state (counting sort)
+BYTES_STATE = 48.0          # read + write of the 24 B particle state (minimal push + deposit)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--particles", type=float, default=2e7)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(sim, deck, threads, target_s=15.0):
+    """Times the oracle's restatement of the reference CPU path (threaded mover, serial deposit and
+    moment sampling) on a sample of the same particle population, mesh and E field."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import oracle_for
+    geo = sim.geo
+    o = oracle_for(deck, geo, None, nthreads=threads)
+    ei, ej = sim.efield()
+    o.set_field("efi", ei)
+    o.set_field("efj", ej)
+    mat = sim.materials[0]
+    p = mat.particles()
+    n_all = len(p["x"])
+    n = min(n_all, 2_000_000)
+    sel = np.linspace(0, n_all - 1, n).astype(np.int64)
+    pos = np.stack([p["x"][sel], p["y"][sel], np.zeros(n)], 1)
+    vel = np.stack([p["u"][sel], p["v"][sel], p["w"][sel]], 1)
+    o.load_particles(0, pos, vel)
+    pushed, t_total, steps = 0, 0.0, 0
+    while t_total < target_s and steps < 50:
+        npre = o.num_particles(0)
+        t0 = time.perf_counter()
+        o.move()
+        o.update_species_fields()
+        t_total += time.perf_counter() - t0
+        pushed += npre
+        steps += 1
+    o.close()
+    return {"value": pushed / t_total, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": "%d particles of the same population on the same 1024x512 mesh and E field, %d steps of "
+                      "KineticMaterial.moveParticles (%d mover threads) + serial updateFields/updateSamples, "
+                      "%.1f s" % (n, steps, threads, t_total)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              "(--gpus %d is not supported; multi-GPU sharding is described in DESIGN.md)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import ctypes as C
+    import numpy as np
+    from starfish_amd import deck as D
+    from starfish_amd.plugin import Simulation
+
+    npart = int(args.particles)
+    deck = D.config_c2(steps=args.steps, particles=npart)
+    sim = Simulation(deck, capacity=int(npart * 1.6) + 1_000_000)
+    mat = sim.materials[0]
+    nparts, drift, temp = deck.preload
+    mat.preload(nparts, drift, temp, seed=1)
+    t0 = time.perf_counter()
+    rc = sim.initial_solve()
+    cold = sim.solve_infos[-1]
+    cold_info = {"ms": cold.ms, "newton": cold.newton_iterations, "pcg_iterations": cold.pcg_iterations,
+                 "rel_residual": cold.rel_residual, "converged": rc == 0, "wall_s": time.perf_counter() - t0}
+    sim.ctx.call("sf_set_timing", 1)
+    for _ in range(args.warmup):
+        sim.step()
+    sim.ctx.call("sf_ctx_synchronize")
+    kern = np.zeros(4)
+    pushed = 0
+    solve_ms, pcg_its, newton_its = [], [], []
+    worst_res = 0.0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        npre = mat.getNp()
+        inj = C.c_int64()
+        sim.ctx.call("sf_sources_sample", C.byref(inj))
+        mat.update()
+        tm = np.zeros(4)
+        sim.ctx.call("sf_last_timings", tm.ctypes.data_as(C.POINTER(C.c_double)))
+        kern += tm
+        pushed += npre + inj.value
+        sim.update_fields()
+        info = sim.solve_infos[-1]
+        solve_ms.append(info.ms)
+        pcg_its.append(info.pcg_iterations)
+        newton_its.append(info.newton_iterations)
+        worst_res = max(worst_res, info.rel_residual)
+        sim.it += 1
+    sim.ctx.call("sf_ctx_synchronize")
+    wall = time.perf_counter() - t_start
+    particle_s = kern.sum() * 1e-3
+    value = pushed / particle_s
+    per_launch = pushed / args.steps
+    tA, tB = kern[0] / args.steps * 1e-3, kern[2] / args.steps * 1e-3
+    rA = BYTES_PUSH_DEPOSIT * per_launch / tA / 1e9
+    rB = BYTES_PUSH_SCATTER * per_launch / tB / 1e9
+    if tB >= tA:
+        dom = {"kernel": "push_scatter", "bytes_per_particle": BYTES_PUSH_SCATTER, "achieved": rB,
+               "ms_per_launch": tB * 1e3}
+    else:
+        dom = {"kernel": "push_deposit_tile", "bytes_per_particle": BYTES_PUSH_DEPOSIT, "achieved": rA,
+               "ms_per_launch": tA * 1e3}
+    roofline = {"bound": "hbm", "achieved": round(dom["achieved"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(dom["achieved"] / PEAK_HBM_GBS, 4), "traffic": None,
+                "kernel": dom["kernel"], "bytes_per_particle": dom["bytes_per_particle"],
+                "ms_per_launch": round(dom["ms_per_launch"], 4),
+                "push_deposit_tile_GBs": round(rA, 1), "push_scatter_GBs": round(rB, 1),
+                "particle_phase_state_GBs": round(BYTES_STATE * pushed / particle_s / 1e9, 1)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(sim, deck, thr)
+    out = {
+        "metric": "particle push+deposit/s (2D ES-PIC, 1024x512 XY, 2e7 particles)",
+        "value": value,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (preloaded drifting Maxwellian, seed 1, plus boundary injection)",
+        "config": {"workload": "ES-PIC O+ past a -100 V cylinder, XY 1024x512 nodes dx=1e-3 m, "
+                               "%d particles preloaded, Boltzmann electrons n0=1e10 kTe=2 eV, dt=1e-7 s" % npart,
+                   "particles_final": mat.getNp(), "state": "int32 cell + f32 offsets + f32 velocity (24 B)",
+                   "deposit": "int64 fixed point, 7 moment channels", "solver": "Newton + FP32 Jacobi-PCG "
+                   "with FP64 residual refinement, rel. residual 1e-8 every step"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "full_step_particles_per_s": pushed / wall,
+        "particle_phase_ms": {"push_deposit": kern[0] / args.steps, "scan": kern[1] / args.steps,
+                              "push_scatter": kern[2] / args.steps, "normalize": kern[3] / args.steps},
+        "pcg": {"ms_per_solve": float(np.mean(solve_ms)), "pcg_iterations_per_solve": float(np.mean(pcg_its)),
+                "newton_per_solve": float(np.mean(newton_its)), "worst_rel_residual": worst_res,
+                "rel_tol": deck.solver.rel_tol, "cold_start": cold_info, "learned_guess": None},
+    }
+    sim.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

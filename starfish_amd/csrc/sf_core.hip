@@ -335,7 +335,6 @@ int sf_particles_add(sf_ctx* ctx, int species, int64_t n, const double* x, const
     if (n < 0 || (n > 0 && (!x || !y || !u || !v || !w))) return sf_fail(ctx, SF_ERR_ARG, "sf_particles_add: bad arrays");
     if (mode != 0 && mode != 1) return sf_fail(ctx, SF_ERR_ARG, "sf_particles_add: mode must be 0 or 1");
     SF_HIP(ctx, hipSetDevice(ctx->device));
-    if (mode == 1) TRY(ef32_refresh(ctx));
     return add_particles_device(ctx, sp, n, x, y, u, v, w, ids, mode);
 }
 
@@ -400,7 +399,6 @@ int sf_push_deposit(sf_ctx* ctx, int species) {
     Species* sp = get_species(ctx, species);
     if (!sp) return sf_fail(ctx, SF_ERR_ARG, "unknown species %d", species);
     SF_HIP(ctx, hipSetDevice(ctx->device));
-    TRY(ef32_refresh(ctx));
     return push_deposit(ctx, sp);
 }
 

@@ -551,9 +551,7 @@ int sf_sources_sample(sf_ctx* ctx, int64_t* injected) {
         if (!x.empty()) {
             int64_t before = sp->np;
             SF_HIP(ctx, hipSetDevice(ctx->device));
-            int rc = ef32_refresh(ctx);
-            if (rc) return rc;
-            rc = add_particles_device(ctx, sp, (int64_t)x.size(), x.data(), y.data(), u.data(), v.data(), w.data(),
+            int rc = add_particles_device(ctx, sp, (int64_t)x.size(), x.data(), y.data(), u.data(), v.data(), w.data(),
                                       nullptr, 1);
             if (rc) return rc;
             total += sp->np - before;

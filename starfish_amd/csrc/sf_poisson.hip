@@ -54,7 +54,7 @@ struct PoissonState {
 };
 
 // device scalars layout
-enum { S_RZ = 0, S_RZ_OLD, S_PQ, S_RR, S_TOL2, S_NORM2, S_R0, S_NSCAL = 8 };
+enum { S_RZ = 0, S_RZ_OLD, S_PQ, S_RR, S_TOL2, S_NORM2, S_R0, S_YMAX, S_NSCAL = 8 };
 
 void poisson_free(PoissonState* p) {
     if (!p) return;
@@ -380,14 +380,39 @@ __global__ void accumulate_y(int64_t nn, const float* __restrict__ d, double* __
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
         y[u] += (double)d[u];
 }
-// x += y (Newton update, Solver.java:366-368 with the sign of the reduced system); y = 0
+// max |y| over the grid (for the Newton step limit)
+__global__ void absmax_kernel(int64_t nn, const double* __restrict__ y, double* __restrict__ partial,
+                              unsigned int* ticket, double* __restrict__ scal, int slot) {
+    __shared__ double sh[16];
+    double acc = 0;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
+        acc = fmax(acc, fabs(y[u]));
+    for (int o = 32; o > 0; o >>= 1) acc = fmax(acc, __shfl_xor(acc, o));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s = fmax(s, sh[w]);
+        partial[blockIdx.x] = s;
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        if (t == gridDim.x - 1) {
+            double tot = 0;
+            for (unsigned int b = 0; b < gridDim.x; b++) tot = fmax(tot, ((volatile double*)partial)[b]);
+            scal[slot] = tot;
+            *ticket = 0;
+        }
+    }
+}
+
+// x += scale * y (Newton update, Solver.java:366-368 with the sign of the reduced system); y = 0
 __global__ void newton_update(int64_t nn, const uint8_t* __restrict__ cls, double* __restrict__ x,
-                              double* __restrict__ y, double* __restrict__ partial, unsigned int* ticket,
-                              double* __restrict__ scal, int slot) {
+                              double* __restrict__ y, double scale, double* __restrict__ partial,
+                              unsigned int* ticket, double* __restrict__ scal, int slot) {
     __shared__ double sh[16];
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
-        double v = y[u];
+        double v = scale * y[u];
         if (cls[u] != CLS_NOT_FREE) x[u] += v;
         acc += v * v;
         y[u] = 0;
@@ -712,7 +737,20 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             hipLaunchKernelGGL(accumulate_y, dim3(ge), dim3(256), 0, st, nn, (const float*)P->d, P->y);
             refine++;
         }
-        hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y,
+        // Newton step limit: an update larger than 10 kTe is scaled down to 10 kTe.  Plain Newton
+        // on the exponential term overflows when it starts far below a dense plasma; the limit
+        // changes the path only, not the discrete solution.
+        double scale = 1.0;
+        if (!D.linear) {
+            hipLaunchKernelGGL(absmax_kernel, dim3(ge), dim3(256), 0, st, nn, (const double*)P->y, P->partial,
+                               P->ticket, P->scal, (int)S_YMAX);
+            SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
+            SF_HIP(ctx, hipStreamSynchronize(st));
+            double ymax = P->host_scal[S_YMAX];
+            double lim = 10.0 * D.Te0;
+            if (ymax > lim) scale = lim / ymax;
+        }
+        hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y, scale,
                            P->partial, P->ticket, P->scal, (int)S_RR);
         SF_HIP(ctx, hipGetLastError());
         if (D.linear) {
