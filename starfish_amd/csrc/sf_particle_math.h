@@ -85,7 +85,7 @@ __device__ __forceinline__ void seg_intersect(const DevSeg& s, const double* p3,
 
 // KineticMaterial.ProcessBoundary (core/materials/KineticMaterial.java:471-750) in logical
 // coordinates.  lc/v are updated in place; dtr receives the remaining time.  Returns alive.
-__device__ __noinline__ bool process_boundary(const MeshP& m, const MeshD& md, double& lc0, double& lc1, double o0,
+__device__ __forceinline__ bool process_boundary(const MeshP& m, const MeshD& md, double& lc0, double& lc1, double o0,
                                               double o1, double& v0, double& v1, double& v2, double& dtr) {
     const int ni = m.ni, nj = m.nj;
     double dt0 = dtr;
@@ -190,6 +190,50 @@ __device__ __forceinline__ bool push_particle(const MeshP& m, const MeshD& md, c
     gather_e(m, lc0, lc1, load, ex, ey);
     v0 += pc.qm_dt * ex;
     v1 += pc.qm_dt * ey;
+    // Fast path: a single sub-move that stays inside the mesh and whose bounding box of nodes
+    // lists no DIRICHLET/SINK segment.  ProcessBoundary then returns alive with no time left,
+    // so this is the reference loop's first and only iteration.  Anything else takes the loop.
+    {
+        double n0, n1, nv0 = v0, nv1 = v1, nv2 = v2;
+        if (m.domain == SF_DOMAIN_RZ) {
+            double xr = m.x0[0] + lc0 * m.dh[0] + v0 * pc.dt;
+            n1 = lc1 + v1 * pc.dt * m.inv_dh[1];
+            double A = v2 * pc.dt;
+            double R = sqrt(A * A + xr * xr);
+            double c = xr / R, s = A / R;
+            nv0 = c * v0 + s * v2;
+            nv2 = -s * v0 + c * v2;
+            n0 = (R - m.x0[0]) * m.inv_dh[0];
+        } else if (m.domain == SF_DOMAIN_ZR) {
+            double yr = m.x0[1] + lc1 * m.dh[1] + v1 * pc.dt;
+            n0 = lc0 + v0 * pc.dt * m.inv_dh[0];
+            double A = v2 * pc.dt;
+            double R = sqrt(A * A + yr * yr);
+            double c = yr / R, s = A / R;
+            nv1 = c * v1 + s * v2;
+            nv2 = -s * v1 + c * v2;
+            n1 = (R - m.x0[1]) * m.inv_dh[1];
+        } else {
+            n0 = lc0 + v0 * pc.dt * m.inv_dh[0];
+            n1 = lc1 + v1 * pc.dt * m.inv_dh[1];
+        }
+        if (n0 >= 0 && n1 >= 0 && n0 < m.ni - 1 && n1 < m.nj - 1) {
+            int imin = (int)fmin(n0, lc0), imax = (int)fmax(n0, lc0);
+            int jmin = (int)fmin(n1, lc1), jmax = (int)fmax(n1, lc1);
+            if (imin >= 0 && jmin >= 0 && imax - imin <= 1 && jmax - jmin <= 1) {
+                const uint8_t* h = md.node_hasseg + (size_t)jmin * m.ni + imin;
+                int seg = h[0] | h[imax - imin] | h[(jmax - jmin) * m.ni] | h[(jmax - jmin) * m.ni + (imax - imin)];
+                if (!seg) {
+                    lc0 = n0;
+                    lc1 = n1;
+                    v0 = nv0;
+                    v1 = nv1;
+                    v2 = nv2;
+                    return true;
+                }
+            }
+        }
+    }
     double dtr = pc.dt;
     int bounces = 0;
     bool alive = true;

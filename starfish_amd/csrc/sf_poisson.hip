@@ -16,7 +16,9 @@
 // a symmetric positive-definite system L x + W b(x) = f with the same discrete solution.
 // Node classes are derived on the host; a free node stores, per neighbour, whether that
 // neighbour is free, fixed (value known) or a face node tied back to the node itself.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "sf_internal.h"
@@ -406,11 +408,17 @@ __global__ void absmax_kernel(int64_t nn, const double* __restrict__ y, double* 
 }
 
 // x += scale * y (Newton update, Solver.java:366-368 with the sign of the reduced system); y = 0
+// The step limit is evaluated on the device: scale = min(1, lim / max|y|) with lim <= 0 off.
 __global__ void newton_update(int64_t nn, const uint8_t* __restrict__ cls, double* __restrict__ x,
-                              double* __restrict__ y, double scale, double* __restrict__ partial,
+                              double* __restrict__ y, double lim, double* __restrict__ partial,
                               unsigned int* ticket, double* __restrict__ scal, int slot) {
     __shared__ double sh[16];
     double acc = 0;
+    double scale = 1.0;
+    if (lim > 0) {
+        double ymax = scal[S_YMAX];
+        if (ymax > lim) scale = lim / ymax;
+    }
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
         double v = scale * y[u];
         if (cls[u] != CLS_NOT_FREE) x[u] += v;
@@ -681,7 +689,8 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     int newton = 0, pcg_total = 0;
     double rel = 0, rnorm = 0;
     bool converged = false;
-    const int batch = 32;
+    const bool debug = getenv("SF_DEBUG_SOLVE") != nullptr;
+    const double eta = getenv("SF_NEWTON_ETA") ? atof(getenv("SF_NEWTON_ETA")) : 1e-3;
     for (int it = 0; it <= nl_max; it++) {
         hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                            (const double*)P->W, (const double*)P->bfix, (const double*)ctx->d_rho,
@@ -696,8 +705,8 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         if (rel <= rel_tol) { converged = true; break; }
         if (it == nl_max) break;
         newton++;
-        // inexact Newton: solve J y = r to max(0.01*||r||, 0.5*tol*ref) with FP32 PCG + FP64 refinement
-        double lin_target = std::max(D.linear ? 0.5 * rel_tol * ref : 1e-2 * rnorm, 0.5 * rel_tol * ref);
+        // inexact Newton: solve J y = r to max(eta*||r||, 0.5*tol*ref) with FP32 PCG + FP64 refinement
+        double lin_target = std::max(D.linear ? 0.5 * rel_tol * ref : eta * rnorm, 0.5 * rel_tol * ref);
         // inner FP32 tolerance: a relative reduction FP32 can deliver
         int refine = 0;
         while (true) {
@@ -717,6 +726,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             int done = 0, its = 0;
             float* pa = P->p0;
             float* pb = P->p1;
+            int batch = 8;    // iterations queued between host checks; grows while unconverged
             while (!done) {
                 for (int b = 0; b < batch; b++) {
                     hipLaunchKernelGGL(pcg_spmv, dim3(P->nblk_tile), dim3(256), 0, st, m, (const uint8_t*)P->cls,
@@ -732,6 +742,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
                 SF_HIP(ctx, hipStreamSynchronize(st));
                 done = P->host_istate[0];
                 its = P->host_istate[1];
+                batch = std::min(2 * batch, 64);
             }
             pcg_total += its;
             hipLaunchKernelGGL(accumulate_y, dim3(ge), dim3(256), 0, st, nn, (const float*)P->d, P->y);
@@ -740,18 +751,20 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         // Newton step limit: an update larger than 10 kTe is scaled down to 10 kTe.  Plain Newton
         // on the exponential term overflows when it starts far below a dense plasma; the limit
         // changes the path only, not the discrete solution.
-        double scale = 1.0;
+        double lim = 0.0;
         if (!D.linear) {
+            lim = 10.0 * D.Te0;
             hipLaunchKernelGGL(absmax_kernel, dim3(ge), dim3(256), 0, st, nn, (const double*)P->y, P->partial,
                                P->ticket, P->scal, (int)S_YMAX);
+        }
+        hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y, lim,
+                           P->partial, P->ticket, P->scal, (int)S_RR);
+        if (debug) {
             SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
             SF_HIP(ctx, hipStreamSynchronize(st));
-            double ymax = P->host_scal[S_YMAX];
-            double lim = 10.0 * D.Te0;
-            if (ymax > lim) scale = lim / ymax;
+            fprintf(stderr, "[newton %d] rel %.3e lin_target/ref %.3e refine %d pcg %d ymax %.3e\n", it, rel,
+                    lin_target / ref, refine, pcg_total, P->host_scal[S_YMAX]);
         }
-        hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y, scale,
-                           P->partial, P->ticket, P->scal, (int)S_RR);
         SF_HIP(ctx, hipGetLastError());
         if (D.linear) {
             // one linear solve; evaluate the final residual on the next pass of the loop

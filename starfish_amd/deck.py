@@ -182,8 +182,11 @@ def parse_deck(path):
     bnds, mats, srcs = [], [], []
     solver = None
     num_it, dt, seed = 0, 1e-7, 0
+    parent = {c: p for p in root.iter() for c in p}
     for el in root.iter():
         tag = el.tag.lower()
+        if tag == "material" and parent.get(el) is not None and parent[el].tag.lower() != "materials":
+            continue                 # <material> inside <boundary> names the boundary's material
         if tag == "domain":
             dom = el.attrib.get("type", "xy").upper()
             for m in el:
