@@ -22,8 +22,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-BYTES_PUSH_DEPOSIT = 28.0   # pass A: read 24 B of state, write the 4 B new-cell key
-BYTES_PUSH_SCATTER = 52.0   # pass B: read 24 B state + 4 B key, write 24 B state (counting sort)
+BYTES_PUSH_INPLACE = 48.0   # in-place step: read + write the 24 B particle state
+BYTES_PUSH_DEPOSIT = 28.0   # sort step, pass A: read 24 B of state, write the 4 B new-cell key
+BYTES_PUSH_SCATTER = 52.0   # sort step, pass B: read 24 B state + 4 B key, write 24 B state
 BYTES_STATE = 48.0          # read + write of the 24 B particle state (minimal push + deposit)
 
 
@@ -35,6 +36,7 @@ def parse():
     ap.add_argument("--particles", type=float, default=2e7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--sort-interval", type=int, default=4)
     return ap.parse_args()
 
 
@@ -98,11 +100,14 @@ def main():
     cold_info = {"ms": cold.ms, "newton": cold.newton_iterations, "pcg_iterations": cold.pcg_iterations,
                  "rel_residual": cold.rel_residual, "converged": rc == 0, "wall_s": time.perf_counter() - t0}
     sim.ctx.call("sf_set_timing", 1)
+    sim.ctx.call("sf_species_set_sort_interval", mat.index, args.sort_interval)
     for _ in range(args.warmup):
         sim.step()
     sim.ctx.call("sf_ctx_synchronize")
     kern = np.zeros(4)
     pushed = 0
+    # per-kernel totals: (ms, particles) for the in-place pass, the sort-step pass A and pass B
+    k_inplace, k_sortA, k_scatter = [0.0, 0, 0], [0.0, 0, 0], [0.0, 0, 0]
     solve_ms, pcg_its, newton_its = [], [], []
     worst_res = 0.0
     t_start = time.perf_counter()
@@ -114,7 +119,12 @@ def main():
         tm = np.zeros(4)
         sim.ctx.call("sf_last_timings", tm.ctypes.data_as(C.POINTER(C.c_double)))
         kern += tm
-        pushed += npre + inj.value
+        npush = npre + inj.value
+        pushed += npush
+        for k, ms in ((k_sortA, tm[0]), (k_scatter, tm[2])) if tm[2] > 0 else ((k_inplace, tm[0]),):
+            k[0] += ms
+            k[1] += npush
+            k[2] += 1
         sim.update_fields()
         info = sim.solve_infos[-1]
         solve_ms.append(info.ms)
@@ -126,21 +136,20 @@ def main():
     wall = time.perf_counter() - t_start
     particle_s = kern.sum() * 1e-3
     value = pushed / particle_s
-    per_launch = pushed / args.steps
-    tA, tB = kern[0] / args.steps * 1e-3, kern[2] / args.steps * 1e-3
-    rA = BYTES_PUSH_DEPOSIT * per_launch / tA / 1e9
-    rB = BYTES_PUSH_SCATTER * per_launch / tB / 1e9
-    if tB >= tA:
-        dom = {"kernel": "push_scatter", "bytes_per_particle": BYTES_PUSH_SCATTER, "achieved": rB,
-               "ms_per_launch": tB * 1e3}
-    else:
-        dom = {"kernel": "push_deposit_tile", "bytes_per_particle": BYTES_PUSH_DEPOSIT, "achieved": rA,
-               "ms_per_launch": tA * 1e3}
-    roofline = {"bound": "hbm", "achieved": round(dom["achieved"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(dom["achieved"] / PEAK_HBM_GBS, 4), "traffic": None,
-                "kernel": dom["kernel"], "bytes_per_particle": dom["bytes_per_particle"],
-                "ms_per_launch": round(dom["ms_per_launch"], 4),
-                "push_deposit_tile_GBs": round(rA, 1), "push_scatter_GBs": round(rB, 1),
+
+    def rate(k, nbytes):
+        return nbytes * k[1] / (k[0] * 1e-3) / 1e9 if k[0] > 0 else 0.0
+
+    kernels = [("push_deposit_tile<inplace>", k_inplace, BYTES_PUSH_INPLACE),
+               ("push_deposit_tile<sort>", k_sortA, BYTES_PUSH_DEPOSIT),
+               ("push_scatter", k_scatter, BYTES_PUSH_SCATTER)]
+    dname, dk, dbytes = max(kernels, key=lambda e: e[1][0])
+    achieved = rate(dk, dbytes)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "kernel": dname, "bytes_per_particle": dbytes,
+                "ms_per_launch": round(dk[0] / max(1, dk[2]), 4), "launches": dk[2],
+                "per_kernel_GBs": {name: round(rate(k, b), 1) for name, k, b in kernels},
                 "particle_phase_state_GBs": round(BYTES_STATE * pushed / particle_s / 1e9, 1)}
     cpu = None
     if not args.no_cpu_baseline:

@@ -149,8 +149,9 @@ int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species);
 int sf_particles_add(sf_ctx* ctx, int species, int64_t n, const double* x, const double* y, const double* u,
                      const double* v, const double* w, const int* ids, int mode);
 int sf_particles_count(sf_ctx* ctx, int species, int64_t* n);
-/* Downloads particles: physical position (x,y), logical coordinates (lc0, lc1), velocities, ids.
-   Any pointer may be NULL.  Order is the device order (sorted by cell). */
+/* Downloads the sf_particles_count() live particles: physical position (x,y), logical coordinates
+   (lc0, lc1), velocities, ids.  Any pointer may be NULL.  Order is the device order (cell order
+   right after a counting sort). */
 int sf_particles_download(sf_ctx* ctx, int species, double* x, double* y, double* lc0, double* lc1, double* u,
                           double* v, double* w, int* ids);
 /* Device-side preload: n particles uniformly over the mesh cells whose four nodes are OPEN,
@@ -168,6 +169,11 @@ int sf_particles_preload(sf_ctx* ctx, int species, int64_t n, const double* drif
  * The species' E field is the context's (sf_poisson_solve / sf_efield_set).
  * ------------------------------------------------------------------------------------------ */
 int sf_push_deposit(sf_ctx* ctx, int species);
+/* Steps between per-cell counting sorts (KineticMaterial.sortParticlesToCells,
+   KineticMaterial.java:1150-1179, orders the particles only for merging; here the order serves
+   the tiled deposit).  Steps in between push in place and leave dead particles as holes that the
+   next sort compacts.  Results do not depend on the interval.  Default 4, minimum 1. */
+int sf_species_set_sort_interval(sf_ctx* ctx, int species, int interval);
 /* which: 0 nd, 1 u, 2 v, 3 w (instantaneous, KineticMaterial.updateFields);
    4 count-sum, 5 u-sum, 6 v-sum, 7 w-sum, 8 uu-sum, 9 vv-sum, 10 ww-sum, 11 mpc-sum
    (KineticMaterial.updateSamples); 12 t, 13 t1, 14 t2, 15 t3, 16 nd-ave, 17 u-ave, 18 v-ave,

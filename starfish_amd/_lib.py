@@ -72,6 +72,7 @@ SIGNATURES = {
     "sf_particles_download": (C.c_int, [P, C.c_int, DP, DP, DP, DP, DP, DP, DP, IP]),
     "sf_particles_preload": (C.c_int, [P, C.c_int, I64, DP, C.c_double, C.c_uint64]),
     "sf_push_deposit": (C.c_int, [P, C.c_int]),
+    "sf_species_set_sort_interval": (C.c_int, [P, C.c_int, C.c_int]),
     "sf_species_field": (C.c_int, [P, C.c_int, C.c_int, DP]),
     "sf_species_clear_samples": (C.c_int, [P, C.c_int]),
     "sf_species_stats": (C.c_int, [P, C.c_int, DP]),

@@ -342,7 +342,7 @@ int sf_particles_count(sf_ctx* ctx, int species, int64_t* n) {
     SF_CHECK_CTX(ctx);
     Species* sp = get_species(ctx, species);
     if (!sp || !n) return sf_fail(ctx, SF_ERR_ARG, "sf_particles_count: bad argument");
-    *n = sp->np;
+    *n = sp->nalive;
     return SF_OK;
 }
 
@@ -370,18 +370,23 @@ int sf_particles_download(sf_ctx* ctx, int species, double* x, double* y, double
         SF_HIP(ctx, hipMemcpy(id.data(), s.id, n * 4, hipMemcpyDeviceToHost));
     }
     const MeshP& m = ctx->mp;
+    int64_t o = 0;   // dead particles leave holes (cell < 0) until the next sort
     for (int64_t k = 0; k < n; k++) {
+        if (cell[k] < 0) continue;
         double l0 = (cell[k] & 0xFFFF) + (double)fx[k];
         double l1 = ((cell[k] >> 16) & 0xFFFF) + (double)fy[k];
-        if (x) x[k] = m.x0[0] + l0 * m.dh[0];
-        if (y) y[k] = m.x0[1] + l1 * m.dh[1];
-        if (lc0) lc0[k] = l0;
-        if (lc1) lc1[k] = l1;
-        if (u) u[k] = vx[k];
-        if (v) v[k] = vy[k];
-        if (w) w[k] = vz[k];
-        if (ids) ids[k] = id[k];
+        if (x) x[o] = m.x0[0] + l0 * m.dh[0];
+        if (y) y[o] = m.x0[1] + l1 * m.dh[1];
+        if (lc0) lc0[o] = l0;
+        if (lc1) lc1[o] = l1;
+        if (u) u[o] = vx[k];
+        if (v) v[o] = vy[k];
+        if (w) w[o] = vz[k];
+        if (ids) ids[o] = id[k];
+        o++;
     }
+    if (o != sp->nalive) return sf_fail(ctx, SF_ERR_STATE, "download found %lld particles, expected %lld",
+                                        (long long)o, (long long)sp->nalive);
     return SF_OK;
 }
 
@@ -400,6 +405,15 @@ int sf_push_deposit(sf_ctx* ctx, int species) {
     if (!sp) return sf_fail(ctx, SF_ERR_ARG, "unknown species %d", species);
     SF_HIP(ctx, hipSetDevice(ctx->device));
     return push_deposit(ctx, sp);
+}
+
+int sf_species_set_sort_interval(sf_ctx* ctx, int species, int interval) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp) return sf_fail(ctx, SF_ERR_ARG, "unknown species %d", species);
+    if (interval < 1 || interval > 64) return sf_fail(ctx, SF_ERR_ARG, "sort interval must be in [1, 64]");
+    sp->sort_interval = interval;
+    return SF_OK;
 }
 
 int sf_species_clear_samples(sf_ctx* ctx, int species) {

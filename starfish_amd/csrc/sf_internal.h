@@ -22,11 +22,12 @@ constexpr double PI = 3.14159265358979323846;
 // Tile of cells owned by one workgroup of the push/deposit pass.
 constexpr int TILE_W = 32;                      // cells along i
 constexpr int TILE_H = 8;                       // cells along j
-constexpr int TILE_NW = TILE_W + 3;             // nodes along i incl. one-cell halo on both sides
-constexpr int TILE_NH = TILE_H + 3;
+constexpr int TILE_HALO = 2;                    // cells of halo on every side (drift between sorts)
+constexpr int TILE_NW = TILE_W + 1 + 2 * TILE_HALO;   // nodes along i
+constexpr int TILE_NH = TILE_H + 1 + 2 * TILE_HALO;
 constexpr int TILE_NODES = TILE_NW * TILE_NH;
-constexpr int TILE_CW = TILE_W + 2;             // cells incl. halo
-constexpr int TILE_CH = TILE_H + 2;
+constexpr int TILE_CW = TILE_W + 2 * TILE_HALO; // cells incl. halo
+constexpr int TILE_CH = TILE_H + 2 * TILE_HALO;
 constexpr int TILE_CELLS = TILE_CW * TILE_CH;
 constexpr int NCH = 7;                          // deposit channels: n, nu, nv, nw, nuu, nvv, nww
 constexpr int PUSH_THREADS = 256;
@@ -82,7 +83,10 @@ struct Species {
     sf_species_desc desc;
     double charge, mass, qm;
     int64_t capacity = 0;
-    int64_t np = 0, nsorted = 0;
+    int64_t np = 0, nsorted = 0;   // slots in use (incl. dead holes between sorts), sorted prefix
+    int64_t nalive = 0;            // particles alive after the last push or add
+    int sort_interval = 4;         // steps between counting sorts; other steps push in place
+    int since_sort = 0;
     int cur = 0;
     PState buf[2]{};
     int32_t* key = nullptr;        // new cell key per particle (-1 dead)

@@ -131,10 +131,13 @@ struct TileLds {
     float2 ef[TILE_NODES];
     int32_t s_cell[CHUNK];
     float s_fx[CHUNK], s_fy[CHUNK], s_vx[CHUNK], s_vy[CHUNK], s_vz[CHUNK];
-    int32_t s_key[CHUNK];
     int seg_start[TILE_H], seg_pre[TILE_H + 1];
 };
 
+// INPLACE = false: sort step, the new cell of every particle goes to keys[] (pass B moves it).
+// INPLACE = true: the pushed state is written back to the particle's own slot; dead particles
+// become holes (cell = -1) until the next sort.
+template <bool INPLACE>
 __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD md, PushC pc, PState s,
                                                                    const int32_t* __restrict__ offsets,
                                                                    int32_t* __restrict__ keys,
@@ -143,9 +146,9 @@ __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD
     __shared__ TileLds L;
     const int t = threadIdx.x;
     const int tx = blockIdx.x % m.ntx, ty = blockIdx.x / m.ntx;
-    const int ti0 = tx * TILE_W, tj0 = ty * TILE_H;      // first cell of the tile
-    const int n_i0 = ti0 - 1, n_j0 = tj0 - 1;            // first node of the LDS tile (halo)
-    const int c_i0 = ti0 - 1, c_j0 = tj0 - 1;            // first cell of the LDS histogram
+    const int ti0 = tx * TILE_W, tj0 = ty * TILE_H;            // first cell of the tile
+    const int n_i0 = ti0 - TILE_HALO, n_j0 = tj0 - TILE_HALO;  // first node of the LDS tile
+    const int c_i0 = ti0 - TILE_HALO, c_j0 = tj0 - TILE_HALO;  // first cell of the LDS histogram
     const size_t nn = (size_t)m.ni * m.nj;
 
     for (int k = t; k < NCH * TILE_NODES; k += PUSH_THREADS) (&L.acc[0][0])[k] = 0ull;
@@ -181,16 +184,18 @@ __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD
         if (li >= 0 && li < TILE_NW && lj >= 0 && lj < TILE_NH) return L.ef[lj * TILE_NW + li];
         return md.ef[(size_t)j * m.ni + i];
     };
+    auto slot = [&](int v) -> int64_t {
+        int r = 0;
+        while (r < TILE_H - 1 && v >= L.seg_pre[r + 1]) r++;
+        return (int64_t)L.seg_start[r] + (v - L.seg_pre[r]);
+    };
     StatAcc st{0, 0, 0, 0, 0};
 
     for (int base = 0; base < ntile; base += CHUNK) {
         const int cn = min(CHUNK, ntile - base);
         // stage the chunk (coalesced loads)
         for (int k = t; k < cn; k += PUSH_THREADS) {
-            int v = base + k;
-            int r = 0;
-            while (r < TILE_H - 1 && v >= L.seg_pre[r + 1]) r++;
-            int64_t g = (int64_t)L.seg_start[r] + (v - L.seg_pre[r]);
+            int64_t g = slot(base + k);
             L.s_cell[k] = s.cell[g];
             L.s_fx[k] = s.fx[g];
             L.s_fy[k] = s.fy[g];
@@ -204,55 +209,70 @@ __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD
             int k = ((t + kk * PUSH_THREADS) * CHUNK_STRIDE) & (CHUNK - 1);
             if (k >= cn) continue;
             int32_t c = L.s_cell[k];
-            double lc0 = cell_i(c) + (double)L.s_fx[k];
-            double lc1 = cell_j(c) + (double)L.s_fy[k];
-            double v0 = L.s_vx[k], v1 = L.s_vy[k], v2 = L.s_vz[k];
-            bool alive = push_particle(m, md, pc, load_e, lc0, lc1, v0, v1, v2);
             int32_t key = -1;
-            if (alive) {
-                int ci, cj;
-                new_cell(m, lc0, lc1, ci, cj);
-                key = pack_cell(ci, cj);
-                float fxn = (float)(lc0 - ci), fyn = (float)(lc1 - cj);
-                float vxn = (float)v0, vyn = (float)v1, vzn = (float)v2;
-                stat_add(st, pc, vxn, vyn, vzn);
-                // histogram of the new cell
-                int hi = ci - c_i0, hj = cj - c_j0;
-                if (hi >= 0 && hi < TILE_CW && hj >= 0 && hj < TILE_CH) atomicAdd(&L.hist[hj * TILE_CW + hi], 1);
-                else atomicAdd(&counts[cj * m.nci + ci], 1);
-                // deposit from the stored (FP32) state; Field2D.scatter skips lc >= n-1
-                double di = (double)fxn, dj = (double)fyn;
-                if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
-                    double val[NCH];
-                    channel_values(pc, vxn, vyn, vzn, val);
-                    int li = ci - n_i0, lj = cj - n_j0;
-                    if (li >= 0 && li + 1 < TILE_NW && lj >= 0 && lj + 1 < TILE_NH) {
-                        double w[4];
-                        scatter_weights(m, ci, cj, di, dj, w);
-                        const int off[4] = {0, 1, TILE_NW + 1, TILE_NW};
-                        int l0 = lj * TILE_NW + li;
+            if (c >= 0) {
+                double lc0 = cell_i(c) + (double)L.s_fx[k];
+                double lc1 = cell_j(c) + (double)L.s_fy[k];
+                double v0 = L.s_vx[k], v1 = L.s_vy[k], v2 = L.s_vz[k];
+                bool alive = push_particle(m, md, pc, load_e, lc0, lc1, v0, v1, v2);
+                if (alive) {
+                    int ci, cj;
+                    new_cell(m, lc0, lc1, ci, cj);
+                    key = pack_cell(ci, cj);
+                    float fxn = (float)(lc0 - ci), fyn = (float)(lc1 - cj);
+                    float vxn = (float)v0, vyn = (float)v1, vzn = (float)v2;
+                    stat_add(st, pc, vxn, vyn, vzn);
+                    // histogram of the new cell
+                    int hi = ci - c_i0, hj = cj - c_j0;
+                    if (hi >= 0 && hi < TILE_CW && hj >= 0 && hj < TILE_CH) atomicAdd(&L.hist[hj * TILE_CW + hi], 1);
+                    else atomicAdd(&counts[cj * m.nci + ci], 1);
+                    // deposit from the stored (FP32) state; Field2D.scatter skips lc >= n-1
+                    double di = (double)fxn, dj = (double)fyn;
+                    if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
+                        double val[NCH];
+                        channel_values(pc, vxn, vyn, vzn, val);
+                        int li = ci - n_i0, lj = cj - n_j0;
+                        if (li >= 0 && li + 1 < TILE_NW && lj >= 0 && lj + 1 < TILE_NH) {
+                            double w[4];
+                            scatter_weights(m, ci, cj, di, dj, w);
+                            const int off[4] = {0, 1, TILE_NW + 1, TILE_NW};
+                            int l0 = lj * TILE_NW + li;
 #pragma unroll
-                        for (int q = 0; q < 4; q++) {
+                            for (int q = 0; q < 4; q++) {
 #pragma unroll
-                            for (int ch = 0; ch < NCH; ch++) {
-                                long long v = quant(w[q], val[ch]);
-                                atomicAdd(&L.acc[ch][l0 + off[q]], (unsigned long long)v);
+                                for (int ch = 0; ch < NCH; ch++) {
+                                    long long v = quant(w[q], val[ch]);
+                                    atomicAdd(&L.acc[ch][l0 + off[q]], (unsigned long long)v);
+                                }
                             }
+                        } else {
+                            deposit_global(m, acc, nn, ci, cj, di, dj, val);
                         }
-                    } else {
-                        deposit_global(m, acc, nn, ci, cj, di, dj, val);
+                    }
+                    if (INPLACE) {
+                        L.s_fx[k] = fxn;
+                        L.s_fy[k] = fyn;
+                        L.s_vx[k] = vxn;
+                        L.s_vy[k] = vyn;
+                        L.s_vz[k] = vzn;
                     }
                 }
             }
-            L.s_key[k] = key;
+            L.s_cell[k] = key;   // new packed cell, -1 for dead (the staged state is not reread)
         }
         __syncthreads();
         for (int k = t; k < cn; k += PUSH_THREADS) {
-            int v = base + k;
-            int r = 0;
-            while (r < TILE_H - 1 && v >= L.seg_pre[r + 1]) r++;
-            int64_t g = (int64_t)L.seg_start[r] + (v - L.seg_pre[r]);
-            keys[g] = L.s_key[k];
+            int64_t g = slot(base + k);
+            if (INPLACE) {
+                s.cell[g] = L.s_cell[k];
+                s.fx[g] = L.s_fx[k];
+                s.fy[g] = L.s_fy[k];
+                s.vx[g] = L.s_vx[k];
+                s.vy[g] = L.s_vy[k];
+                s.vz[g] = L.s_vz[k];
+            } else {
+                keys[g] = L.s_cell[k];
+            }
         }
         __syncthreads();
     }
@@ -279,6 +299,7 @@ __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD
 // ----------------------------------------------------------------------------------------
 // pass A, unsorted tail (particles added since the last sort): global atomics
 // ----------------------------------------------------------------------------------------
+template <bool INPLACE>
 __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, PushC pc, PState s, int64_t p0,
                                                          int64_t p1, int32_t* __restrict__ keys,
                                                          unsigned long long* __restrict__ acc,
@@ -288,6 +309,10 @@ __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, Push
     StatAcc st{0, 0, 0, 0, 0};
     for (int64_t p = p0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < p1;
          p += (int64_t)gridDim.x * blockDim.x) {
+        if (s.cell[p] < 0) {           // hole left by a dead particle
+            if (!INPLACE) keys[p] = -1;
+            continue;
+        }
         PIn q = load_particle(s, p);
         bool alive = push_particle(m, md, pc, load_e, q.lc0, q.lc1, q.v0, q.v1, q.v2);
         int32_t key = -1;
@@ -305,10 +330,26 @@ __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, Push
                 channel_values(pc, vxn, vyn, vzn, val);
                 deposit_global(m, acc, nn, ci, cj, di, dj, val);
             }
+            if (INPLACE) {
+                s.fx[p] = fxn;
+                s.fy[p] = fyn;
+                s.vx[p] = vxn;
+                s.vy[p] = vyn;
+                s.vz[p] = vzn;
+            }
         }
-        keys[p] = key;
+        if (INPLACE) s.cell[p] = key;
+        else keys[p] = key;
     }
     stat_flush(st, stats);
+}
+
+// cell counts of an in-place step only feed the macroparticle-per-cell samples
+__global__ void mpc_accumulate(int n, int32_t* __restrict__ counts, long long* __restrict__ mpc_sum) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        mpc_sum[c] += counts[c];
+        counts[c] = 0;
+    }
 }
 
 // ----------------------------------------------------------------------------------------
@@ -467,7 +508,7 @@ __global__ void __launch_bounds__(256) count_current(MeshP m, PState s, int64_t 
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
         int32_t c = s.cell[p];
         keys[p] = c;
-        atomicAdd(&counts[cell_j(c) * m.nci + cell_i(c)], 1);
+        if (c >= 0) atomicAdd(&counts[cell_j(c) * m.nci + cell_i(c)], 1);
     }
 }
 
@@ -675,10 +716,12 @@ int sort_species(sf_ctx* ctx, Species* sp) {
     int64_t total;
     rc = read_total(ctx, sp, &total);
     if (rc) return rc;
-    if (total != sp->np) return sf_fail(ctx, SF_ERR_STATE, "sort lost particles (%lld of %lld)", (long long)total,
-                                        (long long)sp->np);
+    if (total != sp->nalive)
+        return sf_fail(ctx, SF_ERR_STATE, "sort kept %lld particles of %lld", (long long)total, (long long)sp->nalive);
     sp->cur = 1 - sp->cur;
-    sp->nsorted = sp->np;
+    sp->np = total;
+    sp->nsorted = total;
+    sp->since_sort = 0;
     return SF_OK;
 }
 
@@ -689,20 +732,41 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     PushC pc = make_pushc(ctx, sp);
     PState s = sp->buf[sp->cur], d = sp->buf[1 - sp->cur];
     const int64_t nn = (int64_t)m.ni * m.nj;
+    const int ncells = m.nci * m.ncj;
+    // a sort step compacts and re-sorts; other steps push in place (particles drift at most
+    // sort_interval-1 cells from their tile, inside the LDS halo for sub-cell steps)
+    const bool sort_step = (sp->since_sort + 1 >= sp->sort_interval) || sp->nsorted == 0 ||
+                           (sp->np - sp->nsorted) > sp->np / 8;
     SF_HIP(ctx, hipMemsetAsync(sp->stats, 0, 8 * sizeof(long long), ctx->stream));
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    if (sp->nsorted > 0)
-        hipLaunchKernelGGL(push_deposit_tile, dim3(m.ntx * m.nty), dim3(PUSH_THREADS), 0, ctx->stream, m, md, pc, s,
-                           sp->offsets, sp->key, sp->acc, sp->counts, sp->stats);
-    if (sp->np > sp->nsorted)
-        hipLaunchKernelGGL(push_deposit_tail, dim3(grid_for(sp->np - sp->nsorted, 256, 4096)), dim3(256), 0,
-                           ctx->stream, m, md, pc, s, sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
+    if (sort_step) {
+        if (sp->nsorted > 0)
+            hipLaunchKernelGGL(push_deposit_tile<false>, dim3(m.ntx * m.nty), dim3(PUSH_THREADS), 0, ctx->stream, m, md,
+                               pc, s, sp->offsets, sp->key, sp->acc, sp->counts, sp->stats);
+        if (sp->np > sp->nsorted)
+            hipLaunchKernelGGL(push_deposit_tail<false>, dim3(grid_for(sp->np - sp->nsorted, 256, 4096)), dim3(256), 0,
+                               ctx->stream, m, md, pc, s, sp->nsorted, sp->np, sp->key, sp->acc, sp->counts,
+                               sp->stats);
+    } else {
+        if (sp->nsorted > 0)
+            hipLaunchKernelGGL(push_deposit_tile<true>, dim3(m.ntx * m.nty), dim3(PUSH_THREADS), 0, ctx->stream, m, md,
+                               pc, s, sp->offsets, sp->key, sp->acc, sp->counts, sp->stats);
+        if (sp->np > sp->nsorted)
+            hipLaunchKernelGGL(push_deposit_tail<true>, dim3(grid_for(sp->np - sp->nsorted, 256, 4096)), dim3(256), 0,
+                               ctx->stream, m, md, pc, s, sp->nsorted, sp->np, sp->key, sp->acc, sp->counts,
+                               sp->stats);
+    }
     SF_HIP(ctx, hipGetLastError());
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    int rc = run_scan(ctx, sp, true);
-    if (rc) return rc;
+    if (sort_step) {
+        int rc = run_scan(ctx, sp, true);
+        if (rc) return rc;
+    } else {
+        hipLaunchKernelGGL(mpc_accumulate, dim3(grid_for(ncells, 256, 4096)), dim3(256), 0, ctx->stream, ncells,
+                           sp->counts, sp->mpc_sum);
+    }
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    if (sp->np > 0) {
+    if (sort_step && sp->np > 0) {
         if (sp->desc.track_ids)
             hipLaunchKernelGGL(push_scatter<true>, dim3(grid_for(sp->np, 256, 1 << 20)), dim3(256), 0, ctx->stream, m,
                                md, pc, s, d, sp->np, sp->key, sp->cursor);
@@ -718,12 +782,13 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     SF_HIP(ctx, hipGetLastError());
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     sp->num_samples++;
-    int64_t total;
-    rc = read_total(ctx, sp, &total);
-    if (rc) return rc;
-    // mover sums (KineticMaterial.java:247-259)
+    // mover sums (KineticMaterial.java:247-259); the count is the number of survivors
     long long st[8];
-    SF_HIP(ctx, hipMemcpy(st, sp->stats, 8 * sizeof(long long), hipMemcpyDeviceToHost));
+    SF_HIP(ctx, hipMemcpyAsync(st, sp->stats, 8 * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    int64_t total = 0;
+    if (sort_step) SF_HIP(ctx, hipMemcpyAsync(&total, sp->offsets + ncells, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                              ctx->stream));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     double inv_kv = 1.0 / sp->kscale[1];
     sp->stats_host[0] = (double)st[0] * sp->desc.spwt * sp->mass;
     sp->stats_host[1] = (double)st[1] * inv_kv * sp->desc.spwt * sp->mass;
@@ -741,9 +806,18 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
         ctx->last_ms[2] = c;
         ctx->last_ms[3] = e;
     }
-    sp->np = total;
-    sp->nsorted = total;
-    sp->cur = 1 - sp->cur;
+    sp->nalive = st[0];
+    if (sort_step) {
+        if (total != st[0])
+            return sf_fail(ctx, SF_ERR_STATE, "counting sort kept %lld particles, the mover %lld", (long long)total,
+                           (long long)st[0]);
+        sp->np = total;
+        sp->nsorted = total;
+        sp->cur = 1 - sp->cur;
+        sp->since_sort = 0;
+    } else {
+        sp->since_sort++;
+    }
     return SF_OK;
 }
 
@@ -797,6 +871,7 @@ int add_particles_device(sf_ctx* ctx, Species* sp, int64_t n, const double* x, c
     SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (!(ids && sp->desc.track_ids)) sp->id_counter += (int)kept;
     sp->np += kept;
+    sp->nalive += kept;
     return SF_OK;
 }
 
@@ -837,6 +912,7 @@ int preload(sf_ctx* ctx, Species* sp, int64_t n, const double* drift, double tem
     hipFree(d_cdf);
     sp->id_counter += (int)n;
     sp->np += n;
+    sp->nalive += n;
     return SF_OK;
 }
 

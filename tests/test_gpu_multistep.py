@@ -1,0 +1,81 @@
+"""Multi-step runs: the GPU simulation against the oracle statistically (trajectories differ
+through FP32 state and the solvers' different iterates), and size-independent invariants at
+the full 1024x512 / 2e7-particle size."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import geometry, oracle_for
+from starfish_amd import deck as D
+
+
+@pytest.mark.gpu
+def test_c1_multistep_moments_match_oracle():
+    from starfish_amd.plugin import Simulation
+    steps = 20
+    deck = D.config_c1(steps=steps)
+    geo, st = geometry(deck)
+    o = oracle_for(deck, geo, st)
+    s = deck.sources[0]
+    o.add_source(0, deck.boundary("inlet").points, False, s.mdot, s.v_drift, s.temperature)
+    o.initial_solve()
+    o.step(steps)
+    sim = Simulation(deck, capacity=100000)
+    try:
+        sim.initial_solve()
+        sim.run(steps)
+        mat = sim.materials[0]
+        n_gpu, n_orc = mat.getNp(), o.num_particles(0)
+        print("particles gpu %d oracle %d" % (n_gpu, n_orc), flush=True)
+        assert abs(n_gpu - n_orc) <= 0.02 * n_orc
+        # the shared generator and the injection are identical, so the totals track closely
+        gp, op = mat.particles(), o.particles(0)
+        assert abs(gp["u"].mean() - op["vel"][:, 0].mean()) < 0.01 * abs(op["vel"][:, 0].mean())
+        assert abs(gp["x"].mean() - op["pos"][:, 0].mean()) < 0.01 * 0.8
+        phi_g, phi_o = sim.phi(), o.field("phi")
+        assert np.abs(phi_g - phi_o).max() < 0.02 * 100.0
+        nd_g, nd_o = mat.field("nd-ave"), o.field("count-sum", 0) / steps / geo.node_vol
+        assert abs(nd_g.sum() - nd_o.sum()) < 0.02 * nd_o.sum()
+    finally:
+        sim.close()
+
+
+@pytest.mark.gpu
+def test_full_size_invariants_and_determinism():
+    """1024x512 nodes, 2e7 particles: particle conservation (sort total == alive count of the
+    mover), deposited weight == particles * spwt, cell order, bit-identical density run to run."""
+    from starfish_amd.plugin import Simulation
+    nds = []
+    for rep in range(2):
+        deck = D.config_c2(steps=3)
+        sim = Simulation(deck, capacity=30_000_000)
+        try:
+            mat = sim.materials[0]
+            npart, drift, temp = deck.preload
+            mat.preload(npart, drift, temp, seed=1)
+            assert mat.getNp() == npart
+            sim.initial_solve()
+            for _ in range(3):
+                inj = C.c_int64()
+                sim.ctx.call("sf_sources_sample", C.byref(inj))
+                mat.update()
+                st = mat.stats()
+                alive = st[0] / (deck.materials[0].spwt * deck.materials[0].molwt * D.AMU)
+                assert abs(alive - mat.getNp()) < 0.5
+                sim.update_fields()
+                assert sim.solve_infos[-1].rel_residual <= 1e-8
+            nd = mat.field("nd")
+            geo = sim.geo
+            # weight conservation: particles on the last node line are not deposited (Field2D.scatter)
+            p = mat.particles()
+            m = deck.mesh
+            on_edge = (p["lc0"] >= m.ni - 1) | (p["lc1"] >= m.nj - 1)
+            w = (nd * geo.node_vol).sum() / deck.materials[0].spwt
+            assert abs(w - (len(p["lc0"]) - on_edge.sum())) < 1e-6 * len(p["lc0"])
+            key = np.floor(p["lc1"]).clip(0, m.nj - 2) * (m.ni - 1) + np.floor(p["lc0"]).clip(0, m.ni - 2)
+            assert np.all(np.diff(key) >= 0)
+            nds.append(nd)
+        finally:
+            sim.close()
+    assert np.array_equal(nds[0], nds[1])

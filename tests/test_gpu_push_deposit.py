@@ -33,9 +33,10 @@ def _compare_particles(deck, gp, op):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [20000])
-def test_push_deposit_one_step_matches_oracle(n):
+@pytest.mark.parametrize("interval", [1, 4])
+def test_push_deposit_one_step_matches_oracle(interval):
     from starfish_amd.plugin import Simulation
+    n = 20000
     deck = D.config_c1()
     geo, st = geometry(deck)
     o = oracle_for(deck, geo, st)
@@ -50,8 +51,9 @@ def test_push_deposit_one_step_matches_oracle(n):
     try:
         sim.set_efield(efi, efj)
         mat = sim.materials[0]
+        sim.ctx.call("sf_species_set_sort_interval", mat.index, interval)
         mat.add_particles(x, y, v[:, 0], v[:, 1], v[:, 2], ids=ids, rewind=False)
-        for step in range(2):
+        for step in range(3):
             o.move()
             o.update_species_fields()
             mat.update()
@@ -71,12 +73,46 @@ def test_push_deposit_one_step_matches_oracle(n):
             gs, os_ = mat.stats(), o.stats(0)
             assert abs(gs[0] - os_[0]) <= 2e-5 * abs(os_[0])
             assert np.abs(gs[1:] - os_[1:]).max() <= 1e-4 * np.abs(os_[1:]).max()
-            # cell-sorted order after the push (counting sort)
-            key = np.floor(gp["lc1"]).clip(0, deck.mesh.nj - 2) * (deck.mesh.ni - 1) + \
-                np.floor(gp["lc0"]).clip(0, deck.mesh.ni - 2)
-            assert np.all(np.diff(key) >= 0), "particles are not sorted by cell"
+            # cell-sorted order after a sort step (counting sort)
+            if interval == 1:
+                key = np.floor(gp["lc1"]).clip(0, deck.mesh.nj - 2) * (deck.mesh.ni - 1) + \
+                    np.floor(gp["lc0"]).clip(0, deck.mesh.ni - 2)
+                assert np.all(np.diff(key) >= 0), "particles are not sorted by cell"
     finally:
         sim.close()
+
+
+@pytest.mark.gpu
+def test_fields_do_not_depend_on_the_sort_interval():
+    """Pushes are independent per particle and deposits are integer sums: sorting every step,
+    every 3rd step or every 5th step gives bit-identical fields and particle sets."""
+    from starfish_amd.plugin import Simulation
+    deck = D.config_c1()
+    geo, st = geometry(deck)
+    o = oracle_for(deck, geo, st)
+    o.initial_solve()
+    n = 60000
+    x, y, v, _, _ = open_cell_particles(deck, geo, n, seed=5)
+    ids = np.arange(n, dtype=np.int32)
+    res = []
+    for interval in (1, 3, 5):
+        sim = Simulation(deck, capacity=2 * n, track_ids=True)
+        try:
+            sim.set_efield(o.field("efi"), o.field("efj"))
+            mat = sim.materials[0]
+            sim.ctx.call("sf_species_set_sort_interval", mat.index, interval)
+            mat.add_particles(x, y, v[:, 0], v[:, 1], v[:, 2], ids=ids, rewind=False)
+            for _ in range(7):
+                mat.update()
+            gp = mat.particles(ids=True)
+            order = np.argsort(gp["id"])
+            res.append([mat.field(k) for k in ("nd", "u", "count-sum", "uu-sum", "mpc-sum")] +
+                       [gp[k][order] for k in ("id", "x", "y", "u", "v", "w")])
+        finally:
+            sim.close()
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            assert np.array_equal(a, b)
 
 
 @pytest.mark.gpu
