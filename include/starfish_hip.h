@@ -243,6 +243,47 @@ int sf_sources_sample(sf_ctx* ctx, int64_t* injected);
 int sf_last_timings(sf_ctx* ctx, double* out4);
 int sf_set_timing(sf_ctx* ctx, int enabled);
 
+/* ------------------------------------------------------------------------------------------
+ * Learned initial guess for the potential solve.  No counterpart in the reference: the
+ * reference solvers start from the previous potential (LinearSolverPCG.java:33-37,
+ * LinearSolverGS.java:41-53).  A 3-level encoder-decoder CNN (32/64/128 channels, 3x3
+ * convolutions, BF16 weights and activations, FP32 accumulation on the matrix cores) maps the
+ * solver's input (rho, Dirichlet mask, boundary phi, Boltzmann inversion of rho) to phi.  It is
+ * trained on the device by the library's own forward/backward kernels and Adam.  The guess
+ * only replaces the starting point: sf_poisson_solve still runs to its residual tolerance.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct sf_cnn sf_cnn;
+/* network for an ni x nj node grid (ni, nj <= 0: the context's mesh); max_batch <= 64 */
+int sf_cnn_create(sf_ctx* ctx, int ni, int nj, int max_batch, uint64_t seed, sf_cnn** out);
+int sf_cnn_destroy(sf_cnn* cnn);
+/* padded grid (multiples of 4) and number of FP32 parameters */
+int sf_cnn_dims(const sf_cnn* cnn, int* H, int* W, int64_t* nparams);
+/* training set: snapshot of a solved context (its rho, node classes and phi) or host arrays
+   x4 [H][W][4], target [H][W] (phi / 100 V), mask [H][W] (1 on free nodes) */
+int sf_cnn_snapshot(sf_cnn* cnn, sf_ctx* solved);
+int sf_cnn_add_sample(sf_cnn* cnn, const float* x4, const float* target, const uint8_t* mask);
+int sf_cnn_get_sample(sf_cnn* cnn, int idx, float* x4, float* target, uint8_t* mask);
+int sf_cnn_num_samples(const sf_cnn* cnn);
+/* Adam on random mini-batches; *loss_out = mean masked MSE over the last tenth of the steps */
+int sf_cnn_train(sf_cnn* cnn, int steps, int batch, double lr, double* loss_out);
+/* loss, network output y [H][W] and all parameter gradients (order of sf_cnn_params) of one
+   sample; any output pointer may be NULL */
+int sf_cnn_loss_and_grads(sf_cnn* cnn, int sample, double* loss_out, float* y_out, float* grads_out,
+                          int64_t* n_out);
+/* FP32 parameters: per convolution W[9][N][K] then b[N] (10 convolutions), then the head
+   w[32], b[1].  get = 1 downloads into buf, get = 0 uploads from buf. */
+int sf_cnn_params(sf_cnn* cnn, int get, float* buf, int64_t* n_out);
+/* writes the network's phi for the current rho / boundary values of `target` into its phi,
+   the starting point of the next sf_poisson_solve */
+int sf_cnn_predict(sf_cnn* cnn, sf_ctx* target);
+/* sf_poisson_solve started from the network's guess.  policy 0: always from the guess;
+   policy 1: from whichever of the current phi and the guess has the smaller nonlinear residual.
+   *used_guess (may be NULL) reports the choice.  Same tolerance and status as sf_poisson_solve. */
+int sf_poisson_solve_learned(sf_ctx* ctx, sf_cnn* cnn, int policy, sf_solve_info* info, int* used_guess);
+/* d[16][16] = a[16][32] * b[32][16] through one BF16 matrix-core instruction with the operand
+   layout the convolution kernels use (a, b rounded to BF16) */
+int sf_cnn_mfma_probe(sf_ctx* ctx, const float* a, const float* b, float* d);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,7 @@ class SolveInfo(C.Structure):
 P = C.c_void_p
 DP = C.POINTER(C.c_double)
 IP = C.POINTER(C.c_int)
+FP = C.POINTER(C.c_float)
 I64 = C.c_int64
 
 # name -> (restype, argtypes); every symbol declared in include/starfish_hip.h
@@ -92,6 +93,19 @@ SIGNATURES = {
     "sf_sources_sample": (C.c_int, [P, C.POINTER(I64)]),
     "sf_last_timings": (C.c_int, [P, DP]),
     "sf_set_timing": (C.c_int, [P, C.c_int]),
+    "sf_cnn_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]),
+    "sf_cnn_destroy": (C.c_int, [P]),
+    "sf_cnn_dims": (C.c_int, [P, IP, IP, C.POINTER(I64)]),
+    "sf_cnn_snapshot": (C.c_int, [P, P]),
+    "sf_cnn_add_sample": (C.c_int, [P, FP, FP, C.POINTER(C.c_uint8)]),
+    "sf_cnn_get_sample": (C.c_int, [P, C.c_int, FP, FP, C.POINTER(C.c_uint8)]),
+    "sf_cnn_num_samples": (C.c_int, [P]),
+    "sf_cnn_train": (C.c_int, [P, C.c_int, C.c_int, C.c_double, DP]),
+    "sf_cnn_loss_and_grads": (C.c_int, [P, C.c_int, DP, FP, FP, C.POINTER(I64)]),
+    "sf_cnn_params": (C.c_int, [P, C.c_int, FP, C.POINTER(I64)]),
+    "sf_cnn_predict": (C.c_int, [P, P]),
+    "sf_poisson_solve_learned": (C.c_int, [P, P, C.c_int, C.POINTER(SolveInfo), IP]),
+    "sf_cnn_mfma_probe": (C.c_int, [P, FP, FP, FP]),
 }
 
 _lib = None

@@ -1,0 +1,984 @@
+// Learned initial guess for the potential solve: a 3-level encoder-decoder CNN
+// (32/64/128 channels, 3x3 convolutions, BF16 weights and activations, FP32 accumulation on
+// the gfx950 matrix cores) mapping (rho, Dirichlet mask, boundary phi) to phi.  Training runs
+// on the GPU with the kernels below (forward, backward-data, backward-weights, Adam).  The
+// network only supplies the starting point of the Newton/PCG solve; every solve still runs to
+// the requested relative residual, so plain PCG remains the fallback and the judge.
+//
+// Layout: activations are NHWC BF16 tensors addressed through views (pointer, channel stride,
+// channel offset) so that skip connections concatenate without copies.  3x3 convolutions are
+// implicit GEMMs on v_mfma_f32_16x16x32_bf16: M = 16 pixels of a row, N = 16 output channels,
+// K = 32 input channels of one tap.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "sf_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace sf {
+
+__device__ __host__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)(u >> 16);   // inf / nan
+    u += 0x7FFFu + ((u >> 16) & 1u);                                     // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __host__ __forceinline__ float bf2f(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
+    uint4 v = *reinterpret_cast<const uint4*>(p);
+    bf16x8 r;
+    memcpy(&r, &v, 16);
+    return r;
+}
+__device__ __forceinline__ bf16x8 zero8() {
+    uint4 v = make_uint4(0, 0, 0, 0);
+    bf16x8 r;
+    memcpy(&r, &v, 16);
+    return r;
+}
+
+// ------------------------------------------------------------------------------------------
+// 3x3 convolution as an implicit GEMM (stride 1, zero padding 1).
+//   forward:       out[p][n] = sum_tap sum_k in[p + d(tap)][k] * w[tap][n][k]   (+ bias, ReLU)
+//   backward data: din[p][k'] = sum_tap sum_n' g[p - d(tap)][n'] * w[tap][n'][k'] with the weight
+//                  stored transposed as wt[tap][k'][n'] -> the same kernel with sgn = -1.
+// One workgroup: 64 pixels of one row x 64 output channels; wave w owns pixels 16w..16w+15.
+// Requirements (checked on the host): K multiple of 32, N multiple of 16, channel strides and
+// offsets multiples of 8 (16-byte operand loads).
+// ------------------------------------------------------------------------------------------
+struct View {
+    uint16_t* p;
+    int cs;   // channel stride (elements per pixel)
+    int co;   // channel offset
+};
+struct FView {
+    float* p;
+    int cs, co;
+};
+
+template <int MODE>   // 0: forward BF16 out (+bias, optional ReLU); 1: backward data, FP32 out (+= or =)
+__global__ void __launch_bounds__(256) conv3x3_mfma(const uint16_t* __restrict__ in, int in_cs, int in_co, int K,
+                                                    const uint16_t* __restrict__ w, const float* __restrict__ bias,
+                                                    int N, uint16_t* __restrict__ out, float* __restrict__ fout,
+                                                    int out_cs, int out_co, int H, int Wd, int relu, int sgn,
+                                                    int accumulate) {
+    const int ntn = (N + 63) / 64;
+    const int x0 = blockIdx.x * 64, y = blockIdx.y;
+    const int b = blockIdx.z / ntn, n0 = (blockIdx.z % ntn) * 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int prow = lane & 15, kq = (lane >> 4) * 8;
+    const int px = x0 + wave * 16 + prow;
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; tap++) {
+        const int dy = sgn * (tap / 3 - 1), dx = sgn * (tap % 3 - 1);
+        const int yy = y + dy, xx = px + dx;
+        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < Wd;
+        const uint16_t* arow = in + (((size_t)b * H + (ok ? yy : 0)) * Wd + (ok ? xx : 0)) * in_cs + in_co;
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            bf16x8 a = ok ? load8(arow + k0 + kq) : zero8();
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int n = n0 + t * 16 + prow;
+                bf16x8 bv = (n < N) ? load8(w + ((size_t)tap * N + n) * K + k0 + kq) : zero8();
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    // D layout: lane holds rows (lane/16)*4 + i (pixels), column lane%16 (channel)
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int n = n0 + t * 16 + prow;
+        if (n >= N) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int xo = x0 + wave * 16 + (lane >> 4) * 4 + i;
+            if (xo >= Wd) continue;
+            const size_t o = (((size_t)b * H + y) * Wd + xo) * out_cs + out_co + n;
+            float v = acc[t][i];
+            if (MODE == 0) {
+                v += bias[n];
+                if (relu) v = fmaxf(v, 0.f);
+                out[o] = f2bf(v);
+            } else {
+                fout[o] = accumulate ? fout[o] + v : v;
+            }
+        }
+    }
+}
+
+// Weight gradient: dw[tap][n][k] += sum_p in[p + d(tap)][k] * g[p][n] (FP32 atomics per block),
+// the MFMA reduction dimension being 32 consecutive pixels of one row.
+// Block: one tap, 32 k x 32 n (2x2 tiles over the 4 waves), a band of rows.
+__global__ void __launch_bounds__(256) conv3x3_wgrad(const uint16_t* __restrict__ in, int in_cs, int in_co, int K,
+                                                     const uint16_t* __restrict__ g, int g_cs, int N,
+                                                     float* __restrict__ dw, int H, int Wd, int B,
+                                                     int rows_per_block) {
+    const int tap = blockIdx.y;
+    const int nkt = K / 32;
+    const int k0 = (blockIdx.z % nkt) * 32, n0 = (blockIdx.z / nkt) * 32;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tk = wave & 1, tn = wave >> 1;           // this wave's 16x16 tile of the 32x32 block
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int mrow = lane & 15, pq = (lane >> 4) * 8;
+    const int kk = k0 + tk * 16 + mrow;                 // A row = input channel
+    const int nn = n0 + tn * 16 + mrow;                 // B column = output channel
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int row_begin = blockIdx.x * rows_per_block;
+    const int total_rows = B * H;
+    for (int r = row_begin; r < min(row_begin + rows_per_block, total_rows); r++) {
+        const int b = r / H, y = r % H;
+        const int yy = y + dy;
+        for (int x0 = 0; x0 < Wd; x0 += 32) {
+            uint16_t av[8], bv[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int x = x0 + pq + j;
+                const int xx = x + dx;
+                const bool okg = x < Wd;
+                const bool oki = okg && yy >= 0 && yy < H && xx >= 0 && xx < Wd;
+                av[j] = oki ? in[(((size_t)b * H + yy) * Wd + xx) * in_cs + in_co + kk] : (uint16_t)0;
+                bv[j] = okg ? g[(((size_t)b * H + y) * Wd + x) * g_cs + nn] : (uint16_t)0;
+            }
+            bf16x8 a, bb;
+            memcpy(&a, av, 16);
+            memcpy(&bb, bv, 16);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc, 0, 0, 0);
+        }
+    }
+    // D rows (lane/16)*4+i = k, column lane%16 = n
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int k = k0 + tk * 16 + (lane >> 4) * 4 + i;
+        const int n = n0 + tn * 16 + mrow;
+        atomicAdd(&dw[((size_t)tap * N + n) * K + k], acc[i]);
+    }
+}
+
+// bias gradient: db[n] += sum_p g[p][n]
+__global__ void bias_grad(const uint16_t* __restrict__ g, int g_cs, int N, int64_t npix, float* __restrict__ db) {
+    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+        float s = 0;
+        for (int64_t p = blockIdx.x; p < npix; p += gridDim.x) s += bf2f(g[p * g_cs + n]);
+        atomicAdd(&db[n], s);
+    }
+}
+
+// ReLU backward into a BF16 operand: g = (act > 0) ? d : 0
+__global__ void relu_bwd(const uint16_t* __restrict__ act, int a_cs, int a_co, const float* __restrict__ d, int d_cs,
+                         int d_co, int C, int64_t npix, uint16_t* __restrict__ g) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix * C; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t p = i / C;
+        int c = (int)(i % C);
+        float a = bf2f(act[p * a_cs + a_co + c]);
+        g[p * C + c] = f2bf(a > 0.f ? d[p * d_cs + d_co + c] : 0.f);
+    }
+}
+
+// 2x2 average pooling (forward BF16, backward FP32 accumulate)
+__global__ void avgpool_fwd(const uint16_t* __restrict__ in, int in_cs, int in_co, int C, int H, int Wd, int B,
+                            uint16_t* __restrict__ out, int out_cs, int out_co) {
+    const int Ho = H / 2, Wo = Wd / 2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * Ho * Wo * C;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int c = (int)(i % C);
+        int64_t q = i / C;
+        int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), b = (int)(q / ((int64_t)Wo * Ho));
+        float s = 0;
+        for (int a = 0; a < 2; a++)
+            for (int e = 0; e < 2; e++) s += bf2f(in[(((size_t)b * H + 2 * yo + a) * Wd + 2 * xo + e) * in_cs + in_co + c]);
+        out[(((size_t)b * Ho + yo) * Wo + xo) * out_cs + out_co + c] = f2bf(0.25f * s);
+    }
+}
+__global__ void avgpool_bwd(const float* __restrict__ dout, int d_cs, int d_co, int C, int H, int Wd, int B,
+                            float* __restrict__ din, int i_cs, int i_co) {
+    const int Ho = H / 2, Wo = Wd / 2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * H * Wd * C;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int c = (int)(i % C);
+        int64_t q = i / C;
+        int x = (int)(q % Wd), y = (int)((q / Wd) % H), b = (int)(q / ((int64_t)Wd * H));
+        float v = 0.25f * dout[(((size_t)b * Ho + y / 2) * Wo + x / 2) * d_cs + d_co + c];
+        din[(((size_t)b * H + y) * Wd + x) * i_cs + i_co + c] += v;
+    }
+}
+// nearest 2x upsampling (forward BF16 into a view, backward FP32 sum of the four children)
+__global__ void upsample_fwd(const uint16_t* __restrict__ in, int in_cs, int C, int Hi, int Wi, int B,
+                             uint16_t* __restrict__ out, int out_cs, int out_co) {
+    const int H = 2 * Hi, Wd = 2 * Wi;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * H * Wd * C;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int c = (int)(i % C);
+        int64_t q = i / C;
+        int x = (int)(q % Wd), y = (int)((q / Wd) % H), b = (int)(q / ((int64_t)Wd * H));
+        out[(((size_t)b * H + y) * Wd + x) * out_cs + out_co + c] = in[(((size_t)b * Hi + y / 2) * Wi + x / 2) * in_cs + c];
+    }
+}
+__global__ void upsample_bwd(const float* __restrict__ dout, int d_cs, int d_co, int C, int Hi, int Wi, int B,
+                             float* __restrict__ din) {
+    const int H = 2 * Hi, Wd = 2 * Wi;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * Hi * Wi * C;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int c = (int)(i % C);
+        int64_t q = i / C;
+        int x = (int)(q % Wi), y = (int)((q / Wi) % Hi), b = (int)(q / ((int64_t)Wi * Hi));
+        float s = 0;
+        for (int a = 0; a < 2; a++)
+            for (int e = 0; e < 2; e++) s += dout[(((size_t)b * H + 2 * y + a) * Wd + 2 * x + e) * d_cs + d_co + c];
+        din[(((size_t)b * Hi + y) * Wi + x) * C + c] = s;
+    }
+}
+// 1x1 head C -> 1 in FP32 on top of a base channel of the input: y = base + sum_c w[c] act[c] + b
+__global__ void head_fwd(const uint16_t* __restrict__ act, int C, int64_t npix, const float* __restrict__ w,
+                         const float* __restrict__ bias, const uint16_t* __restrict__ base, int base_cs,
+                         int base_co, float* __restrict__ y) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
+        float s = bias[0] + bf2f(base[p * base_cs + base_co]);
+        for (int c = 0; c < C; c++) s += w[c] * bf2f(act[p * C + c]);
+        y[p] = s;
+    }
+}
+// masked MSE loss: L = mean over counted pixels of (y - t)^2; dy = 2 (y - t) / count
+__global__ void mse_loss(const float* __restrict__ y, const float* __restrict__ t, const uint8_t* __restrict__ m,
+                         int64_t npix, float inv_count, float* __restrict__ dy, float* __restrict__ loss) {
+    float acc = 0;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
+        float e = m[p] ? (y[p] - t[p]) : 0.f;
+        dy[p] = 2.f * e * inv_count;
+        acc += e * e;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, acc * inv_count);
+}
+// head backward: dw[c] += sum_p dy[p] act[p][c]; db += sum dy; dact[p][c] = dy[p] w[c] (FP32)
+__global__ void head_bwd(const uint16_t* __restrict__ act, int C, int64_t npix, const float* __restrict__ w,
+                         const float* __restrict__ dy, float* __restrict__ dw, float* __restrict__ db,
+                         float* __restrict__ dact) {
+    __shared__ float sw[128], sdb[4];
+    for (int c = threadIdx.x; c < C; c += blockDim.x) sw[c] = 0;
+    if (threadIdx.x == 0) sdb[0] = 0;
+    __syncthreads();
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
+        float d = dy[p];
+        for (int c = 0; c < C; c++) {
+            atomicAdd(&sw[c], d * bf2f(act[p * C + c]));
+            dact[p * C + c] = d * w[c];
+        }
+        atomicAdd(&sdb[0], d);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&dw[c], sw[c]);
+    if (threadIdx.x == 0) atomicAdd(db, sdb[0]);
+}
+// Adam (Kingma & Ba 2015) on FP32 master weights
+__global__ void adam_step(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                          float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
+                          float c2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float gi = g[i];
+        float mi = b1 * m[i] + (1 - b1) * gi;
+        float vi = b2 * v[i] + (1 - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+// master FP32 weights [tap][n][k] (k < Kreal used, padded to K) -> BF16 forward [tap][n][K] and
+// transposed backward [tap][k][n] copies
+__global__ void pack_weights(const float* __restrict__ wm, int N, int K, uint16_t* __restrict__ wf,
+                             uint16_t* __restrict__ wt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 9LL * N * K; i += (int64_t)gridDim.x * blockDim.x) {
+        int k = (int)(i % K);
+        int64_t q = i / K;
+        int n = (int)(q % N), tap = (int)(q / N);
+        uint16_t h = f2bf(wm[i]);
+        wf[i] = h;
+        wt[((size_t)tap * K + k) * N + n] = h;
+    }
+}
+
+}  // namespace sf
+
+// ==========================================================================================
+// host: the network, its training loop and the C-ABI
+// ==========================================================================================
+namespace sf {
+
+struct ConvL {
+    int K = 0, N = 0, Kreal = 0;       // padded input channels, output channels, real inputs
+    float *wm = nullptr, *b = nullptr, *gw = nullptr, *gb = nullptr;
+    float *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;
+    uint16_t *wf = nullptr, *wt = nullptr;
+    size_t nw() const { return (size_t)9 * N * K; }
+};
+
+}  // namespace sf
+
+struct sf_cnn {
+    sf_ctx* ctx = nullptr;
+    int ni = 0, nj = 0;       // mesh
+    int H = 0, W = 0;         // network grid (mesh padded to multiples of 4)
+    int Bmax = 0;
+    int C1 = 32, C2 = 64, C3 = 128;
+    sf::ConvL c1a, c1b, c2a, c2b, c3a, c3b, c4a, c4b, c5a, c5b;
+    float *hw = nullptr, *hb = nullptr, *ghw = nullptr, *ghb = nullptr;
+    float *mhw = nullptr, *vhw = nullptr, *mhb = nullptr, *vhb = nullptr;
+    // activations (BF16 NHWC)
+    uint16_t *x0 = nullptr, *e1a = nullptr, *u1 = nullptr, *p1 = nullptr, *e2a = nullptr, *u2 = nullptr,
+             *p2 = nullptr, *e3a = nullptr, *e3 = nullptr, *d2a = nullptr, *d2 = nullptr, *d1a = nullptr,
+             *d1 = nullptr, *gop = nullptr;
+    float* y = nullptr;
+    // gradients (FP32)
+    float *g_e1a = nullptr, *g_u1 = nullptr, *g_p1 = nullptr, *g_e2a = nullptr, *g_u2 = nullptr, *g_p2 = nullptr,
+          *g_e3a = nullptr, *g_e3 = nullptr, *g_d2a = nullptr, *g_d2 = nullptr, *g_d1a = nullptr, *g_d1 = nullptr,
+          *dy = nullptr, *loss = nullptr;
+    // dataset
+    std::vector<uint16_t*> s_in;      // [H][W][4] BF16: rho, mask, phi_b, 0
+    std::vector<float*> s_t;          // [H][W] target phi / phi_ref
+    std::vector<uint8_t*> s_m;        // [H][W] 1 on free nodes (loss mask)
+    uint8_t* bmask = nullptr;         // batch loss mask
+    float* bt = nullptr;              // batch target
+    double rho_ref = sf::QE * 1e16, phi_ref = 100.0;
+    int adam_t = 0;
+    std::mt19937_64 rng{1};
+    double* phi_save = nullptr;       // [ni*nj] stash of a context's potential
+    std::vector<void*> allocs;
+};
+
+namespace sf {
+
+static int grid1(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 16384));
+}
+
+template <class T>
+static int cnn_alloc(sf_cnn* c, T** p, size_t n) {
+    *p = nullptr;
+    hipError_t e = hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return sf_fail(c->ctx, SF_ERR_OOM, "cnn allocation of %zu bytes failed", n * sizeof(T));
+    hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T));
+    c->allocs.push_back((void*)*p);
+    return SF_OK;
+}
+
+static int conv_init(sf_cnn* c, ConvL& L, int Kreal, int N, std::mt19937_64& rng) {
+    L.Kreal = Kreal;
+    L.K = (Kreal + 31) / 32 * 32;
+    L.N = N;
+    size_t nw = L.nw();
+    int rc = 0;
+    rc |= cnn_alloc(c, &L.wm, nw);
+    rc |= cnn_alloc(c, &L.gw, nw);
+    rc |= cnn_alloc(c, &L.mw, nw);
+    rc |= cnn_alloc(c, &L.vw, nw);
+    rc |= cnn_alloc(c, &L.b, N);
+    rc |= cnn_alloc(c, &L.gb, N);
+    rc |= cnn_alloc(c, &L.mb, N);
+    rc |= cnn_alloc(c, &L.vb, N);
+    rc |= cnn_alloc(c, &L.wf, nw);
+    rc |= cnn_alloc(c, &L.wt, nw);
+    if (rc) return SF_ERR_OOM;
+    // He initialisation over the real fan-in; padded input channels stay zero
+    std::normal_distribution<double> nd(0.0, std::sqrt(2.0 / (9.0 * Kreal)));
+    std::vector<float> h(nw, 0.f);
+    for (int tap = 0; tap < 9; tap++)
+        for (int n = 0; n < N; n++)
+            for (int k = 0; k < Kreal; k++) h[((size_t)tap * N + n) * L.K + k] = (float)nd(rng);
+    if (hipMemcpy(L.wm, h.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess) return SF_ERR_HIP;
+    return SF_OK;
+}
+
+static void conv_pack(sf_cnn* c, ConvL& L) {
+    hipLaunchKernelGGL(pack_weights, dim3(grid1(L.nw())), dim3(256), 0, c->ctx->stream, (const float*)L.wm, L.N, L.K,
+                       L.wf, L.wt);
+}
+
+// forward conv: in view (cs, co, K channels) -> out view, bias + ReLU
+static void conv_fwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs,
+                     int out_co, int H, int W, int B) {
+    dim3 g((W + 63) / 64, H, B * ((L.N + 63) / 64));
+    hipLaunchKernelGGL(conv3x3_mfma<0>, g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, L.K, (const uint16_t*)L.wf,
+                       (const float*)L.b, L.N, out, (float*)nullptr, out_cs, out_co, H, W, 1, 1, 0);
+}
+// backward of one conv: g (BF16 [npix][N]) -> weight / bias gradients and, if din != null, din
+static void conv_bwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, int in_co, const uint16_t* g,
+                     float* din, int din_cs, int din_co, int H, int W, int B) {
+    const int rows = 8;
+    dim3 gw((B * H + rows - 1) / rows, 9, (L.K / 32) * (L.N / 32));
+    hipLaunchKernelGGL(conv3x3_wgrad, gw, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, L.K, g, L.N, L.N, L.gw, H, W,
+                       B, rows);
+    hipLaunchKernelGGL(bias_grad, dim3(256), dim3(256), 0, c->ctx->stream, g, L.N, L.N, (int64_t)B * H * W, L.gb);
+    if (din) {
+        dim3 gd((W + 63) / 64, H, B * ((L.K + 63) / 64));
+        hipLaunchKernelGGL(conv3x3_mfma<1>, gd, dim3(256), 0, c->ctx->stream, g, L.N, 0, L.N, (const uint16_t*)L.wt,
+                           (const float*)nullptr, L.K, (uint16_t*)nullptr, din, din_cs, din_co, H, W, 0, -1, 0);
+    }
+}
+static void relu_op(sf_cnn* c, const uint16_t* act, int a_cs, int a_co, const float* d, int d_cs, int d_co, int C,
+                    int64_t npix) {
+    hipLaunchKernelGGL(relu_bwd, dim3(grid1(npix * C)), dim3(256), 0, c->ctx->stream, act, a_cs, a_co, d, d_cs, d_co,
+                       C, npix, c->gop);
+}
+
+static void forward(sf_cnn* c, int B) {
+    const int H = c->H, W = c->W, C1 = c->C1, C2 = c->C2, C3 = c->C3;
+    const int H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
+    hipStream_t st = c->ctx->stream;
+    for (ConvL* L : {&c->c1a, &c->c1b, &c->c2a, &c->c2b, &c->c3a, &c->c3b, &c->c4a, &c->c4b, &c->c5a, &c->c5b})
+        conv_pack(c, *L);
+    conv_fwd(c, c->c1a, c->x0, 32, 0, c->e1a, C1, 0, H, W, B);
+    conv_fwd(c, c->c1b, c->e1a, C1, 0, c->u1, C2 + C1, C2, H, W, B);                  // e1 -> u1[C2:]
+    hipLaunchKernelGGL(avgpool_fwd, dim3(grid1((int64_t)B * H2 * W2 * C1)), dim3(256), 0, st, (const uint16_t*)c->u1,
+                       C2 + C1, C2, C1, H, W, B, c->p1, C1, 0);
+    conv_fwd(c, c->c2a, c->p1, C1, 0, c->e2a, C2, 0, H2, W2, B);
+    conv_fwd(c, c->c2b, c->e2a, C2, 0, c->u2, C3 + C2, C3, H2, W2, B);                // e2 -> u2[C3:]
+    hipLaunchKernelGGL(avgpool_fwd, dim3(grid1((int64_t)B * H4 * W4 * C2)), dim3(256), 0, st, (const uint16_t*)c->u2,
+                       C3 + C2, C3, C2, H2, W2, B, c->p2, C2, 0);
+    conv_fwd(c, c->c3a, c->p2, C2, 0, c->e3a, C3, 0, H4, W4, B);
+    conv_fwd(c, c->c3b, c->e3a, C3, 0, c->e3, C3, 0, H4, W4, B);
+    hipLaunchKernelGGL(upsample_fwd, dim3(grid1((int64_t)B * H2 * W2 * C3)), dim3(256), 0, st, (const uint16_t*)c->e3,
+                       C3, C3, H4, W4, B, c->u2, C3 + C2, 0);
+    conv_fwd(c, c->c4a, c->u2, C3 + C2, 0, c->d2a, C2, 0, H2, W2, B);
+    conv_fwd(c, c->c4b, c->d2a, C2, 0, c->d2, C2, 0, H2, W2, B);
+    hipLaunchKernelGGL(upsample_fwd, dim3(grid1((int64_t)B * H * W * C2)), dim3(256), 0, st, (const uint16_t*)c->d2, C2,
+                       C2, H2, W2, B, c->u1, C2 + C1, 0);
+    conv_fwd(c, c->c5a, c->u1, C2 + C1, 0, c->d1a, C1, 0, H, W, B);
+    conv_fwd(c, c->c5b, c->d1a, C1, 0, c->d1, C1, 0, H, W, B);
+    hipLaunchKernelGGL(head_fwd, dim3(grid1((int64_t)B * H * W)), dim3(256), 0, st, (const uint16_t*)c->d1, C1,
+                       (int64_t)B * H * W, (const float*)c->hw, (const float*)c->hb, (const uint16_t*)c->x0, 32, 3, c->y);
+}
+
+static void backward(sf_cnn* c, int B) {
+    const int H = c->H, W = c->W, C1 = c->C1, C2 = c->C2, C3 = c->C3;
+    const int H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
+    const int64_t n1 = (int64_t)B * H * W, n2 = (int64_t)B * H2 * W2, n4 = (int64_t)B * H4 * W4;
+    hipStream_t st = c->ctx->stream;
+    hipLaunchKernelGGL(head_bwd, dim3(256), dim3(256), 0, st, (const uint16_t*)c->d1, C1, n1, (const float*)c->hw,
+                       (const float*)c->dy, c->ghw, c->ghb, c->g_d1);
+    relu_op(c, c->d1, C1, 0, c->g_d1, C1, 0, C1, n1);
+    conv_bwd(c, c->c5b, c->d1a, C1, 0, c->gop, c->g_d1a, C1, 0, H, W, B);
+    relu_op(c, c->d1a, C1, 0, c->g_d1a, C1, 0, C1, n1);
+    conv_bwd(c, c->c5a, c->u1, C2 + C1, 0, c->gop, c->g_u1, C2 + C1, 0, H, W, B);
+    hipLaunchKernelGGL(upsample_bwd, dim3(grid1(n2 * C2)), dim3(256), 0, st, (const float*)c->g_u1, C2 + C1, 0, C2,
+                       H2, W2, B, c->g_d2);
+    relu_op(c, c->d2, C2, 0, c->g_d2, C2, 0, C2, n2);
+    conv_bwd(c, c->c4b, c->d2a, C2, 0, c->gop, c->g_d2a, C2, 0, H2, W2, B);
+    relu_op(c, c->d2a, C2, 0, c->g_d2a, C2, 0, C2, n2);
+    conv_bwd(c, c->c4a, c->u2, C3 + C2, 0, c->gop, c->g_u2, C3 + C2, 0, H2, W2, B);
+    hipLaunchKernelGGL(upsample_bwd, dim3(grid1(n4 * C3)), dim3(256), 0, st, (const float*)c->g_u2, C3 + C2, 0, C3,
+                       H4, W4, B, c->g_e3);
+    relu_op(c, c->e3, C3, 0, c->g_e3, C3, 0, C3, n4);
+    conv_bwd(c, c->c3b, c->e3a, C3, 0, c->gop, c->g_e3a, C3, 0, H4, W4, B);
+    relu_op(c, c->e3a, C3, 0, c->g_e3a, C3, 0, C3, n4);
+    conv_bwd(c, c->c3a, c->p2, C2, 0, c->gop, c->g_p2, C2, 0, H4, W4, B);
+    hipLaunchKernelGGL(avgpool_bwd, dim3(grid1(n2 * C2)), dim3(256), 0, st, (const float*)c->g_p2, C2, 0, C2, H2, W2,
+                       B, c->g_u2, C3 + C2, C3);
+    relu_op(c, c->u2, C3 + C2, C3, c->g_u2, C3 + C2, C3, C2, n2);
+    conv_bwd(c, c->c2b, c->e2a, C2, 0, c->gop, c->g_e2a, C2, 0, H2, W2, B);
+    relu_op(c, c->e2a, C2, 0, c->g_e2a, C2, 0, C2, n2);
+    conv_bwd(c, c->c2a, c->p1, C1, 0, c->gop, c->g_p1, C1, 0, H2, W2, B);
+    hipLaunchKernelGGL(avgpool_bwd, dim3(grid1(n1 * C1)), dim3(256), 0, st, (const float*)c->g_p1, C1, 0, C1, H, W, B,
+                       c->g_u1, C2 + C1, C2);
+    relu_op(c, c->u1, C2 + C1, C2, c->g_u1, C2 + C1, C2, C1, n1);
+    conv_bwd(c, c->c1b, c->e1a, C1, 0, c->gop, c->g_e1a, C1, 0, H, W, B);
+    relu_op(c, c->e1a, C1, 0, c->g_e1a, C1, 0, C1, n1);
+    conv_bwd(c, c->c1a, c->x0, 32, 0, c->gop, nullptr, 0, 0, H, W, B);
+}
+
+static std::vector<std::pair<float*, size_t>> grads(sf_cnn* c) {
+    std::vector<std::pair<float*, size_t>> g;
+    for (ConvL* L : {&c->c1a, &c->c1b, &c->c2a, &c->c2b, &c->c3a, &c->c3b, &c->c4a, &c->c4b, &c->c5a, &c->c5b}) {
+        g.push_back({L->gw, L->nw()});
+        g.push_back({L->gb, (size_t)L->N});
+    }
+    g.push_back({c->ghw, (size_t)c->C1});
+    g.push_back({c->ghb, 1});
+    return g;
+}
+static std::vector<std::pair<float*, size_t>> params(sf_cnn* c) {
+    std::vector<std::pair<float*, size_t>> g;
+    for (ConvL* L : {&c->c1a, &c->c1b, &c->c2a, &c->c2b, &c->c3a, &c->c3b, &c->c4a, &c->c4b, &c->c5a, &c->c5b}) {
+        g.push_back({L->wm, L->nw()});
+        g.push_back({L->b, (size_t)L->N});
+    }
+    g.push_back({c->hw, (size_t)c->C1});
+    g.push_back({c->hb, 1});
+    return g;
+}
+
+// place sample s into batch slot b of x0 / target / mask
+__global__ void assemble_batch(const uint16_t* __restrict__ sin, const float* __restrict__ st,
+                               const uint8_t* __restrict__ sm, int64_t npix, uint16_t* __restrict__ x0,
+                               float* __restrict__ bt, uint8_t* __restrict__ bm) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
+        for (int c = 0; c < 4; c++) x0[p * 32 + c] = sin[p * 4 + c];
+        for (int c = 4; c < 32; c++) x0[p * 32 + c] = 0;
+        bt[p] = st[p];
+        bm[p] = sm[p];
+    }
+}
+// Snapshot of a context on the padded grid: input channels (rho / rho_ref, Dirichlet mask,
+// boundary phi / phi_ref, Boltzmann inversion (phi0 + kTe ln max(rho / (e n0), 1)) / phi_ref),
+// target phi / phi_ref and the free-node mask.  Pixels outside the mesh are zero.
+__global__ void snapshot_kernel(int ni, int nj, int H, int W, const double* __restrict__ rho,
+                                const double* __restrict__ phi, const int32_t* __restrict__ src,
+                                const uint8_t* __restrict__ cls, const double* __restrict__ fixval, double inv_rho,
+                                double inv_phi, double inv_en0, double kTe, double phi0,
+                                uint16_t* __restrict__ sin, float* __restrict__ st, uint8_t* __restrict__ sm) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < (int64_t)H * W; p += (int64_t)gridDim.x * blockDim.x) {
+        int x = (int)(p % W), y = (int)(p / W);
+        uint16_t r = 0, mk = 0, pb = 0, bz = 0;
+        float t = 0;
+        uint8_t m = 0;
+        if (x < ni && y < nj) {
+            size_t u = (size_t)y * ni + x;
+            bool fixed = src[u] < 0;
+            r = f2bf((float)(rho[u] * inv_rho));
+            mk = f2bf(fixed ? 1.f : 0.f);
+            pb = f2bf(fixed ? (float)(fixval[u] * inv_phi) : 0.f);
+            if (kTe > 0) bz = f2bf((float)((phi0 + kTe * log(fmax(rho[u] * inv_en0, 1.0))) * inv_phi));
+            if (phi) t = (float)(phi[u] * inv_phi);
+            m = cls[u] != 0xFF ? 1 : 0;
+        }
+        sin[p * 4 + 0] = r;
+        sin[p * 4 + 1] = mk;
+        sin[p * 4 + 2] = pb;
+        sin[p * 4 + 3] = bz;
+        if (st) st[p] = t;
+        if (sm) sm[p] = m;
+    }
+}
+// network output -> phi on the free nodes of the context (fixed / face nodes set by the solver)
+__global__ void guess_to_phi(int ni, int nj, int W, const float* __restrict__ y, double phi_ref,
+                             double* __restrict__ phi) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < (int64_t)ni * nj; u += (int64_t)gridDim.x * blockDim.x) {
+        int x = (int)(u % ni), yy = (int)(u / ni);
+        phi[u] = (double)y[(size_t)yy * W + x] * phi_ref;
+    }
+}
+
+}  // namespace sf
+
+// accessors used by the solver (defined in sf_poisson.hip)
+namespace sf {
+const int32_t* poisson_src(sf_ctx* ctx);
+const uint8_t* poisson_cls(sf_ctx* ctx);
+const double* poisson_fixval(sf_ctx* ctx);
+const sf_poisson_desc* poisson_desc(sf_ctx* ctx);
+int poisson_rel_residual(sf_ctx* ctx, double* rel);
+
+// parameters of the Boltzmann-inversion input channel (zero channel for a linear solver)
+static double feat_kTe(sf_ctx* t) { return poisson_desc(t)->linear ? 0.0 : poisson_desc(t)->Te0; }
+static double feat_inv_en0(sf_ctx* t) {
+    double n0 = poisson_desc(t)->n0;
+    return n0 > 0 ? 1.0 / (QE * n0) : 0.0;
+}
+}
+
+using namespace sf;
+
+extern "C" {
+
+int sf_cnn_create(sf_ctx* ctx, int ni, int nj, int max_batch, uint64_t seed, sf_cnn** out) {
+    SF_CHECK_CTX(ctx);
+    if (!out || max_batch < 1 || max_batch > 64) return sf_fail(ctx, SF_ERR_ARG, "sf_cnn_create: bad argument");
+    if (ni <= 0 || nj <= 0) {
+        if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "sf_cnn_create: no size given and no mesh");
+        ni = ctx->mp.ni;
+        nj = ctx->mp.nj;
+    }
+    if (ni > 16384 || nj > 16384) return sf_fail(ctx, SF_ERR_ARG, "sf_cnn_create: grid too large");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    sf_cnn* c = new sf_cnn();
+    c->ctx = ctx;
+    c->ni = ni;
+    c->nj = nj;
+    c->W = (c->ni + 3) / 4 * 4;
+    c->H = (c->nj + 3) / 4 * 4;
+    c->Bmax = max_batch;
+    c->rng.seed(seed);
+    const int C1 = c->C1, C2 = c->C2, C3 = c->C3;
+    const int64_t n1 = (int64_t)max_batch * c->H * c->W, n2 = n1 / 4, n4 = n1 / 16;
+    int rc = 0;
+    rc |= conv_init(c, c->c1a, 4, C1, c->rng);
+    rc |= conv_init(c, c->c1b, C1, C1, c->rng);
+    rc |= conv_init(c, c->c2a, C1, C2, c->rng);
+    rc |= conv_init(c, c->c2b, C2, C2, c->rng);
+    rc |= conv_init(c, c->c3a, C2, C3, c->rng);
+    rc |= conv_init(c, c->c3b, C3, C3, c->rng);
+    rc |= conv_init(c, c->c4a, C3 + C2, C2, c->rng);
+    rc |= conv_init(c, c->c4b, C2, C2, c->rng);
+    rc |= conv_init(c, c->c5a, C2 + C1, C1, c->rng);
+    rc |= conv_init(c, c->c5b, C1, C1, c->rng);
+    rc |= cnn_alloc(c, &c->hw, C1);
+    rc |= cnn_alloc(c, &c->hb, 1);
+    rc |= cnn_alloc(c, &c->ghw, C1);
+    rc |= cnn_alloc(c, &c->ghb, 1);
+    rc |= cnn_alloc(c, &c->mhw, C1);
+    rc |= cnn_alloc(c, &c->vhw, C1);
+    rc |= cnn_alloc(c, &c->mhb, 1);
+    rc |= cnn_alloc(c, &c->vhb, 1);
+    rc |= cnn_alloc(c, &c->x0, n1 * 32);
+    rc |= cnn_alloc(c, &c->e1a, n1 * C1);
+    rc |= cnn_alloc(c, &c->u1, n1 * (C2 + C1));
+    rc |= cnn_alloc(c, &c->p1, n2 * C1);
+    rc |= cnn_alloc(c, &c->e2a, n2 * C2);
+    rc |= cnn_alloc(c, &c->u2, n2 * (C3 + C2));
+    rc |= cnn_alloc(c, &c->p2, n4 * C2);
+    rc |= cnn_alloc(c, &c->e3a, n4 * C3);
+    rc |= cnn_alloc(c, &c->e3, n4 * C3);
+    rc |= cnn_alloc(c, &c->d2a, n2 * C2);
+    rc |= cnn_alloc(c, &c->d2, n2 * C2);
+    rc |= cnn_alloc(c, &c->d1a, n1 * C1);
+    rc |= cnn_alloc(c, &c->d1, n1 * C1);
+    rc |= cnn_alloc(c, &c->gop, n1 * (C2 + C1));
+    rc |= cnn_alloc(c, &c->y, n1);
+    rc |= cnn_alloc(c, &c->g_e1a, n1 * C1);
+    rc |= cnn_alloc(c, &c->g_u1, n1 * (C2 + C1));
+    rc |= cnn_alloc(c, &c->g_p1, n2 * C1);
+    rc |= cnn_alloc(c, &c->g_e2a, n2 * C2);
+    rc |= cnn_alloc(c, &c->g_u2, n2 * (C3 + C2));
+    rc |= cnn_alloc(c, &c->g_p2, n4 * C2);
+    rc |= cnn_alloc(c, &c->g_e3a, n4 * C3);
+    rc |= cnn_alloc(c, &c->g_e3, n4 * C3);
+    rc |= cnn_alloc(c, &c->g_d2a, n2 * C2);
+    rc |= cnn_alloc(c, &c->g_d2, n2 * C2);
+    rc |= cnn_alloc(c, &c->g_d1a, n1 * C1);
+    rc |= cnn_alloc(c, &c->g_d1, n1 * C1);
+    rc |= cnn_alloc(c, &c->dy, n1);
+    rc |= cnn_alloc(c, &c->loss, 4);
+    rc |= cnn_alloc(c, &c->bt, n1);
+    rc |= cnn_alloc(c, &c->bmask, n1);
+    rc |= cnn_alloc(c, &c->phi_save, (size_t)ni * nj);
+    if (rc) {
+        for (void* p : c->allocs) hipFree(p);
+        delete c;
+        return SF_ERR_OOM;
+    }
+    std::normal_distribution<double> nd(0.0, std::sqrt(1.0 / C1));
+    std::vector<float> h(C1);
+    for (auto& v : h) v = (float)nd(c->rng);
+    SF_HIP(ctx, hipMemcpy(c->hw, h.data(), C1 * 4, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipDeviceSynchronize());
+    *out = c;
+    return SF_OK;
+}
+
+int sf_cnn_destroy(sf_cnn* c) {
+    if (!c) return SF_ERR_ARG;
+    hipSetDevice(c->ctx->device);
+    hipStreamSynchronize(c->ctx->stream);
+    for (void* p : c->allocs) hipFree(p);
+    for (auto* p : c->s_in) hipFree(p);
+    for (auto* p : c->s_t) hipFree(p);
+    for (auto* p : c->s_m) hipFree(p);
+    delete c;
+    return SF_OK;
+}
+
+int sf_cnn_num_samples(const sf_cnn* c) { return c ? (int)c->s_in.size() : SF_ERR_ARG; }
+
+int sf_cnn_dims(const sf_cnn* c, int* H, int* W, int64_t* nparams) {
+    if (!c) return SF_ERR_ARG;
+    if (H) *H = c->H;
+    if (W) *W = c->W;
+    if (nparams) {
+        int64_t n = 0;
+        for (auto& p : params(const_cast<sf_cnn*>(c))) n += (int64_t)p.second;
+        *nparams = n;
+    }
+    return SF_OK;
+}
+
+// Adds a training sample given on the host: x4 [H][W][4] FP32 input channels (stored as BF16),
+// t [H][W] FP32 target, m [H][W] loss mask.
+int sf_cnn_add_sample(sf_cnn* c, const float* x4, const float* t, const uint8_t* m) {
+    if (!c || !x4 || !t || !m) return SF_ERR_ARG;
+    sf_ctx* ctx = c->ctx;
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t np = (int64_t)c->H * c->W;
+    std::vector<uint16_t> h(np * 4);
+    for (int64_t i = 0; i < np * 4; i++) h[i] = f2bf(x4[i]);
+    uint16_t* sin = nullptr;
+    float* st = nullptr;
+    uint8_t* sm = nullptr;
+    SF_HIP(ctx, hipMalloc((void**)&sin, np * 8));
+    SF_HIP(ctx, hipMalloc((void**)&st, np * 4));
+    SF_HIP(ctx, hipMalloc((void**)&sm, np));
+    SF_HIP(ctx, hipMemcpy(sin, h.data(), np * 8, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(st, t, np * 4, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(sm, m, np, hipMemcpyHostToDevice));
+    c->s_in.push_back(sin);
+    c->s_t.push_back(st);
+    c->s_m.push_back(sm);
+    return SF_OK;
+}
+
+// Reads back sample idx (input channels as stored, i.e. BF16-rounded, widened to FP32).
+int sf_cnn_get_sample(sf_cnn* c, int idx, float* x4, float* t, uint8_t* m) {
+    if (!c || idx < 0 || idx >= (int)c->s_in.size()) return SF_ERR_ARG;
+    sf_ctx* ctx = c->ctx;
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t np = (int64_t)c->H * c->W;
+    if (x4) {
+        std::vector<uint16_t> h(np * 4);
+        SF_HIP(ctx, hipMemcpy(h.data(), c->s_in[idx], np * 8, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < np * 4; i++) x4[i] = bf2f(h[i]);
+    }
+    if (t) SF_HIP(ctx, hipMemcpy(t, c->s_t[idx], np * 4, hipMemcpyDeviceToHost));
+    if (m) SF_HIP(ctx, hipMemcpy(m, c->s_m[idx], np, hipMemcpyDeviceToHost));
+    return SF_OK;
+}
+
+// Adds the current (rho, Dirichlet mask / values, phi) of `src_ctx` to the training set.
+int sf_cnn_snapshot(sf_cnn* c, sf_ctx* src_ctx) {
+    if (!c || !src_ctx) return SF_ERR_ARG;
+    sf_ctx* ctx = c->ctx;
+    if (src_ctx->mp.ni != c->ni || src_ctx->mp.nj != c->nj || !src_ctx->poisson)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_cnn_snapshot: context has another mesh size or no solver");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(src_ctx->stream));
+    const int64_t np = (int64_t)c->H * c->W;
+    uint16_t* sin = nullptr;
+    float* st = nullptr;
+    uint8_t* sm = nullptr;
+    SF_HIP(ctx, hipMalloc((void**)&sin, np * 8));
+    SF_HIP(ctx, hipMalloc((void**)&st, np * 4));
+    SF_HIP(ctx, hipMalloc((void**)&sm, np));
+    hipLaunchKernelGGL(snapshot_kernel, dim3(grid1(np)), dim3(256), 0, ctx->stream, c->ni, c->nj, c->H, c->W,
+                       (const double*)src_ctx->d_rho, (const double*)src_ctx->d_phi, poisson_src(src_ctx),
+                       poisson_cls(src_ctx), poisson_fixval(src_ctx), 1.0 / c->rho_ref, 1.0 / c->phi_ref,
+                       feat_inv_en0(src_ctx), feat_kTe(src_ctx), poisson_desc(src_ctx)->phi0, sin, st, sm);
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    c->s_in.push_back(sin);
+    c->s_t.push_back(st);
+    c->s_m.push_back(sm);
+    return SF_OK;
+}
+
+// Adam training on random mini-batches of the snapshots; returns the mean loss of the last
+// `steps/10` steps in *loss_out (normalised MSE of phi / 100 V on free nodes).
+int sf_cnn_train(sf_cnn* c, int steps, int batch, double lr, double* loss_out) {
+    if (!c) return SF_ERR_ARG;
+    sf_ctx* ctx = c->ctx;
+    if (c->s_in.empty()) return sf_fail(ctx, SF_ERR_STATE, "sf_cnn_train: no snapshots");
+    if (batch < 1 || batch > c->Bmax || steps < 0) return sf_fail(ctx, SF_ERR_ARG, "sf_cnn_train: bad batch/steps");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int64_t np = (int64_t)c->H * c->W;
+    std::uniform_int_distribution<int> pick(0, (int)c->s_in.size() - 1);
+    double tail = 0;
+    int ntail = 0;
+    auto P = params(c);
+    auto G = grads(c);
+    std::vector<std::pair<float*, float*>> mv;
+    for (ConvL* L : {&c->c1a, &c->c1b, &c->c2a, &c->c2b, &c->c3a, &c->c3b, &c->c4a, &c->c4b, &c->c5a, &c->c5b}) {
+        mv.push_back({L->mw, L->vw});
+        mv.push_back({L->mb, L->vb});
+    }
+    mv.push_back({c->mhw, c->vhw});
+    mv.push_back({c->mhb, c->vhb});
+    for (int s = 0; s < steps; s++) {
+        int64_t counted = 0;
+        for (int b = 0; b < batch; b++) {
+            int k = pick(c->rng);
+            hipLaunchKernelGGL(assemble_batch, dim3(grid1(np)), dim3(256), 0, st, (const uint16_t*)c->s_in[k],
+                               (const float*)c->s_t[k], (const uint8_t*)c->s_m[k], np, c->x0 + b * np * 32,
+                               c->bt + b * np, c->bmask + b * np);
+            counted += np;
+        }
+        for (auto& g : G) SF_HIP(ctx, hipMemsetAsync(g.first, 0, g.second * 4, st));
+        SF_HIP(ctx, hipMemsetAsync(c->loss, 0, 4, st));
+        forward(c, batch);
+        hipLaunchKernelGGL(mse_loss, dim3(grid1((int64_t)batch * np)), dim3(256), 0, st, (const float*)c->y,
+                           (const float*)c->bt, (const uint8_t*)c->bmask, (int64_t)batch * np, 1.0f / (float)counted,
+                           c->dy, c->loss);
+        backward(c, batch);
+        c->adam_t++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - powf(b1, (float)c->adam_t), c2 = 1.f - powf(b2, (float)c->adam_t);
+        for (size_t q = 0; q < P.size(); q++)
+            hipLaunchKernelGGL(adam_step, dim3(grid1((int64_t)P[q].second)), dim3(256), 0, st, P[q].first,
+                               (const float*)G[q].first, mv[q].first, mv[q].second, (int64_t)P[q].second, (float)lr,
+                               b1, b2, 1e-8f, c1, c2);
+        SF_HIP(ctx, hipGetLastError());
+        if (s >= steps - std::max(1, steps / 10)) {
+            float l = 0;
+            SF_HIP(ctx, hipMemcpyAsync(&l, c->loss, 4, hipMemcpyDeviceToHost, st));
+            SF_HIP(ctx, hipStreamSynchronize(st));
+            tail += l;
+            ntail++;
+        }
+    }
+    SF_HIP(ctx, hipStreamSynchronize(st));
+    if (loss_out) *loss_out = ntail ? tail / ntail : 0.0;
+    return SF_OK;
+}
+
+// Loss and all parameter gradients of one sample (for gradient checks).  grads_out receives the
+// gradients in the order of sf_cnn_params; n_out the total count.
+int sf_cnn_loss_and_grads(sf_cnn* c, int sample, double* loss_out, float* y_out, float* grads_out,
+                          int64_t* n_out) {
+    if (!c || sample < 0 || sample >= (int)c->s_in.size()) return SF_ERR_ARG;
+    sf_ctx* ctx = c->ctx;
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int64_t np = (int64_t)c->H * c->W;
+    hipLaunchKernelGGL(assemble_batch, dim3(grid1(np)), dim3(256), 0, st, (const uint16_t*)c->s_in[sample],
+                       (const float*)c->s_t[sample], (const uint8_t*)c->s_m[sample], np, c->x0, c->bt, c->bmask);
+    auto G = grads(c);
+    for (auto& g : G) SF_HIP(ctx, hipMemsetAsync(g.first, 0, g.second * 4, st));
+    SF_HIP(ctx, hipMemsetAsync(c->loss, 0, 4, st));
+    forward(c, 1);
+    hipLaunchKernelGGL(mse_loss, dim3(grid1(np)), dim3(256), 0, st, (const float*)c->y, (const float*)c->bt,
+                       (const uint8_t*)c->bmask, np, 1.0f / (float)np, c->dy, c->loss);
+    backward(c, 1);
+    SF_HIP(ctx, hipGetLastError());
+    float l = 0;
+    SF_HIP(ctx, hipMemcpyAsync(&l, c->loss, 4, hipMemcpyDeviceToHost, st));
+    SF_HIP(ctx, hipStreamSynchronize(st));
+    if (loss_out) *loss_out = l;
+    if (y_out) SF_HIP(ctx, hipMemcpy(y_out, c->y, np * 4, hipMemcpyDeviceToHost));
+    int64_t off = 0;
+    for (auto& g : G) {
+        if (grads_out) SF_HIP(ctx, hipMemcpy(grads_out + off, g.first, g.second * 4, hipMemcpyDeviceToHost));
+        off += (int64_t)g.second;
+    }
+    if (n_out) *n_out = off;
+    return SF_OK;
+}
+
+// parameters (FP32 master copies) in a fixed order: for each of the 10 convolutions
+// W[9][N][K] then b[N]; then the head w[C1], b[1]. get = 1 downloads, get = 0 uploads.
+int sf_cnn_params(sf_cnn* c, int get, float* buf, int64_t* n_out) {
+    if (!c) return SF_ERR_ARG;
+    sf_ctx* ctx = c->ctx;
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t off = 0;
+    for (auto& p : params(c)) {
+        if (buf) {
+            if (get) SF_HIP(ctx, hipMemcpy(buf + off, p.first, p.second * 4, hipMemcpyDeviceToHost));
+            else SF_HIP(ctx, hipMemcpy(p.first, buf + off, p.second * 4, hipMemcpyHostToDevice));
+        }
+        off += (int64_t)p.second;
+    }
+    if (n_out) *n_out = off;
+    return SF_OK;
+}
+
+int sf_cnn_predict(sf_cnn* c, sf_ctx* target);
+
+// Potential solve started from the network's guess.  policy 0: always start from the guess;
+// policy 1: start from whichever of the current potential and the guess has the smaller
+// nonlinear residual (the guess is discarded when it does not help).  info->newton_iterations
+// etc. describe the solve; *used_guess (may be NULL) tells which start was taken.
+int sf_poisson_solve_learned(sf_ctx* ctx, sf_cnn* c, int policy, sf_solve_info* info, int* used_guess) {
+    SF_CHECK_CTX(ctx);
+    if (!c || (policy != 0 && policy != 1)) return sf_fail(ctx, SF_ERR_ARG, "sf_poisson_solve_learned: bad argument");
+    if (!ctx->poisson) return sf_fail(ctx, SF_ERR_STATE, "sf_poisson_solve_learned: solver not set up");
+    if (ctx->mp.ni != c->ni || ctx->mp.nj != c->nj)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_poisson_solve_learned: network grid differs from the mesh");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t nn = (size_t)c->ni * c->nj;
+    double r_prev = 0, r_guess = 0;
+    if (policy == 1) {
+        int rc = poisson_rel_residual(ctx, &r_prev);
+        if (rc < 0) return rc;
+        SF_HIP(ctx, hipMemcpyAsync(c->phi_save, ctx->d_phi, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    int rc = sf_cnn_predict(c, ctx);
+    if (rc < 0) return sf_fail(ctx, rc, "sf_poisson_solve_learned: prediction failed: %s", c->ctx->err.c_str());
+    bool use = true;
+    if (policy == 1) {
+        rc = poisson_rel_residual(ctx, &r_guess);
+        if (rc < 0) return rc;
+        if (!(r_guess < r_prev)) {
+            use = false;
+            SF_HIP(ctx, hipMemcpyAsync(ctx->d_phi, c->phi_save, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    }
+    if (used_guess) *used_guess = use ? 1 : 0;
+    return poisson_solve(ctx, info);
+}
+
+// Network output for the current state of `target` (rho, Dirichlet values) written as phi.
+int sf_cnn_predict(sf_cnn* c, sf_ctx* target) {
+    if (!c || !target) return SF_ERR_ARG;
+    sf_ctx* ctx = c->ctx;
+    if (target->mp.ni != c->ni || target->mp.nj != c->nj || !target->poisson)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_cnn_predict: context has another mesh size or no solver");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(target->stream));
+    const int64_t np = (int64_t)c->H * c->W;
+    uint16_t* sin = c->gop;   // scratch [np][4]
+    hipLaunchKernelGGL(snapshot_kernel, dim3(grid1(np)), dim3(256), 0, ctx->stream, c->ni, c->nj, c->H, c->W,
+                       (const double*)target->d_rho, (const double*)nullptr, poisson_src(target), poisson_cls(target),
+                       poisson_fixval(target), 1.0 / c->rho_ref, 1.0 / c->phi_ref, feat_inv_en0(target),
+                       feat_kTe(target), poisson_desc(target)->phi0, sin, (float*)nullptr, (uint8_t*)nullptr);
+    hipLaunchKernelGGL(assemble_batch, dim3(grid1(np)), dim3(256), 0, ctx->stream, (const uint16_t*)sin,
+                       (const float*)c->bt, (const uint8_t*)c->bmask, np, c->x0, c->bt, c->bmask);
+    forward(c, 1);
+    hipLaunchKernelGGL(guess_to_phi, dim3(grid1((int64_t)c->ni * c->nj)), dim3(256), 0, ctx->stream, c->ni, c->nj,
+                       c->W, (const float*)c->y, c->phi_ref, target->d_phi);
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SF_OK;
+}
+
+// MFMA lane-layout probe: D = A(16x32) * B(32x16) with the operand layout the convolution
+// kernels assume; host compares against a plain product.
+__global__ void mfma_probe_kernel(const uint16_t* a, const uint16_t* b, float* d) {
+    const int lane = threadIdx.x;
+    bf16x8 av, bv;
+    uint16_t ta[8], tb[8];
+    for (int j = 0; j < 8; j++) {
+        ta[j] = a[(lane & 15) * 32 + (lane >> 4) * 8 + j];   // A[row][k]
+        tb[j] = b[((lane >> 4) * 8 + j) * 16 + (lane & 15)];  // B[k][col]
+    }
+    memcpy(&av, ta, 16);
+    memcpy(&bv, tb, 16);
+    f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    for (int i = 0; i < 4; i++) d[((lane >> 4) * 4 + i) * 16 + (lane & 15)] = acc[i];
+}
+
+int sf_cnn_mfma_probe(sf_ctx* ctx, const float* a, const float* b, float* d) {
+    SF_CHECK_CTX(ctx);
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<uint16_t> ha(16 * 32), hb(32 * 16);
+    for (int i = 0; i < 16 * 32; i++) ha[i] = f2bf(a[i]);
+    for (int i = 0; i < 32 * 16; i++) hb[i] = f2bf(b[i]);
+    uint16_t *da = nullptr, *db = nullptr;
+    float* dd = nullptr;
+    SF_HIP(ctx, hipMalloc((void**)&da, 16 * 32 * 2));
+    SF_HIP(ctx, hipMalloc((void**)&db, 32 * 16 * 2));
+    SF_HIP(ctx, hipMalloc((void**)&dd, 16 * 16 * 4));
+    SF_HIP(ctx, hipMemcpy(da, ha.data(), 16 * 32 * 2, hipMemcpyHostToDevice));
+    SF_HIP(ctx, hipMemcpy(db, hb.data(), 32 * 16 * 2, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, ctx->stream, (const uint16_t*)da, (const uint16_t*)db,
+                       dd);
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SF_HIP(ctx, hipMemcpy(d, dd, 16 * 16 * 4, hipMemcpyDeviceToHost));
+    hipFree(da);
+    hipFree(db);
+    hipFree(dd);
+    return SF_OK;
+}
+
+}  // extern "C"

@@ -948,3 +948,45 @@ int sf_operator_apply(sf_ctx* ctx, const double* x, double* y) {
 }
 
 }  // extern "C"
+
+namespace sf {
+// node classification of the reduced system, read by the learned-guess network (sf_cnn.hip)
+const int32_t* poisson_src(sf_ctx* ctx) { return ctx->poisson ? ctx->poisson->src : nullptr; }
+const uint8_t* poisson_cls(sf_ctx* ctx) { return ctx->poisson ? ctx->poisson->cls : nullptr; }
+const double* poisson_fixval(sf_ctx* ctx) { return ctx->poisson ? ctx->poisson->fixval : nullptr; }
+}  // namespace sf
+namespace sf {
+const sf_poisson_desc* poisson_desc(sf_ctx* ctx) { return ctx->poisson ? &ctx->poisson->desc : nullptr; }
+}  // namespace sf
+
+namespace sf {
+// Relative nonlinear residual ||F(phi)|| / ||F(0)|| of the context's current potential, with the
+// norms of the solve's stopping test.  Overwrites the solver's residual work vectors only.
+int poisson_rel_residual(sf_ctx* ctx, double* rel) {
+    PoissonState* P = ctx->poisson;
+    if (!P) return sf_fail(ctx, SF_ERR_STATE, "poisson solver not set up");
+    const MeshP& m = ctx->mp;
+    const int64_t nn = (int64_t)m.ni * m.nj;
+    const sf_poisson_desc& D = P->desc;
+    hipStream_t st = ctx->stream;
+    const int ge = P->nblk_elem;
+    const double inv_eps = 1.0 / P->eps;
+    const double Cn0 = D.linear ? 0.0 : P->C * D.n0;
+    const double inv_kTe = D.linear ? 0.0 : 1.0 / D.Te0;
+    hipLaunchKernelGGL(gather_free, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, (const double*)ctx->d_phi,
+                       P->x);
+    hipLaunchKernelGGL(ref_norm_kernel, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
+                       (const double*)P->bfix, (const double*)ctx->d_rho, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
+                       P->partial, P->ticket, P->scal, (int)S_R0);
+    hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
+                       (const double*)P->bfix, (const double*)ctx->d_rho, (const double*)P->x, P->r, P->D, P->P,
+                       inv_eps, Cn0, D.phi0, inv_kTe, D.linear, P->partial, P->ticket, P->scal, (int)S_NORM2);
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
+    SF_HIP(ctx, hipStreamSynchronize(st));
+    double ref = std::sqrt(P->host_scal[S_R0]);
+    if (!(ref > 0)) ref = 1.0;
+    *rel = std::sqrt(P->host_scal[S_NORM2]) / ref;
+    return SF_OK;
+}
+}  // namespace sf

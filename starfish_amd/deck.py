@@ -285,18 +285,19 @@ def config_c1(seed=1, steps=200):
     return deck
 
 
-def config_c2(steps=500, particles=20_000_000):
+def config_c2(steps=500, particles=20_000_000, radius=0.05, bias=-100.0, ni=1024, nj=512):
     """Same physics at 1024x512 nodes, dx = 1e-3 m: 2e7 O+ preloaded uniformly over the open cells
-    with the drifting Maxwellian plus left-edge injection."""
-    mesh = MeshSpec("XY", 1024, 512, (-0.15, 0.0), (1e-3, 1e-3),
-                    {"LEFT": ("DIRICHLET", 0.0), "BOTTOM": ("SYMMETRY", 0.0)})
+    with the drifting Maxwellian plus left-edge injection.  radius / bias vary the cylinder and
+    ni, nj the resolution of the same 1.023 m x 0.511 m domain."""
     Lx, Ly = 1023e-3, 511e-3
+    mesh = MeshSpec("XY", ni, nj, (-0.15, 0.0), (round(Lx / (ni - 1), 12), round(Ly / (nj - 1), 12)),
+                    {"LEFT": ("DIRICHLET", 0.0), "BOTTOM": ("SYMMETRY", 0.0)})
     n_ion = 1e20 / 7000.0                          # density of the inflowing beam
-    area = Lx * Ly - 0.5 * math.pi * 0.05 ** 2
+    area = Lx * Ly - 0.5 * math.pi * radius ** 2
     spwt = n_ion * area / particles
     mdot = ion_source_mdot(1e20, Ly, 16.0)
     deck = Deck(mesh,
-                boundaries=[BoundarySpec("cylinder", "DIRICHLET", -100.0, circle_path(0.05, 128), False, "SS"),
+                boundaries=[BoundarySpec("cylinder", "DIRICHLET", bias, circle_path(radius, 128), False, "SS"),
                             BoundarySpec("inlet", "VIRTUAL", 0.0, [(-0.15, Ly), (-0.15, 0.0)], False, None)],
                 materials=[MaterialSpec("O+", "kinetic_hip", 16.0, 1.0, spwt, 0.0),
                            MaterialSpec("SS", "solid", 52.3, 0.0, 1.0, 0.0)],
