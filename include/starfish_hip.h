@@ -238,7 +238,9 @@ int sf_source_maxwellian(sf_ctx* ctx, int species, int npts, const double* pts, 
 int sf_sources_sample(sf_ctx* ctx, int64_t* injected);
 
 /* ------------------------------------------------------------------------------------------
- * Timing of the last sf_push_deposit (device ms): push+deposit kernel, sort, normalise.
+ * Timing of the last sf_push_deposit (device ms, with sf_set_timing(ctx, 1)): out4[0] pass A
+ * (push + deposit), out4[1] cell scan (sort step) or macro-particle count (in-place step),
+ * out4[2] pass B (push + scatter; exactly 0 on an in-place step), out4[3] normalisation.
  * ------------------------------------------------------------------------------------------ */
 int sf_last_timings(sf_ctx* ctx, double* out4);
 int sf_set_timing(sf_ctx* ctx, int enabled);
@@ -280,6 +282,10 @@ int sf_cnn_predict(sf_cnn* cnn, sf_ctx* target);
    policy 1: from whichever of the current phi and the guess has the smaller nonlinear residual.
    *used_guess (may be NULL) reports the choice.  Same tolerance and status as sf_poisson_solve. */
 int sf_poisson_solve_learned(sf_ctx* ctx, sf_cnn* cnn, int policy, sf_solve_info* info, int* used_guess);
+/* diagnostics: activation `which` of batch slot 0 after the last forward pass, as FP32
+   (0 input [H][W][32], 1 e1a, 2 u1 = up(d2)++e1, 3 p1, 4 e2a, 5 u2 = up(e3)++e2, 6 p2, 7 e3a,
+   8 e3, 9 d2a, 10 d2, 11 d1a, 12 d1, 13 output y); *count = number of values (out may be NULL) */
+int sf_cnn_activation(sf_cnn* cnn, int which, float* out, int64_t* count);
 /* d[16][16] = a[16][32] * b[32][16] through one BF16 matrix-core instruction with the operand
    layout the convolution kernels use (a, b rounded to BF16) */
 int sf_cnn_mfma_probe(sf_ctx* ctx, const float* a, const float* b, float* d);

@@ -117,10 +117,12 @@ def up_bwd(g):
     return g[0::2, 0::2] + g[0::2, 1::2] + g[1::2, 0::2] + g[1::2, 1::2]
 
 
-def loss_and_grads(flat, x4, target, mask, use_bf16=True):
+def loss_and_grads(flat, x4, target, mask, use_bf16=True, acts_out=None):
     """Loss = sum over masked pixels of (y - t)^2 / (H W); returns (loss, y, flat gradient).
 
-    x4 [H][W][4] input channels, target [H][W], mask [H][W] (nonzero = counted)."""
+    x4 [H][W][4] input channels, target [H][W], mask [H][W] (nonzero = counted).  If acts_out
+    is a dict it receives the stored activations under the device's names (x0, e1a, u1, p1,
+    e2a, u2, p2, e3a, e3, d2a, d2, d1a, d1, y)."""
     R = bf16 if use_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     p = unpack(np.asarray(flat))
     Wq = {name: R(p[name + ".W"]) for name, _, _ in LAYERS}
@@ -141,6 +143,10 @@ def loss_and_grads(flat, x4, target, mask, use_bf16=True):
     d2 = conv("c4b", conv("c4a", np.concatenate([up(e3), e2], -1)))
     d1 = conv("c5b", conv("c5a", np.concatenate([up(d2), e1], -1)))
     y = x0[..., 3] + d1 @ p["head.w"] + p["head.b"][0]
+    if acts_out is not None:
+        acts_out.update(x0=x0, e1a=acts["c1a"][1], u1=np.concatenate([up(d2), e1], -1), p1=acts["c2a"][0],
+                        e2a=acts["c2a"][1], u2=np.concatenate([up(e3), e2], -1), p2=acts["c3a"][0],
+                        e3a=acts["c3a"][1], e3=e3, d2a=acts["c4a"][1], d2=d2, d1a=acts["c5a"][1], d1=d1, y=y)
     n = H * W
     e = np.where(mask != 0, y - target, 0.0)
     loss = float((e * e).sum() / n)

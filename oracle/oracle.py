@@ -67,6 +67,7 @@ def load():
         "orc_operator_csr": (None, [P, C.c_int, C.POINTER(C.c_int64), IP, DP]),
         "orc_fixed_nodes": (C.c_int, [P, IP]),
         "orc_set_rng_state": (None, [P, C.c_int64]),
+        "orc_round_state_fp32": (None, [P, C.c_int]),
         "orc_javarandom_doubles": (None, [C.c_int64, C.c_int, DP]),
         "orc_javarandom_ints": (None, [C.c_int64, C.c_int, IP]),
         "orc_javarandom_ints_bounded": (None, [C.c_int64, C.c_int, C.c_int, IP]),
@@ -184,6 +185,10 @@ class Oracle:
 
     def step(self, n=1):
         self.lib.orc_step(self.h, n)
+
+    def round_state_fp32(self, sp):
+        """Rounds offsets and velocities to FP32, as the device stores the particle state."""
+        self.lib.orc_round_state_fp32(self.h, sp)
 
     def stats(self, sp):
         out = np.zeros(5)

@@ -1305,3 +1305,20 @@ void orc_javarandom_ints_bounded(int64_t seed, int bound, int n, int* out) {
 }  // extern "C"
 
 extern "C" void orc_set_rng_state(orc_sim* s, int64_t state) { s->rnd.seed = (uint64_t)state & ((1ULL << 48) - 1); }
+
+// Test hook: rounds every particle's in-cell offsets and velocities to FP32, as the device stores
+// them (int32 cell + FP32 offset, FP32 velocity); positions follow from the rounded offsets.
+extern "C" void orc_round_state_fp32(orc_sim* s, int spi) {
+    Species& sp = *s->species[spi];
+    for (auto& q : sp.parts) {
+        for (int d = 0; d < 2; d++) {
+            double c = std::floor(q.lc[d]);
+            q.lc[d] = c + (double)(float)(q.lc[d] - c);
+        }
+        for (int d = 0; d < 3; d++) q.vel[d] = (double)(float)q.vel[d];
+        double x[2];
+        s->mesh.pos(q.lc[0], q.lc[1], x);
+        q.pos[0] = x[0];
+        q.pos[1] = x[1];
+    }
+}

@@ -105,6 +105,7 @@ SIGNATURES = {
     "sf_cnn_params": (C.c_int, [P, C.c_int, FP, C.POINTER(I64)]),
     "sf_cnn_predict": (C.c_int, [P, P]),
     "sf_poisson_solve_learned": (C.c_int, [P, P, C.c_int, C.POINTER(SolveInfo), IP]),
+    "sf_cnn_activation": (C.c_int, [P, C.c_int, FP, C.POINTER(I64)]),
     "sf_cnn_mfma_probe": (C.c_int, [P, FP, FP, FP]),
 }
 

@@ -863,6 +863,47 @@ int sf_cnn_loss_and_grads(sf_cnn* c, int sample, double* loss_out, float* y_out,
     return SF_OK;
 }
 
+// Diagnostic read-back of an activation of batch slot 0 after the last forward pass, widened to
+// FP32: which = 0 x0 [H][W][32], 1 e1a, 2 u1 [.][96], 3 p1, 4 e2a, 5 u2 [.][192], 6 p2, 7 e3a,
+// 8 e3, 9 d2a, 10 d2, 11 d1a, 12 d1, 13 y [H][W].  *count receives the number of values.
+int sf_cnn_activation(sf_cnn* c, int which, float* out, int64_t* count) {
+    if (!c) return SF_ERR_ARG;
+    sf_ctx* ctx = c->ctx;
+    const int64_t n1 = (int64_t)c->H * c->W, n2 = n1 / 4, n4 = n1 / 16;
+    const int C1 = c->C1, C2 = c->C2, C3 = c->C3;
+    const uint16_t* src = nullptr;
+    int64_t n = 0;
+    switch (which) {
+        case 0: src = c->x0; n = n1 * 32; break;
+        case 1: src = c->e1a; n = n1 * C1; break;
+        case 2: src = c->u1; n = n1 * (C2 + C1); break;
+        case 3: src = c->p1; n = n2 * C1; break;
+        case 4: src = c->e2a; n = n2 * C2; break;
+        case 5: src = c->u2; n = n2 * (C3 + C2); break;
+        case 6: src = c->p2; n = n4 * C2; break;
+        case 7: src = c->e3a; n = n4 * C3; break;
+        case 8: src = c->e3; n = n4 * C3; break;
+        case 9: src = c->d2a; n = n2 * C2; break;
+        case 10: src = c->d2; n = n2 * C2; break;
+        case 11: src = c->d1a; n = n1 * C1; break;
+        case 12: src = c->d1; n = n1 * C1; break;
+        case 13: n = n1; break;
+        default: return sf_fail(ctx, SF_ERR_ARG, "sf_cnn_activation: unknown tensor %d", which);
+    }
+    if (count) *count = n;
+    if (!out) return SF_OK;
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (which == 13) {
+        SF_HIP(ctx, hipMemcpy(out, c->y, n * 4, hipMemcpyDeviceToHost));
+        return SF_OK;
+    }
+    std::vector<uint16_t> h(n);
+    SF_HIP(ctx, hipMemcpy(h.data(), src, n * 2, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) out[i] = bf2f(h[i]);
+    return SF_OK;
+}
+
 // parameters (FP32 master copies) in a fixed order: for each of the 10 convolutions
 // W[9][N][K] then b[N]; then the head w[C1], b[1]. get = 1 downloads, get = 0 uploads.
 int sf_cnn_params(sf_cnn* c, int get, float* buf, int64_t* n_out) {

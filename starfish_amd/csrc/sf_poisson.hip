@@ -58,6 +58,60 @@ struct PoissonState {
 // device scalars layout
 enum { S_RZ = 0, S_RZ_OLD, S_PQ, S_RR, S_TOL2, S_NORM2, S_R0, S_YMAX, S_NSCAL = 8 };
 
+// Grid-wide reduction with a last-block ticket: every block reduces its NV values (sums, or
+// maxima of non-negative values when MAX) and stores them in partial[NV * block + k]; the last
+// block to arrive reduces all partials with all of its threads.  Returns true in every thread
+// of that block, with the totals in tot[]; the caller finalises there.  The summation order is
+// fixed for a given grid, so results are reproducible.
+template <int NV, bool MAX = false>
+__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double* __restrict__ partial, unsigned int* ticket,
+                                            double (&tot)[NV]) {
+    __shared__ double red[NV][16];
+    __shared__ int last;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        for (int o = 32; o > 0; o >>= 1) v[k] = op(v[k], __shfl_xor(v[k], o));
+        if (lane == 0) red[k][wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; k++) {
+            double s = red[k][0];
+            for (int w = 1; w < nw; w++) s = op(s, red[k][w]);
+            partial[NV * blockIdx.x + k] = s;
+        }
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double b[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) b[k] = 0.0;
+    for (unsigned int blk = threadIdx.x; blk < gridDim.x; blk += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; k++) b[k] = op(b[k], ((volatile double*)partial)[NV * blk + k]);
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+        for (int o = 32; o > 0; o >>= 1) b[k] = op(b[k], __shfl_xor(b[k], o));
+    __syncthreads();   // thread 0 has finished reading red[]
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) red[k][wave] = b[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double t = red[k][0];
+        for (int w = 1; w < nw; w++) t = op(t, red[k][w]);
+        tot[k] = t;
+    }
+    if (threadIdx.x == 0) *ticket = 0;
+    return true;
+}
+
 void poisson_free(PoissonState* p) {
     if (!p) return;
     void* ptrs[] = {p->cls, p->W,  p->bfix, p->src, p->fixval, p->x,  p->r,       p->y,    p->rr,
@@ -132,7 +186,6 @@ __global__ void newton_residual(MeshP m, const uint8_t* __restrict__ cls, const 
                                 double* __restrict__ P, double inv_eps, double Cn0, double phi0, double inv_kTe,
                                 int linear, double* __restrict__ partial, unsigned int* ticket,
                                 double* __restrict__ scal, int slot) {
-    __shared__ double sh[16];
     const int64_t nn = (int64_t)m.ni * m.nj;
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
@@ -153,23 +206,8 @@ __global__ void newton_residual(MeshP m, const uint8_t* __restrict__ cls, const 
         D[u] = (float)(diagL + Wu * bp);
         acc += ru * ru;
     }
-    // block reduction, last block finalises
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w];
-        partial[blockIdx.x] = s;
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            double tot = 0;
-            for (unsigned int b = 0; b < gridDim.x; b++) tot += ((volatile double*)partial)[b];
-            scal[slot] = tot;
-            *ticket = 0;
-        }
-    }
+    double v[1] = {acc}, tot[1];
+    if (grid_reduce<1>(v, partial, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
 }
 
 // ||f + bfix - W b(0)||^2 for the fixed reference norm
@@ -177,7 +215,6 @@ __global__ void ref_norm_kernel(MeshP m, const uint8_t* __restrict__ cls, const 
                                 const double* __restrict__ bfix, const double* __restrict__ rho, double inv_eps,
                                 double Cn0, double phi0, double inv_kTe, int linear, double* __restrict__ partial,
                                 unsigned int* ticket, double* __restrict__ scal, int slot) {
-    __shared__ double sh[16];
     const int64_t nn = (int64_t)m.ni * m.nj;
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
@@ -186,22 +223,8 @@ __global__ void ref_norm_kernel(MeshP m, const uint8_t* __restrict__ cls, const 
         double v = W[u] * rho[u] * inv_eps + bfix[u] - W[u] * b0;
         acc += v * v;
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w];
-        partial[blockIdx.x] = s;
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            double tot = 0;
-            for (unsigned int b = 0; b < gridDim.x; b++) tot += ((volatile double*)partial)[b];
-            scal[slot] = tot;
-            *ticket = 0;
-        }
-    }
+    double v[1] = {acc}, tot[1];
+    if (grid_reduce<1>(v, partial, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
 }
 
 // refinement residual rr = r - (L + P) y (FP64) and ||rr||^2; also initialises the FP32 PCG:
@@ -212,7 +235,6 @@ __global__ void refine_residual(MeshP m, const uint8_t* __restrict__ cls, const 
                                 float* __restrict__ res, float* __restrict__ z, float* __restrict__ p0,
                                 double* __restrict__ partial, unsigned int* ticket, double* __restrict__ scal,
                                 int* istate) {
-    __shared__ double sh[2][16];
     const int64_t nn = (int64_t)m.ni * m.nj;
     double a_rr = 0, a_rz = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
@@ -236,32 +258,13 @@ __global__ void refine_residual(MeshP m, const uint8_t* __restrict__ cls, const 
         a_rr += v * v;
         a_rz += (double)rf * (double)zf;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        a_rr += __shfl_xor(a_rr, o);
-        a_rz += __shfl_xor(a_rz, o);
-    }
-    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = a_rr; sh[1][threadIdx.x >> 6] = a_rz; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s0 = 0, s1 = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) { s0 += sh[0][w]; s1 += sh[1][w]; }
-        partial[2 * blockIdx.x] = s0;
-        partial[2 * blockIdx.x + 1] = s1;
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            double t0 = 0, t1 = 0;
-            for (unsigned int b = 0; b < gridDim.x; b++) {
-                t0 += ((volatile double*)partial)[2 * b];
-                t1 += ((volatile double*)partial)[2 * b + 1];
-            }
-            scal[S_NORM2] = t0;
-            scal[S_RR] = t0;
-            scal[S_RZ] = t1;
-            istate[0] = (t0 <= scal[S_TOL2]) ? 1 : 0;
-            istate[2] = 0;   // first iteration uses p = z
-            *ticket = 0;
-        }
+    double v[2] = {a_rr, a_rz}, tot[2];
+    if (grid_reduce<2>(v, partial, ticket, tot) && threadIdx.x == 0) {
+        scal[S_NORM2] = tot[0];
+        scal[S_RR] = tot[0];
+        scal[S_RZ] = tot[1];
+        istate[0] = (tot[0] <= scal[S_TOL2]) ? 1 : 0;
+        istate[2] = 0;   // first iteration uses p = z
     }
 }
 
@@ -275,7 +278,6 @@ __global__ void __launch_bounds__(256) pcg_spmv(MeshP m, const uint8_t* __restri
                                                 unsigned int* ticket, double* __restrict__ scal, int* istate) {
     if (istate[0]) return;   // converged: the remaining queued iterations are no-ops
     __shared__ float ps[(PT_H + 2) * (PT_W + 2)];
-    __shared__ double sh[4];
     const int bx = blockIdx.x % ((m.ni + PT_W - 1) / PT_W), by = blockIdx.x / ((m.ni + PT_W - 1) / PT_W);
     const int i0 = bx * PT_W - 1, j0 = by * PT_H - 1;
     const bool first = istate[2] == 0;
@@ -311,20 +313,8 @@ __global__ void __launch_bounds__(256) pcg_spmv(MeshP m, const uint8_t* __restri
         q[u] = qv;
         acc = (double)pv * (double)qv;
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            double tot = 0;
-            for (unsigned int b = 0; b < gridDim.x; b++) tot += ((volatile double*)partial)[b];
-            scal[S_PQ] = tot;
-            *ticket = 0;
-        }
-    }
+    double v[1] = {acc}, tot[1];
+    if (grid_reduce<1>(v, partial, ticket, tot) && threadIdx.x == 0) scal[S_PQ] = tot[0];
 }
 
 // PCG step 2: alpha = rz / pq; d += alpha p; res -= alpha q; z = res / D; partials rz, rr.
@@ -335,7 +325,6 @@ __global__ void __launch_bounds__(256) pcg_update(int64_t nn, const float* __res
                                                   double* __restrict__ partial, unsigned int* ticket,
                                                   double* __restrict__ scal, int* istate, int max_it) {
     if (istate[0]) return;
-    __shared__ double sh[2][4];
     const float alpha = (float)(scal[S_RZ] / scal[S_PQ]);
     double a_rz = 0, a_rr = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
@@ -349,31 +338,14 @@ __global__ void __launch_bounds__(256) pcg_update(int64_t nn, const float* __res
         a_rz += (double)rv * (double)zv;
         a_rr += (double)rv * (double)rv;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        a_rz += __shfl_xor(a_rz, o);
-        a_rr += __shfl_xor(a_rr, o);
-    }
-    if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = a_rz; sh[1][threadIdx.x >> 6] = a_rr; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
-        partial[2 * blockIdx.x + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            double t0 = 0, t1 = 0;
-            for (unsigned int b = 0; b < gridDim.x; b++) {
-                t0 += ((volatile double*)partial)[2 * b];
-                t1 += ((volatile double*)partial)[2 * b + 1];
-            }
-            scal[S_RZ_OLD] = scal[S_RZ];
-            scal[S_RZ] = t0;
-            scal[S_RR] = t1;
-            istate[1] += 1;
-            istate[2] = 1;
-            if (t1 <= scal[S_TOL2] || istate[1] >= max_it || !(t1 == t1)) istate[0] = 1;
-            *ticket = 0;
-        }
+    double v[2] = {a_rz, a_rr}, tot[2];
+    if (grid_reduce<2>(v, partial, ticket, tot) && threadIdx.x == 0) {
+        scal[S_RZ_OLD] = scal[S_RZ];
+        scal[S_RZ] = tot[0];
+        scal[S_RR] = tot[1];
+        istate[1] += 1;
+        istate[2] = 1;
+        if (tot[1] <= scal[S_TOL2] || istate[1] >= max_it || !(tot[1] == tot[1])) istate[0] = 1;
     }
 }
 
@@ -385,26 +357,11 @@ __global__ void accumulate_y(int64_t nn, const float* __restrict__ d, double* __
 // max |y| over the grid (for the Newton step limit)
 __global__ void absmax_kernel(int64_t nn, const double* __restrict__ y, double* __restrict__ partial,
                               unsigned int* ticket, double* __restrict__ scal, int slot) {
-    __shared__ double sh[16];
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
         acc = fmax(acc, fabs(y[u]));
-    for (int o = 32; o > 0; o >>= 1) acc = fmax(acc, __shfl_xor(acc, o));
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s = fmax(s, sh[w]);
-        partial[blockIdx.x] = s;
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            double tot = 0;
-            for (unsigned int b = 0; b < gridDim.x; b++) tot = fmax(tot, ((volatile double*)partial)[b]);
-            scal[slot] = tot;
-            *ticket = 0;
-        }
-    }
+    double v[1] = {acc}, tot[1];
+    if (grid_reduce<1, true>(v, partial, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
 }
 
 // x += scale * y (Newton update, Solver.java:366-368 with the sign of the reduced system); y = 0
@@ -412,7 +369,6 @@ __global__ void absmax_kernel(int64_t nn, const double* __restrict__ y, double* 
 __global__ void newton_update(int64_t nn, const uint8_t* __restrict__ cls, double* __restrict__ x,
                               double* __restrict__ y, double lim, double* __restrict__ partial,
                               unsigned int* ticket, double* __restrict__ scal, int slot) {
-    __shared__ double sh[16];
     double acc = 0;
     double scale = 1.0;
     if (lim > 0) {
@@ -425,22 +381,8 @@ __global__ void newton_update(int64_t nn, const uint8_t* __restrict__ cls, doubl
         acc += v * v;
         y[u] = 0;
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w];
-        partial[blockIdx.x] = s;
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        if (t == gridDim.x - 1) {
-            double tot = 0;
-            for (unsigned int b = 0; b < gridDim.x; b++) tot += ((volatile double*)partial)[b];
-            scal[slot] = tot;
-            *ticket = 0;
-        }
-    }
+    double v[1] = {acc}, tot[1];
+    if (grid_reduce<1>(v, partial, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
 }
 
 // full potential from the free-node solution: fixed nodes take their value, face nodes the

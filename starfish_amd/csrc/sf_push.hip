@@ -803,7 +803,7 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
         hipEventElapsedTime(&e, ctx->ev[3], ctx->ev[4]);
         ctx->last_ms[0] = a;
         ctx->last_ms[1] = b;
-        ctx->last_ms[2] = c;
+        ctx->last_ms[2] = sort_step ? c : 0.0;   // no pass B on an in-place step
         ctx->last_ms[3] = e;
     }
     sp->nalive = st[0];

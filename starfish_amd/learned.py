@@ -86,6 +86,14 @@ class LearnedGuess:
         self._check(self.lib.sf_cnn_loss_and_grads(self.ptr, idx, C.byref(loss), _fp(y), _fp(g), None))
         return loss.value, y, g
 
+    def activation(self, which):
+        """Activation `which` (0..13, see sf_cnn_activation) of batch slot 0 after the last forward."""
+        n = C.c_int64()
+        self._check(self.lib.sf_cnn_activation(self.ptr, which, None, C.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        self._check(self.lib.sf_cnn_activation(self.ptr, which, _fp(out), None))
+        return out
+
     def predict(self, sim):
         """Writes the network's potential into sim's phi (the next solve's starting point)."""
         self._check(self.lib.sf_cnn_predict(self.ptr, sim.ctx.h))
@@ -101,3 +109,51 @@ def mfma_probe(ctx, a, b):
 
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def solve_from(sim, start, net=None):
+    """Solves sim's potential for its current rho from a chosen start and restores the previous
+    potential afterwards.  start: "zero" (phi = 0), "current" (the potential sim holds) or
+    "learned" (net's guess).  Returns (SolveInfo, status)."""
+    lib = _lib.load()
+    prev = sim.phi()
+    if start == "zero":
+        zero = np.zeros_like(prev)
+        sim.ctx.call("sf_phi_set", zero.ctypes.data_as(C.POINTER(C.c_double)))
+    info = _lib.SolveInfo()
+    sim.ctx.call("sf_rho_update")
+    if start == "learned":
+        used = C.c_int()
+        rc = _lib.check(sim.ctx.h, lib.sf_poisson_solve_learned(sim.ctx.h, net.ptr, 0, C.byref(info), C.byref(used)))
+    else:
+        rc = sim.ctx.call("sf_poisson_solve", None, C.byref(info))
+    sim.ctx.call("sf_phi_set", prev.ctypes.data_as(C.POINTER(C.c_double)))
+    return info, rc
+
+
+def generate_c2_snapshots(net, n_configs, steps, every, particles, ni, nj, seed=0, skip=0, log=None):
+    """Runs C2-style simulations (O+ beam past a cylinder of radius U(0.02, 0.1) m biased at
+    U(-200, 0) V) and adds a snapshot of (rho, boundary, phi) every `every` steps after `skip`
+    steps.  Returns the list of (radius, bias) used."""
+    from . import deck as D
+    from .plugin import Simulation
+    rng = np.random.default_rng(seed)
+    used = []
+    for k in range(n_configs):
+        radius, bias = float(rng.uniform(0.02, 0.1)), float(rng.uniform(-200.0, 0.0))
+        deck = D.config_c2(particles=particles, radius=radius, bias=bias, ni=ni, nj=nj)
+        sim = Simulation(deck, capacity=int(particles * 1.6) + 100_000)
+        try:
+            n, drift, temp = deck.preload
+            sim.materials[0].preload(n, drift, temp, seed=int(rng.integers(1 << 30)))
+            sim.initial_solve()
+            for s in range(steps):
+                sim.step()
+                if s >= skip and (s - skip) % every == 0:
+                    net.snapshot(sim)
+            if log:
+                log("config %d radius %.3f bias %.1f: %d samples" % (k, radius, bias, net.num_samples))
+        finally:
+            sim.close()
+        used.append((radius, bias))
+    return used

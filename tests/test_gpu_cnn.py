@@ -18,6 +18,9 @@ def ctx():
     c.close()
 
 
+ACT_NAMES = ["x0", "e1a", "u1", "p1", "e2a", "u2", "p2", "e3a", "e3", "d2a", "d2", "d1a", "d1", "y"]
+
+
 def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
@@ -66,7 +69,16 @@ def test_loss_output_and_gradients_match_oracle(ctx, ni, nj):
         sx4, st, sm = net.sample(0)
         assert np.array_equal(sx4, N.bf16(x4).astype(np.float32))
         loss, y, g = net.loss_and_grads(0)
-        lo, yo, go = N.loss_and_grads(flat.astype(np.float64), sx4.astype(np.float64), st.astype(np.float64), sm)
+        acts = {}
+        lo, yo, go = N.loss_and_grads(flat.astype(np.float64), sx4.astype(np.float64), st.astype(np.float64), sm,
+                                      acts_out=acts)
+        for k, name in enumerate(ACT_NAMES):
+            a = net.activation(k)
+            r = acts[name].reshape(-1)
+            d = np.abs(a - r).reshape(acts[name].shape)
+            rows = d.reshape(d.shape[0], -1).max(1)
+            print("%-4s rel %.2e  worst row %d of %d (%.2e)" % (name, _rel(a, r), rows.argmax(), len(rows),
+                                                              rows.max()), flush=True)
         print("loss", loss, lo, "y err", _rel(y, yo), flush=True)
         assert _rel(y, yo) < 2e-3
         assert abs(loss - lo) <= 5e-3 * lo

@@ -29,8 +29,13 @@ def test_c1_multistep_moments_match_oracle():
         n_gpu, n_orc = mat.getNp(), o.num_particles(0)
         print("particles gpu %d oracle %d" % (n_gpu, n_orc), flush=True)
         assert abs(n_gpu - n_orc) <= 0.02 * n_orc
-        # the shared generator and the injection are identical, so the totals track closely
+        # The shared generator and the injection are identical.  On this case the oracle itself
+        # moves mean u by ~1e-6 m/s when its state is rounded to FP32 every step and by ~0.05 m/s
+        # between its loose and a 1e-10 solver tolerance, so 1% of mean u (mean u is a small
+        # residual of forward and sheath-reflected ions) still flags a real difference.
         gp, op = mat.particles(), o.particles(0)
+        print("mean u %.6g vs %.6g, mean x %.6g vs %.6g" % (gp["u"].mean(), op["vel"][:, 0].mean(),
+                                                           gp["x"].mean(), op["pos"][:, 0].mean()), flush=True)
         assert abs(gp["u"].mean() - op["vel"][:, 0].mean()) < 0.01 * abs(op["vel"][:, 0].mean())
         assert abs(gp["x"].mean() - op["pos"][:, 0].mean()) < 0.01 * 0.8
         phi_g, phi_o = sim.phi(), o.field("phi")
@@ -44,7 +49,9 @@ def test_c1_multistep_moments_match_oracle():
 @pytest.mark.gpu
 def test_full_size_invariants_and_determinism():
     """1024x512 nodes, 2e7 particles: particle conservation (sort total == alive count of the
-    mover), deposited weight == particles * spwt, cell order, bit-identical density run to run."""
+    mover), deposited weight == particles * spwt, cell order after a sort step, bit-identical
+    density run to run.  The first push sorts the preloaded particles; with a sort interval of 2
+    the third step sorts again and the second pushes in place."""
     from starfish_amd.plugin import Simulation
     nds = []
     for rep in range(2):
@@ -52,6 +59,7 @@ def test_full_size_invariants_and_determinism():
         sim = Simulation(deck, capacity=30_000_000)
         try:
             mat = sim.materials[0]
+            sim.ctx.call("sf_species_set_sort_interval", mat.index, 2)
             npart, drift, temp = deck.preload
             mat.preload(npart, drift, temp, seed=1)
             assert mat.getNp() == npart
