@@ -117,23 +117,19 @@ def up_bwd(g):
     return g[0::2, 0::2] + g[0::2, 1::2] + g[1::2, 0::2] + g[1::2, 1::2]
 
 
-def loss_and_grads(flat, x4, target, mask, use_bf16=True, acts_out=None):
-    """Loss = sum over masked pixels of (y - t)^2 / (H W); returns (loss, y, flat gradient).
-
-    x4 [H][W][4] input channels, target [H][W], mask [H][W] (nonzero = counted).  If acts_out
-    is a dict it receives the stored activations under the device's names (x0, e1a, u1, p1,
-    e2a, u2, p2, e3a, e3, d2a, d2, d1a, d1, y)."""
+def forward(flat, x4, use_bf16=True):
+    """Forward pass.  Returns (y, acts) with acts[name] = (input, output) of every convolution
+    (outputs after ReLU and BF16 rounding) plus acts["d1"] = the head's input."""
     R = bf16 if use_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     p = unpack(np.asarray(flat))
     Wq = {name: R(p[name + ".W"]) for name, _, _ in LAYERS}
-    H, W = target.shape
+    H, W = x4.shape[:2]
     x0 = np.zeros((H, W, 32))
     x0[..., :4] = R(x4)
     acts = {}
 
     def conv(name, x):
-        z = conv_fwd(x, Wq[name], p[name + ".b"])
-        a = R(np.maximum(z, 0.0))
+        a = R(np.maximum(conv_fwd(x, Wq[name], p[name + ".b"]), 0.0))
         acts[name] = (x, a)
         return a
 
@@ -143,15 +139,16 @@ def loss_and_grads(flat, x4, target, mask, use_bf16=True, acts_out=None):
     d2 = conv("c4b", conv("c4a", np.concatenate([up(e3), e2], -1)))
     d1 = conv("c5b", conv("c5a", np.concatenate([up(d2), e1], -1)))
     y = x0[..., 3] + d1 @ p["head.w"] + p["head.b"][0]
-    if acts_out is not None:
-        acts_out.update(x0=x0, e1a=acts["c1a"][1], u1=np.concatenate([up(d2), e1], -1), p1=acts["c2a"][0],
-                        e2a=acts["c2a"][1], u2=np.concatenate([up(e3), e2], -1), p2=acts["c3a"][0],
-                        e3a=acts["c3a"][1], e3=e3, d2a=acts["c4a"][1], d2=d2, d1a=acts["c5a"][1], d1=d1, y=y)
-    n = H * W
-    e = np.where(mask != 0, y - target, 0.0)
-    loss = float((e * e).sum() / n)
-    dy = 2.0 * e / n
+    return y, acts
 
+
+def backward(flat, acts, dy, use_bf16=True):
+    """Parameter gradients (packed) for upstream dL/dy, given the forward activations `acts`
+    (as returned by forward or acts_from_device)."""
+    R = bf16 if use_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    p = unpack(np.asarray(flat))
+    Wq = {name: R(p[name + ".W"]) for name, _, _ in LAYERS}
+    d1 = acts["c5b"][1]
     g = {"head.w": (dy[..., None] * d1).sum((0, 1)), "head.b": np.array([dy.sum()])}
 
     def back(name, dout):
@@ -173,7 +170,51 @@ def loss_and_grads(flat, x4, target, mask, use_bf16=True, acts_out=None):
     g_p1 = back("c2a", back("c2b", g_e2))
     g_e1 = g_e1 + pool_bwd(g_p1)
     back("c1a", back("c1b", g_e1))
-    return loss, y, pack(g)
+    return pack(g)
+
+
+def loss_grad_y(y, target, mask):
+    """Masked MSE over all H*W pixels and its gradient dL/dy."""
+    n = y.size
+    e = np.where(mask != 0, y - target, 0.0)
+    return float((e * e).sum() / n), 2.0 * e / n
+
+
+def loss_and_grads(flat, x4, target, mask, use_bf16=True, acts_out=None):
+    """Loss = sum over masked pixels of (y - t)^2 / (H W); returns (loss, y, flat gradient).
+
+    x4 [H][W][4] input channels, target [H][W], mask [H][W] (nonzero = counted).  If acts_out
+    is a dict it receives the forward activations (see forward)."""
+    y, acts = forward(flat, x4, use_bf16)
+    if acts_out is not None:
+        acts_out.update(acts)
+    loss, dy = loss_grad_y(y, target, mask)
+    return loss, y, backward(flat, acts, dy, use_bf16)
+
+
+# device tensors (sf_cnn_activation order) -> acts of forward()
+DEVICE_TENSORS = ["x0", "e1a", "u1", "p1", "e2a", "u2", "p2", "e3a", "e3", "d2a", "d2", "d1a", "d1", "y"]
+
+
+def acts_from_device(t, H, W):
+    """t: dict name -> flat FP32 array read back from the device (names of DEVICE_TENSORS).
+    Returns acts[name] = (input, output) of every convolution, as forward() does."""
+    sh = {"x0": (H, W, 32), "e1a": (H, W, C1), "u1": (H, W, C2 + C1), "p1": (H // 2, W // 2, C1),
+          "e2a": (H // 2, W // 2, C2), "u2": (H // 2, W // 2, C3 + C2), "p2": (H // 4, W // 4, C2),
+          "e3a": (H // 4, W // 4, C3), "e3": (H // 4, W // 4, C3), "d2a": (H // 2, W // 2, C2),
+          "d2": (H // 2, W // 2, C2), "d1a": (H, W, C1), "d1": (H, W, C1)}
+    a = {k: np.asarray(t[k], np.float64).reshape(v) for k, v in sh.items()}
+    e1, e2 = a["u1"][..., C2:], a["u2"][..., C3:]
+    return {"c1a": (a["x0"], a["e1a"]), "c1b": (a["e1a"], e1), "c2a": (a["p1"], a["e2a"]), "c2b": (a["e2a"], e2),
+            "c3a": (a["p2"], a["e3a"]), "c3b": (a["e3a"], a["e3"]), "c4a": (a["u2"], a["d2a"]),
+            "c4b": (a["d2a"], a["d2"]), "c5a": (a["u1"], a["d1a"]), "c5b": (a["d1a"], a["d1"])}
+
+
+def conv_layer(flat, name, x, use_bf16=True):
+    """One convolution + ReLU (+ BF16 rounding) of layer `name` on input x."""
+    R = bf16 if use_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    p = unpack(np.asarray(flat))
+    return R(np.maximum(conv_fwd(x, R(p[name + ".W"]), p[name + ".b"]), 0.0))
 
 
 def init_params(seed=0):

@@ -45,78 +45,86 @@ struct PoissonState {
     float* D = nullptr;           // [nn] FP32 Jacobian diagonal
     double* P = nullptr;          // [nn] FP64 W * b'(x)
     float *d = nullptr, *res = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
-    double* partial = nullptr;    // [nblk * 4]
+    double* partial = nullptr;    // [nblk_elem] per-block partials of the Newton-level kernels
+    double* pz = nullptr;         // [2][nblk_elem][2] (r.z, r.r) partials of PCG, by iteration parity
+    double* ppq = nullptr;        // [nblk_tile] p.q partials of PCG
     double* scal = nullptr;       // device scalars
-    unsigned int* ticket = nullptr;
-    int* istate = nullptr;        // [0] done flag, [1] iterations
+    int* istate = nullptr;        // [0] done (1 converged, 2 not finite), [1] iterations at done
     int nblk_tile = 0, nblk_elem = 0;
     double* host_scal = nullptr;  // pinned
     int* host_istate = nullptr;
     hipEvent_t e0{}, e1{};
 };
 
-// device scalars layout
-enum { S_RZ = 0, S_RZ_OLD, S_PQ, S_RR, S_TOL2, S_NORM2, S_R0, S_YMAX, S_NSCAL = 8 };
+// device scalars layout (FP64): PCG tolerance, squared norms from the finalize kernel
+enum { S_TOL2 = 0, S_RZ0, S_NORM2, S_R0, S_YMAX, S_NSCAL = 8 };
 
-// Grid-wide reduction with a last-block ticket: every block reduces its NV values (sums, or
-// maxima of non-negative values when MAX) and stores them in partial[NV * block + k]; the last
-// block to arrive reduces all partials with all of its threads.  Returns true in every thread
-// of that block, with the totals in tot[]; the caller finalises there.  The summation order is
-// fixed for a given grid, so results are reproducible.
+// Reductions without grid-wide synchronisation.  A kernel leaves one partial per block
+// (block_partials); the next kernel on the stream combines them (combine_partials, every
+// block redundantly, in a fixed order) or a one-block finalize_partials kernel stores the total
+// in the scalar array.  Device-scope fences and ticket atomics are avoided on purpose: on a
+// multi-XCD part a release fence per block costs tens of microseconds per launch.
+template <bool MAX>
+__device__ __forceinline__ double red_op(double a, double b) { return MAX ? fmax(a, b) : a + b; }
+
 template <int NV, bool MAX = false>
-__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double* __restrict__ partial, unsigned int* ticket,
-                                            double (&tot)[NV]) {
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ partial) {
     __shared__ double red[NV][16];
-    __shared__ int last;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
 #pragma unroll
     for (int k = 0; k < NV; k++) {
-        for (int o = 32; o > 0; o >>= 1) v[k] = op(v[k], __shfl_xor(v[k], o));
+        for (int o = 32; o > 0; o >>= 1) v[k] = red_op<MAX>(v[k], __shfl_xor(v[k], o));
         if (lane == 0) red[k][wave] = v[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
         for (int k = 0; k < NV; k++) {
-            double s = red[k][0];
-            for (int w = 1; w < nw; w++) s = op(s, red[k][w]);
-            partial[NV * blockIdx.x + k] = s;
+            double t = red[k][0];
+            for (int w = 1; w < nw; w++) t = red_op<MAX>(t, red[k][w]);
+            partial[NV * blockIdx.x + k] = t;
         }
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+}
+
+// Totals of nb partial NV-vectors written by an earlier kernel; every thread receives them.
+template <int NV, bool MAX = false>
+__device__ __forceinline__ void combine_partials(const double* __restrict__ partial, int nb, double (&tot)[NV]) {
+    __shared__ double red[NV][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    double a[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) a[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; k++) a[k] = red_op<MAX>(a[k], partial[NV * b + k]);
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        for (int o = 32; o > 0; o >>= 1) a[k] = red_op<MAX>(a[k], __shfl_xor(a[k], o));
+        if (lane == 0) red[k][wave] = a[k];
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double b[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) b[k] = 0.0;
-    for (unsigned int blk = threadIdx.x; blk < gridDim.x; blk += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; k++) b[k] = op(b[k], ((volatile double*)partial)[NV * blk + k]);
-#pragma unroll
-    for (int k = 0; k < NV; k++)
-        for (int o = 32; o > 0; o >>= 1) b[k] = op(b[k], __shfl_xor(b[k], o));
-    __syncthreads();   // thread 0 has finished reading red[]
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < NV; k++) red[k][wave] = b[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         double t = red[k][0];
-        for (int w = 1; w < nw; w++) t = op(t, red[k][w]);
+        for (int w = 1; w < nw; w++) t = red_op<MAX>(t, red[k][w]);
         tot[k] = t;
     }
-    if (threadIdx.x == 0) *ticket = 0;
-    return true;
+    __syncthreads();   // red[] may be reused by the next call
+}
+
+template <int NV, bool MAX = false>
+__global__ void __launch_bounds__(256) finalize_partials(const double* __restrict__ partial, int nb,
+                                                         double* __restrict__ scal, int slot) {
+    double tot[NV];
+    combine_partials<NV, MAX>(partial, nb, tot);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; k++) scal[slot + k] = tot[k];
 }
 
 void poisson_free(PoissonState* p) {
     if (!p) return;
     void* ptrs[] = {p->cls, p->W,  p->bfix, p->src, p->fixval, p->x,  p->r,       p->y,    p->rr,
                     p->D,   p->P,  p->d,    p->res, p->z,      p->p0, p->p1,      p->q,    p->partial,
-                    p->scal, p->ticket, p->istate};
+                    p->pz,  p->ppq, p->scal, p->istate};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (p->host_scal) hipHostFree(p->host_scal);
@@ -184,8 +192,7 @@ __global__ void newton_residual(MeshP m, const uint8_t* __restrict__ cls, const 
                                 const double* __restrict__ bfix, const double* __restrict__ rho,
                                 const double* __restrict__ x, double* __restrict__ r, float* __restrict__ D,
                                 double* __restrict__ P, double inv_eps, double Cn0, double phi0, double inv_kTe,
-                                int linear, double* __restrict__ partial, unsigned int* ticket,
-                                double* __restrict__ scal, int slot) {
+                                int linear, double* __restrict__ partial) {
     const int64_t nn = (int64_t)m.ni * m.nj;
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
@@ -206,15 +213,15 @@ __global__ void newton_residual(MeshP m, const uint8_t* __restrict__ cls, const 
         D[u] = (float)(diagL + Wu * bp);
         acc += ru * ru;
     }
-    double v[1] = {acc}, tot[1];
-    if (grid_reduce<1>(v, partial, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
+    double v[1] = {acc};
+    block_partials<1>(v, partial);
 }
 
 // ||f + bfix - W b(0)||^2 for the fixed reference norm
 __global__ void ref_norm_kernel(MeshP m, const uint8_t* __restrict__ cls, const double* __restrict__ W,
                                 const double* __restrict__ bfix, const double* __restrict__ rho, double inv_eps,
-                                double Cn0, double phi0, double inv_kTe, int linear, double* __restrict__ partial,
-                                unsigned int* ticket, double* __restrict__ scal, int slot) {
+                                double Cn0, double phi0, double inv_kTe, int linear,
+                                double* __restrict__ partial) {
     const int64_t nn = (int64_t)m.ni * m.nj;
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
@@ -223,18 +230,18 @@ __global__ void ref_norm_kernel(MeshP m, const uint8_t* __restrict__ cls, const 
         double v = W[u] * rho[u] * inv_eps + bfix[u] - W[u] * b0;
         acc += v * v;
     }
-    double v[1] = {acc}, tot[1];
-    if (grid_reduce<1>(v, partial, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
+    double v[1] = {acc};
+    block_partials<1>(v, partial);
 }
 
-// refinement residual rr = r - (L + P) y (FP64) and ||rr||^2; also initialises the FP32 PCG:
-// d = 0, res = rr, z = rr / D, rz partial
+// refinement residual rr = r - (L + P) y (FP64); also initialises the FP32 PCG:
+// d = 0, res = rr, z = rr / D, and the (r.z, r.r) partials of iteration 0 (r.r from the FP64
+// residual: its total is the refinement norm)
 __global__ void refine_residual(MeshP m, const uint8_t* __restrict__ cls, const double* __restrict__ r,
                                 const double* __restrict__ y, const double* __restrict__ P,
                                 const float* __restrict__ D, double* __restrict__ rr, float* __restrict__ d,
                                 float* __restrict__ res, float* __restrict__ z, float* __restrict__ p0,
-                                double* __restrict__ partial, unsigned int* ticket, double* __restrict__ scal,
-                                int* istate) {
+                                double* __restrict__ pz0) {
     const int64_t nn = (int64_t)m.ni * m.nj;
     double a_rr = 0, a_rz = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
@@ -258,44 +265,61 @@ __global__ void refine_residual(MeshP m, const uint8_t* __restrict__ cls, const 
         a_rr += v * v;
         a_rz += (double)rf * (double)zf;
     }
-    double v[2] = {a_rr, a_rz}, tot[2];
-    if (grid_reduce<2>(v, partial, ticket, tot) && threadIdx.x == 0) {
-        scal[S_NORM2] = tot[0];
-        scal[S_RR] = tot[0];
-        scal[S_RZ] = tot[1];
-        istate[0] = (tot[0] <= scal[S_TOL2]) ? 1 : 0;
-        istate[2] = 0;   // first iteration uses p = z
-    }
+    double v[2] = {a_rz, a_rr};
+    block_partials<2>(v, pz0);   // (r.z, r.r) of iteration 0, read by the first PCG step
 }
 
-// PCG step 1: p_new = z + beta p_old (beta = rz/rz_old; p = z on the first iteration);
-// q = J p_new; partial p.q
-constexpr int PT_W = 64, PT_H = 4;   // interior nodes per block (256 threads, one node each)
+// Jacobi-PCG in FP32, two kernels per iteration k, no grid-wide synchronisation: each kernel
+// combines the partial sums the previous one left.  (r.z, r.r) partials alternate between
+// pz[k & 1] (read at iteration k) and pz[(k + 1) & 1] (written by its update), so r.z of
+// iteration k - 1 is still available for beta.  Convergence (r.r <= tol^2) is detected by
+// every block of the step-1 kernel from the same sums; block 0 records it in istate, which
+// turns every later queued kernel into a no-op.
+constexpr int PT_W = 64, PT_H = 16;   // nodes per step-1 block (256 threads, 4 rows each)
+
+// step 1: beta = rz_k / rz_{k-1} (0 at k = 0); p = z + beta p_old; q = J p; p.q partials
 __global__ void __launch_bounds__(256) pcg_spmv(MeshP m, const uint8_t* __restrict__ cls,
                                                 const float* __restrict__ D, const float* __restrict__ z,
                                                 const float* __restrict__ pold, float* __restrict__ pnew,
-                                                float* __restrict__ q, double* __restrict__ partial,
-                                                unsigned int* ticket, double* __restrict__ scal, int* istate) {
-    if (istate[0]) return;   // converged: the remaining queued iterations are no-ops
+                                                float* __restrict__ q, const double* __restrict__ pz_cur,
+                                                const double* __restrict__ pz_prev, int nbz, int k,
+                                                double* __restrict__ ppq, const double* __restrict__ scal,
+                                                int* istate) {
+    if (istate[0]) return;
+    double cur[2];
+    combine_partials<2>(pz_cur, nbz, cur);   // (rz_k, rr_k)
+    if (!(cur[1] > scal[S_TOL2])) {          // converged, or not finite
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            istate[0] = (cur[1] == cur[1]) ? 1 : 2;
+            istate[1] = k;
+        }
+        return;
+    }
+    float beta = 0.0f;
+    if (k > 0) {
+        double prev[2];
+        combine_partials<2>(pz_prev, nbz, prev);
+        beta = (float)(cur[0] / prev[0]);
+    }
     __shared__ float ps[(PT_H + 2) * (PT_W + 2)];
-    const int bx = blockIdx.x % ((m.ni + PT_W - 1) / PT_W), by = blockIdx.x / ((m.ni + PT_W - 1) / PT_W);
+    const int nbx = (m.ni + PT_W - 1) / PT_W;
+    const int bx = blockIdx.x % nbx, by = blockIdx.x / nbx;
     const int i0 = bx * PT_W - 1, j0 = by * PT_H - 1;
-    const bool first = istate[2] == 0;
-    const float beta = first ? 0.0f : (float)(scal[S_RZ] / scal[S_RZ_OLD]);
-    for (int k = threadIdx.x; k < (PT_H + 2) * (PT_W + 2); k += blockDim.x) {
-        int i = i0 + k % (PT_W + 2), j = j0 + k / (PT_W + 2);
+    for (int t = threadIdx.x; t < (PT_H + 2) * (PT_W + 2); t += blockDim.x) {
+        int i = i0 + t % (PT_W + 2), j = j0 + t / (PT_W + 2);
         float v = 0.0f;
         if (i >= 0 && j >= 0 && i < m.ni && j < m.nj) {
             size_t u = (size_t)j * m.ni + i;
-            v = first ? z[u] : z[u] + beta * pold[u];
+            v = k == 0 ? z[u] : z[u] + beta * pold[u];
         }
-        ps[k] = v;
+        ps[t] = v;
     }
     __syncthreads();
-    const int li = threadIdx.x % PT_W, lj = threadIdx.x / PT_W;
-    const int i = i0 + 1 + li, j = j0 + 1 + lj;
+    const int li = threadIdx.x % PT_W;
     double acc = 0;
-    if (i < m.ni && j < m.nj) {
+    for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += 256 / PT_W) {
+        const int i = i0 + 1 + li, j = j0 + 1 + lj;
+        if (i >= m.ni || j >= m.nj) continue;
         size_t u = (size_t)j * m.ni + i;
         uint8_t c = cls[u];
         int l = (lj + 1) * (PT_W + 2) + (li + 1);
@@ -311,21 +335,25 @@ __global__ void __launch_bounds__(256) pcg_spmv(MeshP m, const uint8_t* __restri
         }
         pnew[u] = pv;
         q[u] = qv;
-        acc = (double)pv * (double)qv;
+        acc += (double)pv * (double)qv;
     }
-    double v[1] = {acc}, tot[1];
-    if (grid_reduce<1>(v, partial, ticket, tot) && threadIdx.x == 0) scal[S_PQ] = tot[0];
+    double v[1] = {acc};
+    block_partials<1>(v, ppq);
 }
 
-// PCG step 2: alpha = rz / pq; d += alpha p; res -= alpha q; z = res / D; partials rz, rr.
-// The last block sets rz_old, rz, the done flag and the iteration count.
+// step 2: alpha = rz_k / p.q; d += alpha p; res -= alpha q; z = res / D; (r.z, r.r) partials
+// of iteration k + 1 into pz_next.
 __global__ void __launch_bounds__(256) pcg_update(int64_t nn, const float* __restrict__ D, const float* __restrict__ p,
                                                   const float* __restrict__ q, float* __restrict__ d,
                                                   float* __restrict__ res, float* __restrict__ z,
-                                                  double* __restrict__ partial, unsigned int* ticket,
-                                                  double* __restrict__ scal, int* istate, int max_it) {
+                                                  const double* __restrict__ ppq, int nbq,
+                                                  const double* __restrict__ pz_cur, int nbz,
+                                                  double* __restrict__ pz_next, const int* istate) {
     if (istate[0]) return;
-    const float alpha = (float)(scal[S_RZ] / scal[S_PQ]);
+    double pq[1], cur[2];
+    combine_partials<1>(ppq, nbq, pq);
+    combine_partials<2>(pz_cur, nbz, cur);
+    const float alpha = (float)(cur[0] / pq[0]);
     double a_rz = 0, a_rr = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
         float pv = p[u], qv = q[u];
@@ -338,15 +366,8 @@ __global__ void __launch_bounds__(256) pcg_update(int64_t nn, const float* __res
         a_rz += (double)rv * (double)zv;
         a_rr += (double)rv * (double)rv;
     }
-    double v[2] = {a_rz, a_rr}, tot[2];
-    if (grid_reduce<2>(v, partial, ticket, tot) && threadIdx.x == 0) {
-        scal[S_RZ_OLD] = scal[S_RZ];
-        scal[S_RZ] = tot[0];
-        scal[S_RR] = tot[1];
-        istate[1] += 1;
-        istate[2] = 1;
-        if (tot[1] <= scal[S_TOL2] || istate[1] >= max_it || !(tot[1] == tot[1])) istate[0] = 1;
-    }
+    double v[2] = {a_rz, a_rr};
+    block_partials<2>(v, pz_next);
 }
 
 // y += d (FP64 accumulation of the FP32 correction)
@@ -355,34 +376,27 @@ __global__ void accumulate_y(int64_t nn, const float* __restrict__ d, double* __
         y[u] += (double)d[u];
 }
 // max |y| over the grid (for the Newton step limit)
-__global__ void absmax_kernel(int64_t nn, const double* __restrict__ y, double* __restrict__ partial,
-                              unsigned int* ticket, double* __restrict__ scal, int slot) {
+__global__ void absmax_kernel(int64_t nn, const double* __restrict__ y, double* __restrict__ partial) {
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
         acc = fmax(acc, fabs(y[u]));
-    double v[1] = {acc}, tot[1];
-    if (grid_reduce<1, true>(v, partial, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
+    double v[1] = {acc};
+    block_partials<1, true>(v, partial);
 }
 
 // x += scale * y (Newton update, Solver.java:366-368 with the sign of the reduced system); y = 0
 // The step limit is evaluated on the device: scale = min(1, lim / max|y|) with lim <= 0 off.
 __global__ void newton_update(int64_t nn, const uint8_t* __restrict__ cls, double* __restrict__ x,
-                              double* __restrict__ y, double lim, double* __restrict__ partial,
-                              unsigned int* ticket, double* __restrict__ scal, int slot) {
-    double acc = 0;
+                              double* __restrict__ y, double lim, const double* __restrict__ scal) {
     double scale = 1.0;
     if (lim > 0) {
         double ymax = scal[S_YMAX];
         if (ymax > lim) scale = lim / ymax;
     }
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
-        double v = scale * y[u];
-        if (cls[u] != CLS_NOT_FREE) x[u] += v;
-        acc += v * v;
+        if (cls[u] != CLS_NOT_FREE) x[u] += scale * y[u];
         y[u] = 0;
     }
-    double v[1] = {acc}, tot[1];
-    if (grid_reduce<1>(v, partial, ticket, tot) && threadIdx.x == 0) scal[slot] = tot[0];
 }
 
 // full potential from the free-node solution: fixed nodes take their value, face nodes the
@@ -450,7 +464,7 @@ __global__ void apply_reference_operator(MeshP m, const uint8_t* __restrict__ ro
 // ----------------------------------------------------------------------------------------
 static int grid_elem(int64_t n) {
     int64_t b = (n + 255) / 256;
-    if (b > 1024) b = 1024;
+    if (b > 512) b = 512;
     if (b < 1) b = 1;
     return (int)b;
 }
@@ -566,16 +580,18 @@ int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* dsc) {
         if (!e) e = A((void**)&P->p1, nn * 4);
         if (!e) e = A((void**)&P->q, nn * 4);
         if (!e) e = A((void**)&P->partial, npart * 8);
+        if (!e) e = A((void**)&P->pz, (size_t)4 * P->nblk_elem * 8);
+        if (!e) e = A((void**)&P->ppq, (size_t)P->nblk_tile * 8);
         if (!e) e = A((void**)&P->scal, S_NSCAL * 8 * 2);
-        if (!e) e = A((void**)&P->ticket, 16);
         if (!e) e = A((void**)&P->istate, 16);
         if (!e) e = hipHostMalloc((void**)&P->host_scal, S_NSCAL * 8 * 2);
         if (!e) e = hipHostMalloc((void**)&P->host_istate, 16);
         if (!e) e = hipEventCreate(&P->e0);
         if (!e) e = hipEventCreate(&P->e1);
         if (e) return sf_fail(ctx, SF_ERR_OOM, "poisson allocation failed: %s", hipGetErrorString(e));
-        hipMemset(P->ticket, 0, 16);
         hipMemset(P->istate, 0, 16);
+        hipMemset(P->pz, 0, (size_t)4 * P->nblk_elem * 8);
+        hipMemset(P->ppq, 0, (size_t)P->nblk_tile * 8);
         hipMemset(P->y, 0, nn * 8);
         hipMemset(P->x, 0, nn * 8);
         hipMemset(P->d, 0, nn * 4);
@@ -602,6 +618,19 @@ int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* dsc) {
     return SF_OK;
 }
 
+// total of the per-block partials left by the last kernel, into scal[slot..slot+NV)
+template <int NV, bool MAX = false>
+static void finalize(PoissonState* P, hipStream_t st, const double* partial, int nb, int slot) {
+    hipLaunchKernelGGL((finalize_partials<NV, MAX>), dim3(1), dim3(256), 0, st, partial, nb, P->scal, slot);
+}
+
+static int read_scalars(sf_ctx* ctx, PoissonState* P) {
+    SF_HIP(ctx, hipGetLastError());
+    SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, ctx->stream));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SF_OK;
+}
+
 int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     PoissonState* P = ctx->poisson;
     if (!P) return sf_fail(ctx, SF_ERR_STATE, "poisson solver not set up");
@@ -609,22 +638,24 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     const int64_t nn = (int64_t)m.ni * m.nj;
     const sf_poisson_desc& D = P->desc;
     hipStream_t st = ctx->stream;
-    const int ge = P->nblk_elem;
+    const int ge = P->nblk_elem, gt = P->nblk_tile;
     const double inv_eps = 1.0 / P->eps;
     const double Cn0 = D.linear ? 0.0 : P->C * D.n0;
     const double inv_kTe = D.linear ? 0.0 : 1.0 / D.Te0;
     const double rel_tol = D.rel_tol > 0 ? D.rel_tol : 1e-8;
     const int max_it = D.max_it > 0 ? D.max_it : 5000;
     const int nl_max = D.linear ? 1 : (D.nl_max_it > 0 ? D.nl_max_it : 50);
+    double* pz0 = P->pz;
+    double* pz1 = P->pz + 2 * ge;
     SF_HIP(ctx, hipEventRecord(P->e0, st));
     hipLaunchKernelGGL(gather_free, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, (const double*)ctx->d_phi,
                        P->x);
     hipLaunchKernelGGL(ref_norm_kernel, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
                        (const double*)P->bfix, (const double*)ctx->d_rho, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
-                       P->partial, P->ticket, P->scal, (int)S_R0);
-    SF_HIP(ctx, hipGetLastError());
-    SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
-    SF_HIP(ctx, hipStreamSynchronize(st));
+                       P->partial);
+    finalize<1>(P, st, P->partial, ge, S_R0);
+    int rc = read_scalars(ctx, P);
+    if (rc) return rc;
     double ref2 = P->host_scal[S_R0];
     double ref = std::sqrt(ref2);
     if (!(ref > 0)) ref = 1.0;
@@ -637,10 +668,10 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                            (const double*)P->W, (const double*)P->bfix, (const double*)ctx->d_rho,
                            (const double*)P->x, P->r, P->D, P->P, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
-                           P->partial, P->ticket, P->scal, (int)S_NORM2);
-        SF_HIP(ctx, hipGetLastError());
-        SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
-        SF_HIP(ctx, hipStreamSynchronize(st));
+                           P->partial);
+        finalize<1>(P, st, P->partial, ge, S_NORM2);
+        rc = read_scalars(ctx, P);
+        if (rc) return rc;
         rnorm = std::sqrt(P->host_scal[S_NORM2]);
         rel = rnorm / ref;
         if (!std::isfinite(rel)) return sf_fail(ctx, SF_ERR_STATE, "Newton residual is not finite (diverged)");
@@ -649,41 +680,45 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         newton++;
         // inexact Newton: solve J y = r to max(eta*||r||, 0.5*tol*ref) with FP32 PCG + FP64 refinement
         double lin_target = std::max(D.linear ? 0.5 * rel_tol * ref : eta * rnorm, 0.5 * rel_tol * ref);
-        // inner FP32 tolerance: a relative reduction FP32 can deliver
         int refine = 0;
         while (true) {
             hipLaunchKernelGGL(refine_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                                (const double*)P->r, (const double*)P->y, (const double*)P->P, (const float*)P->D,
-                               P->rr, P->d, P->res, P->z, P->p0, P->partial, P->ticket, P->scal, P->istate);
-            SF_HIP(ctx, hipGetLastError());
-            SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
-            SF_HIP(ctx, hipStreamSynchronize(st));
+                               P->rr, P->d, P->res, P->z, P->p0, pz0);
+            finalize<2>(P, st, pz0, ge, S_RZ0);   // (r.z, r.r) -> S_RZ0, S_NORM2
+            rc = read_scalars(ctx, P);
+            if (rc) return rc;
             double rr = std::sqrt(P->host_scal[S_NORM2]);
             if (rr <= lin_target || refine >= 8 || pcg_total >= max_it * nl_max) break;
+            // inner FP32 tolerance: a relative reduction FP32 can deliver
             double inner_tol = std::max(lin_target, 1e-5 * rr);
             double tol2 = inner_tol * inner_tol;
             SF_HIP(ctx, hipMemcpyAsync(P->scal + S_TOL2, &tol2, 8, hipMemcpyHostToDevice, st));
-            int zero[3] = {0, 0, 0};
-            SF_HIP(ctx, hipMemcpyAsync(P->istate, zero, 12, hipMemcpyHostToDevice, st));
-            int done = 0, its = 0;
+            int zero[2] = {0, 0};
+            SF_HIP(ctx, hipMemcpyAsync(P->istate, zero, 8, hipMemcpyHostToDevice, st));
             float* pa = P->p0;
             float* pb = P->p1;
-            int batch = 8;    // iterations queued between host checks; grows while unconverged
-            while (!done) {
-                for (int b = 0; b < batch; b++) {
-                    hipLaunchKernelGGL(pcg_spmv, dim3(P->nblk_tile), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+            int k = 0, its = 0, batch = 8;   // iterations queued between host checks; grows while unconverged
+            while (true) {
+                const int nb = std::min(batch, max_it - k);
+                for (int b = 0; b < nb; b++, k++) {
+                    double* cur = (k & 1) ? pz1 : pz0;
+                    double* nxt = (k & 1) ? pz0 : pz1;
+                    hipLaunchKernelGGL(pcg_spmv, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                                        (const float*)P->D, (const float*)P->z, (const float*)pa, pb, P->q,
-                                       P->partial, P->ticket, P->scal, P->istate);
+                                       (const double*)cur, (const double*)nxt, ge, k, P->ppq,
+                                       (const double*)P->scal, P->istate);
                     hipLaunchKernelGGL(pcg_update, dim3(ge), dim3(256), 0, st, nn, (const float*)P->D,
-                                       (const float*)pb, (const float*)P->q, P->d, P->res, P->z, P->partial,
-                                       P->ticket, P->scal, P->istate, max_it);
+                                       (const float*)pb, (const float*)P->q, P->d, P->res, P->z,
+                                       (const double*)P->ppq, gt, (const double*)cur, ge, nxt,
+                                       (const int*)P->istate);
                     std::swap(pa, pb);
                 }
                 SF_HIP(ctx, hipGetLastError());
-                SF_HIP(ctx, hipMemcpyAsync(P->host_istate, P->istate, 12, hipMemcpyDeviceToHost, st));
+                SF_HIP(ctx, hipMemcpyAsync(P->host_istate, P->istate, 8, hipMemcpyDeviceToHost, st));
                 SF_HIP(ctx, hipStreamSynchronize(st));
-                done = P->host_istate[0];
-                its = P->host_istate[1];
+                if (P->host_istate[0]) { its = P->host_istate[1]; break; }
+                if (k >= max_it) { its = k; break; }
                 batch = std::min(2 * batch, 64);
             }
             pcg_total += its;
@@ -696,22 +731,18 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         double lim = 0.0;
         if (!D.linear) {
             lim = 10.0 * D.Te0;
-            hipLaunchKernelGGL(absmax_kernel, dim3(ge), dim3(256), 0, st, nn, (const double*)P->y, P->partial,
-                               P->ticket, P->scal, (int)S_YMAX);
+            hipLaunchKernelGGL(absmax_kernel, dim3(ge), dim3(256), 0, st, nn, (const double*)P->y, P->partial);
+            finalize<1, true>(P, st, P->partial, ge, S_YMAX);
         }
         hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y, lim,
-                           P->partial, P->ticket, P->scal, (int)S_RR);
+                           (const double*)P->scal);
         if (debug) {
-            SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
-            SF_HIP(ctx, hipStreamSynchronize(st));
+            rc = read_scalars(ctx, P);
+            if (rc) return rc;
             fprintf(stderr, "[newton %d] rel %.3e lin_target/ref %.3e refine %d pcg %d ymax %.3e\n", it, rel,
-                    lin_target / ref, refine, pcg_total, P->host_scal[S_YMAX]);
+                    lin_target / ref, refine, pcg_total, D.linear ? 0.0 : P->host_scal[S_YMAX]);
         }
         SF_HIP(ctx, hipGetLastError());
-        if (D.linear) {
-            // one linear solve; evaluate the final residual on the next pass of the loop
-            continue;
-        }
     }
     hipLaunchKernelGGL(expand_phi, dim3(ge), dim3(256), 0, st, nn, (const int32_t*)P->src, (const double*)P->fixval,
                        (const double*)P->x, ctx->d_phi);
@@ -919,13 +950,14 @@ int poisson_rel_residual(sf_ctx* ctx, double* rel) {
                        P->x);
     hipLaunchKernelGGL(ref_norm_kernel, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
                        (const double*)P->bfix, (const double*)ctx->d_rho, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
-                       P->partial, P->ticket, P->scal, (int)S_R0);
+                       P->partial);
+    finalize<1>(P, st, P->partial, ge, S_R0);
     hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
                        (const double*)P->bfix, (const double*)ctx->d_rho, (const double*)P->x, P->r, P->D, P->P,
-                       inv_eps, Cn0, D.phi0, inv_kTe, D.linear, P->partial, P->ticket, P->scal, (int)S_NORM2);
-    SF_HIP(ctx, hipGetLastError());
-    SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
-    SF_HIP(ctx, hipStreamSynchronize(st));
+                       inv_eps, Cn0, D.phi0, inv_kTe, D.linear, P->partial);
+    finalize<1>(P, st, P->partial, ge, S_NORM2);
+    int rc = read_scalars(ctx, P);
+    if (rc) return rc;
     double ref = std::sqrt(P->host_scal[S_R0]);
     if (!(ref > 0)) ref = 1.0;
     *rel = std::sqrt(P->host_scal[S_NORM2]) / ref;

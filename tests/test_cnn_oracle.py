@@ -78,3 +78,23 @@ def test_parameter_count_and_packing_round_trip():
     flat = N.init_params(seed=3)
     assert len(flat) == N.num_params() == 479905
     assert np.array_equal(N.pack(N.unpack(flat)), flat)
+
+
+def test_device_activation_layout_round_trip():
+    """acts_from_device reads the device's concatenated tensors (u1 = up(d2) ++ e1,
+    u2 = up(e3) ++ e2) back into per-layer (input, output) pairs: the backward from them equals
+    the backward from the forward pass itself."""
+    rng = np.random.default_rng(4)
+    H, W = 8, 12
+    flat = N.init_params(seed=5)
+    x4 = rng.normal(size=(H, W, 4))
+    y, acts = N.forward(flat, x4)
+    A = {k: v for k, v in acts.items()}
+    dev = {"x0": A["c1a"][0], "e1a": A["c1a"][1], "u1": A["c5a"][0], "p1": A["c2a"][0], "e2a": A["c2a"][1],
+           "u2": A["c4a"][0], "p2": A["c3a"][0], "e3a": A["c3a"][1], "e3": A["c3b"][1], "d2a": A["c4a"][1],
+           "d2": A["c4b"][1], "d1a": A["c5a"][1], "d1": A["c5b"][1]}
+    back = N.acts_from_device({k: v.ravel() for k, v in dev.items()}, H, W)
+    for name in acts:
+        assert np.array_equal(back[name][0], acts[name][0]) and np.array_equal(back[name][1], acts[name][1])
+    dy = rng.normal(size=(H, W))
+    assert np.array_equal(N.backward(flat, back, dy), N.backward(flat, acts, dy))

@@ -15,6 +15,11 @@ def test_c1_multistep_moments_match_oracle():
     from starfish_amd.plugin import Simulation
     steps = 20
     deck = D.config_c1(steps=steps)
+    # At the first injection the reference's undamped Newton overshoots exp(phi/kTe) and then
+    # descends about kTe per iteration, so its default cap of 50 iterations stops it far from
+    # the solution (residual ~1e13 at the beam nodes); 400 lets it converge.  The GPU solve is
+    # damped and reaches the 1e-8 residual either way.
+    deck.solver.nl_max_it = 400
     geo, st = geometry(deck)
     o = oracle_for(deck, geo, st)
     s = deck.sources[0]
@@ -28,16 +33,16 @@ def test_c1_multistep_moments_match_oracle():
         mat = sim.materials[0]
         n_gpu, n_orc = mat.getNp(), o.num_particles(0)
         print("particles gpu %d oracle %d" % (n_gpu, n_orc), flush=True)
-        assert abs(n_gpu - n_orc) <= 0.02 * n_orc
+        assert abs(n_gpu - n_orc) <= 0.002 * n_orc
         # The shared generator and the injection are identical.  On this case the oracle itself
         # moves mean u by ~1e-6 m/s when its state is rounded to FP32 every step and by ~0.05 m/s
-        # between its loose and a 1e-10 solver tolerance, so 1% of mean u (mean u is a small
-        # residual of forward and sheath-reflected ions) still flags a real difference.
+        # between two converged solver tolerances, so 0.1% of mean u (a small residual of forward
+        # and sheath-reflected ions, ~350 m/s) flags a real difference.
         gp, op = mat.particles(), o.particles(0)
         print("mean u %.6g vs %.6g, mean x %.6g vs %.6g" % (gp["u"].mean(), op["vel"][:, 0].mean(),
                                                            gp["x"].mean(), op["pos"][:, 0].mean()), flush=True)
-        assert abs(gp["u"].mean() - op["vel"][:, 0].mean()) < 0.01 * abs(op["vel"][:, 0].mean())
-        assert abs(gp["x"].mean() - op["pos"][:, 0].mean()) < 0.01 * 0.8
+        assert abs(gp["u"].mean() - op["vel"][:, 0].mean()) < 1e-3 * abs(op["vel"][:, 0].mean())
+        assert abs(gp["x"].mean() - op["pos"][:, 0].mean()) < 1e-4 * 0.8
         phi_g, phi_o = sim.phi(), o.field("phi")
         assert np.abs(phi_g - phi_o).max() < 0.02 * 100.0
         nd_g, nd_o = mat.field("nd-ave"), o.field("count-sum", 0) / steps / geo.node_vol
@@ -50,8 +55,8 @@ def test_c1_multistep_moments_match_oracle():
 def test_full_size_invariants_and_determinism():
     """1024x512 nodes, 2e7 particles: particle conservation (sort total == alive count of the
     mover), deposited weight == particles * spwt, cell order after a sort step, bit-identical
-    density run to run.  The first push sorts the preloaded particles; with a sort interval of 2
-    the third step sorts again and the second pushes in place."""
+    density run to run.  The preload leaves the particles cell-sorted; with a sort interval of 3
+    the first two steps push in place and the third sorts."""
     from starfish_amd.plugin import Simulation
     nds = []
     for rep in range(2):
@@ -59,7 +64,7 @@ def test_full_size_invariants_and_determinism():
         sim = Simulation(deck, capacity=30_000_000)
         try:
             mat = sim.materials[0]
-            sim.ctx.call("sf_species_set_sort_interval", mat.index, 2)
+            sim.ctx.call("sf_species_set_sort_interval", mat.index, 3)
             npart, drift, temp = deck.preload
             mat.preload(npart, drift, temp, seed=1)
             assert mat.getNp() == npart
