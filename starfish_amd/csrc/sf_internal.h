@@ -31,7 +31,7 @@ constexpr int TILE_CH = TILE_H + 2 * TILE_HALO;
 constexpr int TILE_CELLS = TILE_CW * TILE_CH;
 constexpr int NCH = 7;                          // deposit channels: n, nu, nv, nw, nuu, nvv, nww
 constexpr int PUSH_THREADS = 256;
-constexpr int CHUNK = 512;                      // particles staged in LDS per round
+constexpr int CHUNK = 256;                      // particles staged in LDS per round
 constexpr int CHUNK_STRIDE = 37;                // odd stride of the lane->particle permutation
 
 // Particle state, structure of arrays, one buffer.

@@ -183,9 +183,12 @@ __device__ __forceinline__ bool process_boundary(const MeshP& m, const MeshD& md
 
 // One particle through ParticleMover.run (KineticMaterial.java:299-422), B = 0 branch.
 // Inputs: logical coordinate and velocity; outputs updated in place.  Returns alive.
-template <class Load>
+// HasSeg: (i, j) -> nonzero if node (i, j) lists a DIRICHLET/SINK segment (md.node_hasseg, or a
+// staged copy of it).
+template <class Load, class HasSeg>
 __device__ __forceinline__ bool push_particle(const MeshP& m, const MeshD& md, const PushC& pc, Load load,
-                                              double& lc0, double& lc1, double& v0, double& v1, double& v2) {
+                                              HasSeg hasseg, double& lc0, double& lc1, double& v0, double& v1,
+                                              double& v2) {
     double ex, ey;
     gather_e(m, lc0, lc1, load, ex, ey);
     v0 += pc.qm_dt * ex;
@@ -221,8 +224,7 @@ __device__ __forceinline__ bool push_particle(const MeshP& m, const MeshD& md, c
             int imin = (int)fmin(n0, lc0), imax = (int)fmax(n0, lc0);
             int jmin = (int)fmin(n1, lc1), jmax = (int)fmax(n1, lc1);
             if (imin >= 0 && jmin >= 0 && imax - imin <= 1 && jmax - jmin <= 1) {
-                const uint8_t* h = md.node_hasseg + (size_t)jmin * m.ni + imin;
-                int seg = h[0] | h[imax - imin] | h[(jmax - jmin) * m.ni] | h[(jmax - jmin) * m.ni + (imax - imin)];
+                int seg = hasseg(imin, jmin) | hasseg(imax, jmin) | hasseg(imin, jmax) | hasseg(imax, jmax);
                 if (!seg) {
                     lc0 = n0;
                     lc1 = n1;

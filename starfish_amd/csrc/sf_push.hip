@@ -53,8 +53,10 @@ __device__ __forceinline__ void deposit_global(const MeshP& m, unsigned long lon
     double w[4];
     scatter_weights(m, i, j, di, dj, w);
     const int ci[4] = {0, 1, 1, 0}, cj[4] = {0, 0, 1, 1};
+#pragma unroll
     for (int c = 0; c < 4; c++) {
         size_t u = (size_t)(j + cj[c]) * m.ni + (i + ci[c]);
+#pragma unroll
         for (int ch = 0; ch < NCH; ch++) {
             long long q = quant(w[c], val[ch]);
             if (q != 0) atomicAdd(acc + ch * nn + u, (unsigned long long)q);
@@ -129,16 +131,25 @@ struct TileLds {
     unsigned long long acc[NCH][TILE_NODES];
     int hist[TILE_CELLS];
     float2 ef[TILE_NODES];
+    uint8_t hs[TILE_NODES];
     int32_t s_cell[CHUNK];
     float s_fx[CHUNK], s_fy[CHUNK], s_vx[CHUNK], s_vy[CHUNK], s_vz[CHUNK];
     int seg_start[TILE_H], seg_pre[TILE_H + 1];
+};
+
+// One chunk of staged particle state held in registers (CHUNK / PUSH_THREADS per thread), so that
+// the next chunk's loads are in flight while the current chunk is pushed.
+constexpr int PER_T = CHUNK / PUSH_THREADS;
+struct ChunkRegs {
+    int32_t c[PER_T];
+    float fx[PER_T], fy[PER_T], vx[PER_T], vy[PER_T], vz[PER_T];
 };
 
 // INPLACE = false: sort step, the new cell of every particle goes to keys[] (pass B moves it).
 // INPLACE = true: the pushed state is written back to the particle's own slot; dead particles
 // become holes (cell = -1) until the next sort.
 template <bool INPLACE>
-__global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD md, PushC pc, PState s,
+__global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile(MeshP m, MeshD md, PushC pc, PState s,
                                                                    const int32_t* __restrict__ offsets,
                                                                    int32_t* __restrict__ keys,
                                                                    unsigned long long* __restrict__ acc,
@@ -158,6 +169,7 @@ __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD
         i = min(max(i, 0), m.ni - 1);
         j = min(max(j, 0), m.nj - 1);
         L.ef[k] = md.ef[(size_t)j * m.ni + i];
+        L.hs[k] = md.node_hasseg[(size_t)j * m.ni + i];
     }
     if (t == 0) {
         int pre = 0;
@@ -179,33 +191,61 @@ __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD
     __syncthreads();
     const int ntile = L.seg_pre[TILE_H];
 
+    // clamped loads reproduce the global array for in-mesh nodes (the window is clamped above)
     auto load_e = [&](int i, int j) -> float2 {
         int li = i - n_i0, lj = j - n_j0;
         if (li >= 0 && li < TILE_NW && lj >= 0 && lj < TILE_NH) return L.ef[lj * TILE_NW + li];
         return md.ef[(size_t)j * m.ni + i];
+    };
+    auto hasseg = [&](int i, int j) -> int {
+        int li = i - n_i0, lj = j - n_j0;
+        if (li >= 0 && li < TILE_NW && lj >= 0 && lj < TILE_NH) return L.hs[lj * TILE_NW + li];
+        return md.node_hasseg[(size_t)j * m.ni + i];
     };
     auto slot = [&](int v) -> int64_t {
         int r = 0;
         while (r < TILE_H - 1 && v >= L.seg_pre[r + 1]) r++;
         return (int64_t)L.seg_start[r] + (v - L.seg_pre[r]);
     };
+    auto fetch = [&](int base, ChunkRegs& R) {
+        const int cn = min(CHUNK, ntile - base);
+#pragma unroll
+        for (int q = 0; q < PER_T; q++) {
+            const int k = t + q * PUSH_THREADS;
+            if (k < cn) {
+                int64_t g = slot(base + k);
+                R.c[q] = s.cell[g];
+                R.fx[q] = s.fx[g];
+                R.fy[q] = s.fy[g];
+                R.vx[q] = s.vx[g];
+                R.vy[q] = s.vy[g];
+                R.vz[q] = s.vz[g];
+            }
+        }
+    };
     StatAcc st{0, 0, 0, 0, 0};
+    ChunkRegs R;
+    if (ntile > 0) fetch(0, R);
 
     for (int base = 0; base < ntile; base += CHUNK) {
         const int cn = min(CHUNK, ntile - base);
-        // stage the chunk (coalesced loads)
-        for (int k = t; k < cn; k += PUSH_THREADS) {
-            int64_t g = slot(base + k);
-            L.s_cell[k] = s.cell[g];
-            L.s_fx[k] = s.fx[g];
-            L.s_fy[k] = s.fy[g];
-            L.s_vx[k] = s.vx[g];
-            L.s_vy[k] = s.vy[g];
-            L.s_vz[k] = s.vz[g];
+        // stage the chunk held in registers, then start loading the next one
+#pragma unroll
+        for (int q = 0; q < PER_T; q++) {
+            const int k = t + q * PUSH_THREADS;
+            if (k < cn) {
+                L.s_cell[k] = R.c[q];
+                L.s_fx[k] = R.fx[q];
+                L.s_fy[k] = R.fy[q];
+                L.s_vx[k] = R.vx[q];
+                L.s_vy[k] = R.vy[q];
+                L.s_vz[k] = R.vz[q];
+            }
         }
         __syncthreads();
+        if (base + CHUNK < ntile) fetch(base + CHUNK, R);
         // process in a cell-strided order so that the lanes of a wave hit different cells
-        for (int kk = 0; kk < CHUNK / PUSH_THREADS; kk++) {
+        for (int kk = 0; kk < PER_T; kk++) {
             int k = ((t + kk * PUSH_THREADS) * CHUNK_STRIDE) & (CHUNK - 1);
             if (k >= cn) continue;
             int32_t c = L.s_cell[k];
@@ -214,7 +254,7 @@ __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile(MeshP m, MeshD
                 double lc0 = cell_i(c) + (double)L.s_fx[k];
                 double lc1 = cell_j(c) + (double)L.s_fy[k];
                 double v0 = L.s_vx[k], v1 = L.s_vy[k], v2 = L.s_vz[k];
-                bool alive = push_particle(m, md, pc, load_e, lc0, lc1, v0, v1, v2);
+                bool alive = push_particle(m, md, pc, load_e, hasseg, lc0, lc1, v0, v1, v2);
                 if (alive) {
                     int ci, cj;
                     new_cell(m, lc0, lc1, ci, cj);
@@ -306,6 +346,7 @@ __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, Push
                                                          int32_t* __restrict__ counts, long long* stats) {
     const size_t nn = (size_t)m.ni * m.nj;
     auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
+    auto hasseg = [&](int i, int j) -> int { return md.node_hasseg[(size_t)j * m.ni + i]; };
     StatAcc st{0, 0, 0, 0, 0};
     for (int64_t p = p0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < p1;
          p += (int64_t)gridDim.x * blockDim.x) {
@@ -314,7 +355,7 @@ __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, Push
             continue;
         }
         PIn q = load_particle(s, p);
-        bool alive = push_particle(m, md, pc, load_e, q.lc0, q.lc1, q.v0, q.v1, q.v2);
+        bool alive = push_particle(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
         int32_t key = -1;
         if (alive) {
             int ci, cj;
@@ -467,6 +508,7 @@ template <bool IDS>
 __global__ void __launch_bounds__(256) push_scatter(MeshP m, MeshD md, PushC pc, PState s, PState d, int64_t np,
                                                     const int32_t* __restrict__ keys, int32_t* __restrict__ cursor) {
     auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
+    auto hasseg = [&](int i, int j) -> int { return md.node_hasseg[(size_t)j * m.ni + i]; };
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t nloop = (np + stride - 1) / stride * stride;   // uniform trip count for the wave ops
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nloop; p += stride) {
@@ -478,7 +520,7 @@ __global__ void __launch_bounds__(256) push_scatter(MeshP m, MeshD md, PushC pc,
         int32_t idv = 0;
         if (valid) {
             PIn q = load_particle(s, p);
-            push_particle(m, md, pc, load_e, q.lc0, q.lc1, q.v0, q.v1, q.v2);
+            push_particle(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
             ci = cell_i(key);
             cj = cell_j(key);
             lkey = cj * m.nci + ci;
