@@ -37,7 +37,63 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sort-interval", type=int, default=4)
+    ap.add_argument("--learned-configs", type=int, default=8,
+                    help="C2-style runs (random cylinder radius / bias) that provide training snapshots")
+    ap.add_argument("--learned-train-steps", type=int, default=600)
+    ap.add_argument("--no-learned", action="store_true")
     return ap.parse_args()
+
+
+def learned_guess(sim, deck, args):
+    """Trains the learned initial guess on snapshots of C2-style runs (cylinder radius
+    U(0.02, 0.1) m, bias U(-200, 0) V; none is the benchmark's cylinder) and compares, on the
+    benchmark's own states, the solve from phi = 0 (plain), from the previous step's phi (warm)
+    and from the network's guess (learned).  Every solve runs to the same residual tolerance."""
+    import ctypes as C
+    import numpy as np
+    from starfish_amd.learned import LearnedGuess, generate_c2_snapshots, solve_from
+    t0 = time.perf_counter()
+    m = deck.mesh
+    net = LearnedGuess(sim.ctx, max_batch=4, seed=1)
+    try:
+        generate_c2_snapshots(net, args.learned_configs, 24, 3, int(args.particles), m.ni, m.nj, seed=12345,
+                              skip=3)
+        t_gen = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        loss = 0.0
+        for r in range(4):
+            loss = net.train(args.learned_train_steps // 4, batch=4, lr=1e-3 if r < 3 else 3e-4)
+        sim.ctx.call("sf_ctx_synchronize")
+        t_train = time.perf_counter() - t1
+        rows = []
+        mat = sim.materials[0]
+        for _ in range(4):
+            inj = C.c_int64()
+            sim.ctx.call("sf_sources_sample", C.byref(inj))
+            mat.update()
+            row = {}
+            for start in ("zero", "current", "learned"):
+                info, rc = solve_from(sim, start, net)
+                row[start] = (info.pcg_iterations, info.newton_iterations, info.ms, info.rel_residual, rc)
+            rows.append(row)
+            sim.update_fields()
+            sim.it += 1
+
+        def agg(key, k):
+            return float(np.mean([r[key][k] for r in rows]))
+        out = {"samples": net.num_samples, "configs": args.learned_configs, "train_steps": args.learned_train_steps,
+               "final_loss": loss, "generation_s": round(t_gen, 2), "training_s": round(t_train, 2),
+               "states": len(rows),
+               "plain": {"pcg_iterations": agg("zero", 0), "newton": agg("zero", 1), "ms": agg("zero", 2)},
+               "warm": {"pcg_iterations": agg("current", 0), "newton": agg("current", 1), "ms": agg("current", 2)},
+               "learned": {"pcg_iterations": agg("learned", 0), "newton": agg("learned", 1), "ms": agg("learned", 2)},
+               "worst_rel_residual": max(max(r[k][3] for k in r) for r in rows),
+               "all_converged": all(r[k][4] == 0 for r in rows for k in r)}
+        out["iteration_ratio_plain_over_learned"] = out["plain"]["pcg_iterations"] / max(1.0, out["learned"]["pcg_iterations"])
+        return out, net
+    except Exception:
+        net.close()
+        raise
 
 
 def cpu_baseline(sim, deck, threads, target_s=15.0):
@@ -155,6 +211,10 @@ def main():
     if not args.no_cpu_baseline:
         thr = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(sim, deck, thr)
+    learned = None
+    if not args.no_learned:
+        learned, net = learned_guess(sim, deck, args)
+        net.close()
     out = {
         "metric": "particle push+deposit/s (2D ES-PIC, 1024x512 XY, 2e7 particles)",
         "value": value,
@@ -180,7 +240,7 @@ def main():
                               "push_scatter": kern[2] / args.steps, "normalize": kern[3] / args.steps},
         "pcg": {"ms_per_solve": float(np.mean(solve_ms)), "pcg_iterations_per_solve": float(np.mean(pcg_its)),
                 "newton_per_solve": float(np.mean(newton_its)), "worst_rel_residual": worst_res,
-                "rel_tol": deck.solver.rel_tol, "cold_start": cold_info, "learned_guess": None},
+                "rel_tol": deck.solver.rel_tol, "cold_start": cold_info, "learned_guess": learned},
     }
     sim.close()
     print(json.dumps(out), flush=True)
