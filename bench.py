@@ -196,8 +196,8 @@ def main():
     def rate(k, nbytes):
         return nbytes * k[1] / (k[0] * 1e-3) / 1e9 if k[0] > 0 else 0.0
 
-    kernels = [("push_deposit_tile<inplace>", k_inplace, BYTES_PUSH_INPLACE),
-               ("push_deposit_tile<sort>", k_sortA, BYTES_PUSH_DEPOSIT),
+    kernels = [("push_deposit_tile_inplace", k_inplace, BYTES_PUSH_INPLACE),
+               ("push_deposit_tile_sort", k_sortA, BYTES_PUSH_DEPOSIT),
                ("push_scatter", k_scatter, BYTES_PUSH_SCATTER)]
     dname, dk, dbytes = max(kernels, key=lambda e: e[1][0])
     achieved = rate(dk, dbytes)

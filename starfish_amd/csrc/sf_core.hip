@@ -68,6 +68,8 @@ int sf_ctx_create(int device, sf_ctx** out) {
     sf_ctx* c = new sf_ctx();
     c->device = device;
     c->rng.set_seed(0);    // Starfish's shared generator is new Random(0)
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->ncu <= 0)
+        c->ncu = 256;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return SF_ERR_HIP;

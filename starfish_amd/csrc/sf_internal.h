@@ -132,6 +132,8 @@ void poisson_free(PoissonState* p);
 
 struct sf_ctx {
     int device = 0;
+    int ncu = 0;                    // compute units of the device (persistent-grid sizing)
+    int grid_inplace = 0, grid_sort = 0;   // resident workgroups of the persistent tile kernels
     hipStream_t stream = nullptr;
     std::string err;
     double dt = 1e-7;

@@ -132,31 +132,49 @@ struct TileLds {
     int hist[TILE_CELLS];
     float2 ef[TILE_NODES];
     uint8_t hs[TILE_NODES];
-    int32_t s_cell[CHUNK];
-    float s_fx[CHUNK], s_fy[CHUNK], s_vx[CHUNK], s_vy[CHUNK], s_vz[CHUNK];
     int seg_start[TILE_H], seg_pre[TILE_H + 1];
+    // results of one window, by slot: written in the permuted order, stored coalesced
+    int32_t o_cell[PUSH_THREADS];
+    float o_fx[PUSH_THREADS], o_fy[PUSH_THREADS], o_vx[PUSH_THREADS], o_vy[PUSH_THREADS], o_vz[PUSH_THREADS];
 };
 
-// One chunk of staged particle state held in registers (CHUNK / PUSH_THREADS per thread), so that
-// the next chunk's loads are in flight while the current chunk is pushed.
-constexpr int PER_T = CHUNK / PUSH_THREADS;
-struct ChunkRegs {
-    int32_t c[PER_T];
-    float fx[PER_T], fy[PER_T], vx[PER_T], vy[PER_T], vz[PER_T];
+// Particle state in registers (loaded one iteration ahead of its use).
+struct PReg {
+    int64_t g;
+    int32_t c;
+    float fx, fy, vx, vy, vz;
 };
 
 // INPLACE = false: sort step, the new cell of every particle goes to keys[] (pass B moves it).
 // INPLACE = true: the pushed state is written back to the particle's own slot; dead particles
 // become holes (cell = -1) until the next sort.
+// The tile's particles (its cell rows, contiguous after the last sort) are visited in windows of
+// PUSH_THREADS; thread t takes the window's particle (t * CHUNK_STRIDE) mod PUSH_THREADS, so the
+// lanes of a wave fall about one cell apart and their LDS atomics hit different nodes.  Each
+// thread loads its particle straight into registers, one window ahead.  Results go back through
+// LDS by slot and are stored coalesced: stores in the permuted order write partial cache lines
+// and run at about half the bandwidth of coalesced ones on this part.
 template <bool INPLACE>
-__global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile(MeshP m, MeshD md, PushC pc, PState s,
-                                                                   const int32_t* __restrict__ offsets,
-                                                                   int32_t* __restrict__ keys,
-                                                                   unsigned long long* __restrict__ acc,
-                                                                   int32_t* __restrict__ counts, long long* stats) {
+__device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const MeshD& md, const PushC& pc,
+                                                       const PState& s, const int32_t* __restrict__ offsets,
+                                                       int32_t* __restrict__ keys,
+                                                       unsigned long long* __restrict__ acc,
+                                                       int32_t* __restrict__ counts, long long* stats) {
     __shared__ TileLds L;
+    __shared__ int s_tile;
     const int t = threadIdx.x;
-    const int tx = blockIdx.x % m.ntx, ty = blockIdx.x / m.ntx;
+    const int ntiles = m.ntx * m.nty;
+    unsigned int* tile_counter = (unsigned int*)&stats[7];
+    StatAcc st{0, 0, 0, 0, 0};
+    // Persistent workgroups take tiles from a counter until none is left, so tiles that
+    // take longer (bounces near objects, particles that left the LDS window) do not leave the
+    // rest of the device idle at the end of the launch.
+    for (;;) {
+    if (t == 0) s_tile = (int)atomicAdd(tile_counter, 1u);
+    __syncthreads();
+    const int tile = s_tile;
+    if (tile >= ntiles) break;
+    const int tx = tile % m.ntx, ty = tile / m.ntx;
     const int ti0 = tx * TILE_W, tj0 = ty * TILE_H;            // first cell of the tile
     const int n_i0 = ti0 - TILE_HALO, n_j0 = tj0 - TILE_HALO;  // first node of the LDS tile
     const int c_i0 = ti0 - TILE_HALO, c_j0 = tj0 - TILE_HALO;  // first cell of the LDS histogram
@@ -189,7 +207,6 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile(MeshP m, Me
         L.seg_pre[TILE_H] = pre;
     }
     __syncthreads();
-    const int ntile = L.seg_pre[TILE_H];
 
     // clamped loads reproduce the global array for in-mesh nodes (the window is clamped above)
     auto load_e = [&](int i, int j) -> float2 {
@@ -202,121 +219,106 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile(MeshP m, Me
         if (li >= 0 && li < TILE_NW && lj >= 0 && lj < TILE_NH) return L.hs[lj * TILE_NW + li];
         return md.node_hasseg[(size_t)j * m.ni + i];
     };
-    auto slot = [&](int v) -> int64_t {
-        int r = 0;
-        while (r < TILE_H - 1 && v >= L.seg_pre[r + 1]) r++;
-        return (int64_t)L.seg_start[r] + (v - L.seg_pre[r]);
-    };
-    auto fetch = [&](int base, ChunkRegs& R) {
-        const int cn = min(CHUNK, ntile - base);
-#pragma unroll
-        for (int q = 0; q < PER_T; q++) {
-            const int k = t + q * PUSH_THREADS;
-            if (k < cn) {
-                int64_t g = slot(base + k);
-                R.c[q] = s.cell[g];
-                R.fx[q] = s.fx[g];
-                R.fy[q] = s.fy[g];
-                R.vx[q] = s.vx[g];
-                R.vy[q] = s.vy[g];
-                R.vz[q] = s.vz[g];
-            }
+    const int pt = (t * CHUNK_STRIDE) & (PUSH_THREADS - 1);
+    // The tile's particles: TILE_H contiguous row segments, each visited in windows of
+    // PUSH_THREADS slots (the last window of a row may be partial).
+    for (int cr = 0; cr < TILE_H; cr++) {
+    const int rlen = L.seg_pre[cr + 1] - L.seg_pre[cr];
+    const int64_t rst = L.seg_start[cr];
+    for (int cb = 0; cb < rlen; cb += PUSH_THREADS) {
+        const bool ok = cb + pt < rlen;
+        PReg cur;
+        if (ok) {
+            cur.g = rst + cb + pt;
+            cur.c = s.cell[cur.g];
+            cur.fx = s.fx[cur.g];
+            cur.fy = s.fy[cur.g];
+            cur.vx = s.vx[cur.g];
+            cur.vy = s.vy[cur.g];
+            cur.vz = s.vz[cur.g];
+        } else {
+            cur.g = 0;
+            cur.c = -1;
+            cur.fx = cur.fy = cur.vx = cur.vy = cur.vz = 0.0f;
         }
-    };
-    StatAcc st{0, 0, 0, 0, 0};
-    ChunkRegs R;
-    if (ntile > 0) fetch(0, R);
-
-    for (int base = 0; base < ntile; base += CHUNK) {
-        const int cn = min(CHUNK, ntile - base);
-        // stage the chunk held in registers, then start loading the next one
+        int32_t key = -1;
+        float fxn = cur.fx, fyn = cur.fy, vxn = cur.vx, vyn = cur.vy, vzn = cur.vz;
+        if (ok && cur.c >= 0) {
+            double lc0 = cell_i(cur.c) + (double)cur.fx;
+            double lc1 = cell_j(cur.c) + (double)cur.fy;
+            double v0 = cur.vx, v1 = cur.vy, v2 = cur.vz;
+            bool alive = push_particle(m, md, pc, load_e, hasseg, lc0, lc1, v0, v1, v2);
+            if (alive) {
+                int ci, cj;
+                new_cell(m, lc0, lc1, ci, cj);
+                key = pack_cell(ci, cj);
+                fxn = (float)(lc0 - ci);
+                fyn = (float)(lc1 - cj);
+                vxn = (float)v0;
+                vyn = (float)v1;
+                vzn = (float)v2;
+                stat_add(st, pc, vxn, vyn, vzn);
+                // histogram of the new cell
+                int hi = ci - c_i0, hj = cj - c_j0;
+                if (hi >= 0 && hi < TILE_CW && hj >= 0 && hj < TILE_CH) atomicAdd(&L.hist[hj * TILE_CW + hi], 1);
+                else atomicAdd(&counts[cj * m.nci + ci], 1);
+                // deposit from the stored (FP32) state; Field2D.scatter skips lc >= n-1
+                double di = (double)fxn, dj = (double)fyn;
+                if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
+                    double val[NCH];
+                    channel_values(pc, vxn, vyn, vzn, val);
+                    int li = ci - n_i0, lj = cj - n_j0;
+                    if (li >= 0 && li + 1 < TILE_NW && lj >= 0 && lj + 1 < TILE_NH) {
+                        double w4[4];
+                        scatter_weights(m, ci, cj, di, dj, w4);
+                        const int off[4] = {0, 1, TILE_NW + 1, TILE_NW};
+                        int l0 = lj * TILE_NW + li;
 #pragma unroll
-        for (int q = 0; q < PER_T; q++) {
-            const int k = t + q * PUSH_THREADS;
-            if (k < cn) {
-                L.s_cell[k] = R.c[q];
-                L.s_fx[k] = R.fx[q];
-                L.s_fy[k] = R.fy[q];
-                L.s_vx[k] = R.vx[q];
-                L.s_vy[k] = R.vy[q];
-                L.s_vz[k] = R.vz[q];
-            }
-        }
-        __syncthreads();
-        if (base + CHUNK < ntile) fetch(base + CHUNK, R);
-        // process in a cell-strided order so that the lanes of a wave hit different cells
-        for (int kk = 0; kk < PER_T; kk++) {
-            int k = ((t + kk * PUSH_THREADS) * CHUNK_STRIDE) & (CHUNK - 1);
-            if (k >= cn) continue;
-            int32_t c = L.s_cell[k];
-            int32_t key = -1;
-            if (c >= 0) {
-                double lc0 = cell_i(c) + (double)L.s_fx[k];
-                double lc1 = cell_j(c) + (double)L.s_fy[k];
-                double v0 = L.s_vx[k], v1 = L.s_vy[k], v2 = L.s_vz[k];
-                bool alive = push_particle(m, md, pc, load_e, hasseg, lc0, lc1, v0, v1, v2);
-                if (alive) {
-                    int ci, cj;
-                    new_cell(m, lc0, lc1, ci, cj);
-                    key = pack_cell(ci, cj);
-                    float fxn = (float)(lc0 - ci), fyn = (float)(lc1 - cj);
-                    float vxn = (float)v0, vyn = (float)v1, vzn = (float)v2;
-                    stat_add(st, pc, vxn, vyn, vzn);
-                    // histogram of the new cell
-                    int hi = ci - c_i0, hj = cj - c_j0;
-                    if (hi >= 0 && hi < TILE_CW && hj >= 0 && hj < TILE_CH) atomicAdd(&L.hist[hj * TILE_CW + hi], 1);
-                    else atomicAdd(&counts[cj * m.nci + ci], 1);
-                    // deposit from the stored (FP32) state; Field2D.scatter skips lc >= n-1
-                    double di = (double)fxn, dj = (double)fyn;
-                    if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
-                        double val[NCH];
-                        channel_values(pc, vxn, vyn, vzn, val);
-                        int li = ci - n_i0, lj = cj - n_j0;
-                        if (li >= 0 && li + 1 < TILE_NW && lj >= 0 && lj + 1 < TILE_NH) {
-                            double w[4];
-                            scatter_weights(m, ci, cj, di, dj, w);
-                            const int off[4] = {0, 1, TILE_NW + 1, TILE_NW};
-                            int l0 = lj * TILE_NW + li;
+                        for (int q = 0; q < 4; q++) {
 #pragma unroll
-                            for (int q = 0; q < 4; q++) {
-#pragma unroll
-                                for (int ch = 0; ch < NCH; ch++) {
-                                    long long v = quant(w[q], val[ch]);
-                                    atomicAdd(&L.acc[ch][l0 + off[q]], (unsigned long long)v);
-                                }
+                            for (int ch = 0; ch < NCH; ch++) {
+                                long long v = quant(w4[q], val[ch]);
+                                atomicAdd(&L.acc[ch][l0 + off[q]], (unsigned long long)v);
                             }
-                        } else {
-                            deposit_global(m, acc, nn, ci, cj, di, dj, val);
                         }
-                    }
-                    if (INPLACE) {
-                        L.s_fx[k] = fxn;
-                        L.s_fy[k] = fyn;
-                        L.s_vx[k] = vxn;
-                        L.s_vy[k] = vyn;
-                        L.s_vz[k] = vzn;
+                    } else {
+                        deposit_global(m, acc, nn, ci, cj, di, dj, val);
                     }
                 }
             }
-            L.s_cell[k] = key;   // new packed cell, -1 for dead (the staged state is not reread)
+        }
+        if (ok) {
+            L.o_cell[pt] = key;   // -1: dead, a hole until the next sort
+            if (INPLACE) {
+                L.o_fx[pt] = fxn;
+                L.o_fy[pt] = fyn;
+                L.o_vx[pt] = vxn;
+                L.o_vy[pt] = vyn;
+                L.o_vz[pt] = vzn;
+            }
         }
         __syncthreads();
-        for (int k = t; k < cn; k += PUSH_THREADS) {
-            int64_t g = slot(base + k);
+        const int wlen = min(PUSH_THREADS, rlen - cb);
+        if (t < wlen) {
+            const int64_t g = rst + cb + t;
+            const int32_t kc = L.o_cell[t];
             if (INPLACE) {
-                s.cell[g] = L.s_cell[k];
-                s.fx[g] = L.s_fx[k];
-                s.fy[g] = L.s_fy[k];
-                s.vx[g] = L.s_vx[k];
-                s.vy[g] = L.s_vy[k];
-                s.vz[g] = L.s_vz[k];
+                s.cell[g] = kc;
+                if (kc >= 0) {
+                    s.fx[g] = L.o_fx[t];
+                    s.fy[g] = L.o_fy[t];
+                    s.vx[g] = L.o_vx[t];
+                    s.vy[g] = L.o_vy[t];
+                    s.vz[g] = L.o_vz[t];
+                }
             } else {
-                keys[g] = L.s_cell[k];
+                keys[g] = kc;
             }
         }
         __syncthreads();
     }
-    stat_flush(st, stats);
+    }
+    __syncthreads();
     // flush LDS accumulators (nonzero entries) with 64-bit integer atomics
     for (int k = t; k < TILE_NODES; k += PUSH_THREADS) {
         int i = n_i0 + k % TILE_NW, j = n_j0 + k / TILE_NW;
@@ -334,6 +336,26 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile(MeshP m, Me
         int ci = c_i0 + k % TILE_CW, cj = c_j0 + k / TILE_CW;
         atomicAdd(&counts[cj * m.nci + ci], h);
     }
+    __syncthreads();   // LDS and s_tile are reused by the next tile
+    }
+    stat_flush(st, stats);
+}
+
+// In-place step: four workgroups per CU (39 KB of LDS each, at most 128 VGPRs).
+__global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile_inplace(MeshP m, MeshD md, PushC pc, PState s,
+                                                                           const int32_t* __restrict__ offsets,
+                                                                           unsigned long long* __restrict__ acc,
+                                                                           int32_t* __restrict__ counts,
+                                                                           long long* stats) {
+    push_deposit_tile_body<true>(m, md, pc, s, offsets, nullptr, acc, counts, stats);
+}
+// Sort step (pass A): the key write needs a few more registers; no register cap, so no spills.
+__global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile_sort(MeshP m, MeshD md, PushC pc, PState s,
+                                                                      const int32_t* __restrict__ offsets,
+                                                                      int32_t* __restrict__ keys,
+                                                                      unsigned long long* __restrict__ acc,
+                                                                      int32_t* __restrict__ counts, long long* stats) {
+    push_deposit_tile_body<false>(m, md, pc, s, offsets, keys, acc, counts, stats);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -524,8 +546,13 @@ __global__ void __launch_bounds__(256) push_scatter(MeshP m, MeshD md, PushC pc,
             ci = cell_i(key);
             cj = cell_j(key);
             lkey = cj * m.nci + ci;
-            fxn = (float)(q.lc0 - ci);
-            fyn = (float)(q.lc1 - cj);
+            // the slot belongs to pass A's cell; should the recomputed push land a rounding step
+            // across its edge, the offset is clamped into that cell (an offset of exactly 1 is
+            // legitimate only in the last cell row / column: a particle on the last node line)
+            fxn = fmaxf((float)(q.lc0 - ci), 0.0f);
+            fyn = fmaxf((float)(q.lc1 - cj), 0.0f);
+            if (ci < m.nci - 1) fxn = fminf(fxn, 0.99999994f);
+            if (cj < m.ncj - 1) fyn = fminf(fyn, 0.99999994f);
             vxn = (float)q.v0;
             vyn = (float)q.v1;
             vzn = (float)q.v2;
@@ -767,6 +794,22 @@ int sort_species(sf_ctx* ctx, Species* sp) {
     return SF_OK;
 }
 
+// persistent grid of a tile kernel: the workgroups that are resident at once (occupancy x CUs),
+// at most one per tile
+static int tile_grid(sf_ctx* ctx, bool inplace) {
+    int& g = inplace ? ctx->grid_inplace : ctx->grid_sort;
+    if (g == 0) {
+        int per_cu = 0;
+        hipError_t e = inplace ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, push_deposit_tile_inplace,
+                                                                                PUSH_THREADS, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, push_deposit_tile_sort,
+                                                                                PUSH_THREADS, 0);
+        if (e != hipSuccess || per_cu <= 0) per_cu = 2;
+        g = per_cu * std::max(1, ctx->ncu);
+    }
+    return std::max(1, std::min(ctx->mp.ntx * ctx->mp.nty, g));
+}
+
 int push_deposit(sf_ctx* ctx, Species* sp) {
     if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
     const MeshP& m = ctx->mp;
@@ -783,7 +826,7 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (sort_step) {
         if (sp->nsorted > 0)
-            hipLaunchKernelGGL(push_deposit_tile<false>, dim3(m.ntx * m.nty), dim3(PUSH_THREADS), 0, ctx->stream, m, md,
+            hipLaunchKernelGGL(push_deposit_tile_sort, dim3(tile_grid(ctx, false)), dim3(PUSH_THREADS), 0, ctx->stream, m, md,
                                pc, s, sp->offsets, sp->key, sp->acc, sp->counts, sp->stats);
         if (sp->np > sp->nsorted)
             hipLaunchKernelGGL(push_deposit_tail<false>, dim3(grid_for(sp->np - sp->nsorted, 256, 4096)), dim3(256), 0,
@@ -791,8 +834,8 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
                                sp->stats);
     } else {
         if (sp->nsorted > 0)
-            hipLaunchKernelGGL(push_deposit_tile<true>, dim3(m.ntx * m.nty), dim3(PUSH_THREADS), 0, ctx->stream, m, md,
-                               pc, s, sp->offsets, sp->key, sp->acc, sp->counts, sp->stats);
+            hipLaunchKernelGGL(push_deposit_tile_inplace, dim3(tile_grid(ctx, true)), dim3(PUSH_THREADS), 0, ctx->stream, m,
+                               md, pc, s, sp->offsets, sp->acc, sp->counts, sp->stats);
         if (sp->np > sp->nsorted)
             hipLaunchKernelGGL(push_deposit_tail<true>, dim3(grid_for(sp->np - sp->nsorted, 256, 4096)), dim3(256), 0,
                                ctx->stream, m, md, pc, s, sp->nsorted, sp->np, sp->key, sp->acc, sp->counts,

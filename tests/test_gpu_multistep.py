@@ -92,3 +92,44 @@ def test_full_size_invariants_and_determinism():
         finally:
             sim.close()
     assert np.array_equal(nds[0], nds[1])
+
+
+@pytest.mark.gpu
+def test_c3_rz_full_size_invariants():
+    """RZ 2048x1024 nodes, 1e8 Xe+ preloaded (the RZ configuration at full size): after an initial
+    solve and two steps, the mover's survivor count equals the sort total, the Ruyten-weighted
+    deposit carries every particle's weight (sum nd * nodeVol = spwt * deposited particles), and
+    the solve reaches its residual."""
+    from starfish_amd.plugin import Simulation
+    deck = D.config_c3(particles=100_000_000)
+    sim = Simulation(deck, capacity=110_000_000)
+    try:
+        mat = sim.materials[0]
+        sim.ctx.call("sf_species_set_sort_interval", mat.index, 2)
+        npart, drift, temp = deck.preload
+        mat.preload(npart, drift, temp, seed=3)
+        assert mat.getNp() == npart
+        rc = sim.initial_solve()
+        print("C3 initial solve", sim.solve_infos[-1].newton_iterations, sim.solve_infos[-1].pcg_iterations,
+              sim.solve_infos[-1].rel_residual, sim.solve_infos[-1].ms, flush=True)
+        assert rc == 0 and sim.solve_infos[-1].rel_residual <= 1e-8
+        for _ in range(2):
+            mat.update()
+            st = mat.stats()
+            alive = st[0] / (deck.materials[0].spwt * deck.materials[0].molwt * D.AMU)
+            assert abs(alive - mat.getNp()) < 0.5
+            sim.update_fields()
+            assert sim.solve_infos[-1].rel_residual <= 1e-8
+        p = mat.particles()
+        m = deck.mesh
+        on_edge = (p["lc0"] >= m.ni - 1) | (p["lc1"] >= m.nj - 1)
+        nd = mat.field("nd")
+        w = (nd * sim.geo.node_vol).sum() / deck.materials[0].spwt
+        print("C3 particles %d, deposited weight %.6e" % (len(p["lc0"]), w), flush=True)
+        assert abs(w - (len(p["lc0"]) - on_edge.sum())) < 1e-6 * len(p["lc0"])
+        key = np.floor(p["lc1"]).clip(0, m.nj - 2) * (m.ni - 1) + np.floor(p["lc0"]).clip(0, m.ni - 2)
+        bad = int(np.count_nonzero(np.diff(key) < 0))
+        print("C3 cell-order violations: %d" % bad, flush=True)
+        assert bad == 0    # the second step sorted (interval 2 after the preload sort)
+    finally:
+        sim.close()
