@@ -26,6 +26,22 @@ BYTES_PUSH_INPLACE = 48.0   # in-place step: read + write the 24 B particle stat
 BYTES_PUSH_DEPOSIT = 28.0   # sort step, pass A: read 24 B of state, write the 4 B new-cell key
 BYTES_PUSH_SCATTER = 52.0   # sort step, pass B: read 24 B state + 4 B key, write 24 B state
 BYTES_STATE = 48.0          # read + write of the 24 B particle state (minimal push + deposit)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2_bench.txt")
+TRAFFIC_PROFILE_PARTICLES = 19.8e6   # average particles per push dispatch in the profiled runs
+
+
+def measured_bytes_per_particle(kernel):
+    """HBM bytes per particle of one push kernel, from the committed FETCH_SIZE + WRITE_SIZE summary
+    (already scaled by the copy-kernel calibration); None when the summary or the kernel is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if parts and parts[0] == kernel:
+                    return float(parts[-1]) / TRAFFIC_PROFILE_PARTICLES
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -201,8 +217,14 @@ def main():
                ("push_scatter", k_scatter, BYTES_PUSH_SCATTER)]
     dname, dk, dbytes = max(kernels, key=lambda e: e[1][0])
     achieved = rate(dk, dbytes)
+    per_launch = dk[1] / max(1, dk[2])
+    bpp = measured_bytes_per_particle(dname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_HBM_GBS, 4),
+                "traffic": round(bpp * per_launch) if bpp else None,
+                "traffic_bytes_per_particle": round(bpp, 2) if bpp else None,
+                "algorithmic_bytes_per_launch": round(dbytes * per_launch),
+                "particles_per_launch": round(per_launch),
                 "kernel": dname, "bytes_per_particle": dbytes,
                 "ms_per_launch": round(dk[0] / max(1, dk[2]), 4), "launches": dk[2],
                 "per_kernel_GBs": {name: round(rate(k, b), 1) for name, k, b in kernels},
