@@ -220,10 +220,22 @@ int sf_rho_get(sf_ctx* ctx, double* rho);
 int sf_poisson_solve(sf_ctx* ctx, double* phi_io, sf_solve_info* info);
 int sf_phi_set(sf_ctx* ctx, const double* phi);
 int sf_phi_get(sf_ctx* ctx, double* phi);
+/* BoltzmannElectronsMaterial.electronModelBoltzmann (core/materials/BoltzmannElectronsMaterial.java:
+   179-215): den = den0 exp((phi - phi0)/kTe0) floored at 1 m^-3, from the device-resident phi, in
+   node order u = j*ni + i.  den0 <= 0 or kTe0 <= 0 takes den0, kTe0 and phi0 from the solver (as
+   the reference takes them from the PotentialSolver).  The electron temperature field is the
+   constant kTe0 * EVtoK and is left to the caller. */
+int sf_boltzmann_density(sf_ctx* ctx, double den0, double kTe0, double phi0, double* den);
 /* PotentialSolver.updateGradientField (pic/PotentialSolver.java:55-70): E = -grad(phi) */
 int sf_efield_update(sf_ctx* ctx);
 int sf_efield_set(sf_ctx* ctx, const double* efi, const double* efj);
 int sf_efield_get(sf_ctx* ctx, double* efi, double* efj);
+/* Magnetic field of the domain (DomainModule "bfi"/"bfj", core/domain/DomainModule.java:174,
+   filled by load_field in the reference), node order u = j*ni + i, tesla, kept in FP32 like E.
+   Where the gathered (Bi, Bj) is nonzero the mover and the addParticle rewind use
+   UpdateVelocityBoris (core/materials/KineticMaterial.java:336-353, 779-794, 847-893); elsewhere
+   the electric kick.  Both NULL, or a field that is zero everywhere, removes it. */
+int sf_bfield_set(sf_ctx* ctx, const double* bfi, const double* bfj);
 /* y = A x with the reference operator (diagnostic; x, y host [ni*nj] FP64) */
 int sf_operator_apply(sf_ctx* ctx, const double* x, double* y);
 
@@ -243,6 +255,10 @@ int sf_sources_sample(sf_ctx* ctx, int64_t* injected);
  * out4[2] pass B (push + scatter; exactly 0 on an in-place step), out4[3] normalisation.
  * ------------------------------------------------------------------------------------------ */
 int sf_last_timings(sf_ctx* ctx, double* out4);
+/* Split of out4[0]: out3[0] the tiled kernel over the cell-sorted particles, out3[1] the kernel
+ * over the particles appended since the last sort (injected ones), out3[2] how many slots that
+ * second kernel covered. */
+int sf_last_push_timings(sf_ctx* ctx, double* out3);
 int sf_set_timing(sf_ctx* ctx, int enabled);
 
 /* ------------------------------------------------------------------------------------------

@@ -68,6 +68,7 @@ def load():
         "orc_fixed_nodes": (C.c_int, [P, IP]),
         "orc_set_rng_state": (None, [P, C.c_int64]),
         "orc_round_state_fp32": (None, [P, C.c_int]),
+        "orc_boltzmann_density": (None, [C.c_int64, DP, C.c_double, C.c_double, C.c_double, DP]),
         "orc_javarandom_doubles": (None, [C.c_int64, C.c_int, DP]),
         "orc_javarandom_ints": (None, [C.c_int64, C.c_int, IP]),
         "orc_javarandom_ints_bounded": (None, [C.c_int64, C.c_int, C.c_int, IP]),
@@ -214,6 +215,14 @@ class Oracle:
         out = np.zeros(self.nn, np.int32)
         self.lib.orc_fixed_nodes(self.h, _ip(out))
         return out
+
+
+def boltzmann_density(phi, den0, kTe0, phi0):
+    """Boltzmann electron density of the reference's BOLTZMANN electron model on every node."""
+    phi = np.ascontiguousarray(phi, np.float64)
+    out = np.zeros_like(phi)
+    load().orc_boltzmann_density(phi.size, _dp(phi), den0, kTe0, phi0, _dp(out))
+    return out
 
 
 def java_doubles(seed, n):

@@ -401,6 +401,7 @@ struct orc_sim {
     std::vector<std::unique_ptr<Species>> species;
     std::vector<Source> sources;
     Field phi, rho, efi, efj;
+    Field bfi, bfj;               /* DomainModule "bfi"/"bfj" (DomainModule.java:174); zero unless loaded */
     std::atomic<double> boundary_charge{0.0};
     void add_boundary_charge(double q) {
         double cur = boundary_charge.load();
@@ -682,6 +683,36 @@ struct orc_sim {
     }
 
     /* ---------------- particles ---------------- */
+    /* UpdateVelocityBoris (KineticMaterial.java:847-893) with E = (Ei, Ej, 0) and B = (Bi, Bj, 0):
+       half electric kick, rotation by t = (q/m) B dt/2 and s = 2t/(1+|t|^2), half kick */
+    static void boris(double qm, double dtp, const double E[3], const double B[3], double vel[3]) {
+        double t[3], s[3], vm[3], vp[3], c[3];
+        for (int d = 0; d < 3; d++) t[d] = qm * B[d] * 0.5 * dtp;
+        double t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        for (int d = 0; d < 3; d++) s[d] = 2 * t[d] / (1 + t2);
+        for (int d = 0; d < 3; d++) vm[d] = vel[d] + qm * E[d] * 0.5 * dtp;
+        c[0] = vm[1] * t[2] - vm[2] * t[1];
+        c[1] = -vm[0] * t[2] + vm[2] * t[0];
+        c[2] = vm[0] * t[1] - vm[1] * t[0];
+        for (int d = 0; d < 3; d++) vp[d] = vm[d] + c[d];
+        c[0] = vp[1] * s[2] - vp[2] * s[1];
+        c[1] = -vp[0] * s[2] + vp[2] * s[0];
+        c[2] = vp[0] * s[1] - vp[1] * s[0];
+        for (int d = 0; d < 3; d++) vel[d] = vm[d] + c[d] + qm * E[d] * 0.5 * dtp;
+    }
+    /* velocity update of ParticleMover.run and addParticle (KineticMaterial.java:336-353, 779-794):
+       the electric kick over part.dt when the gathered B is zero, Boris otherwise */
+    void update_velocity(const Species& sp, Particle& part) {
+        double E[3] = {gather(efi, part.lc[0], part.lc[1]), gather(efj, part.lc[0], part.lc[1]), 0};
+        double B[3] = {gather(bfi, part.lc[0], part.lc[1]), gather(bfj, part.lc[0], part.lc[1]), 0};
+        if (B[0] == 0 && B[1] == 0) {
+            part.vel[0] += sp.qm * E[0] * part.dt;
+            part.vel[1] += sp.qm * E[1] * part.dt;
+        } else {
+            boris(sp.qm, part.dt, E, B, part.vel);
+        }
+    }
+
     /* KineticMaterial.addParticle(MeshData, Particle) (KineticMaterial.java:759-802) */
     void add_particle(Species& sp, Particle part, bool have_lc) {
         if (!have_lc) {
@@ -690,10 +721,7 @@ struct orc_sim {
             if (part.lc[1] >= mesh.nj) part.lc[1] = mesh.nj - 1;
         }
         part.dt = -0.5 * dt;
-        double e0 = gather(efi, part.lc[0], part.lc[1]);
-        double e1 = gather(efj, part.lc[0], part.lc[1]);
-        part.vel[0] += sp.qm * e0 * part.dt;          /* B = 0 branch */
-        part.vel[1] += sp.qm * e1 * part.dt;
+        update_velocity(sp, part);
         part.dt = 0;
         part.id = sp.id_counter++;
         /* MeshData.addParticle drops non-finite velocities (KineticMaterial.java:1356-1361) */
@@ -826,10 +854,7 @@ struct orc_sim {
             alive_flag[p] = 1;
             if (part.mpw <= 0) { alive_flag[p] = 0; continue; }
             part.dt += dt;
-            double e0 = gather(efi, part.lc[0], part.lc[1]);
-            double e1 = gather(efj, part.lc[0], part.lc[1]);
-            part.vel[0] += sp.qm * e0 * part.dt;       /* B = 0 branch (no magnetic field is loaded) */
-            part.vel[1] += sp.qm * e1 * part.dt;
+            update_velocity(sp, part);
             int bounces = 0;
             bool alive = true;
             while (part.dt > 0 && bounces++ < max_bounces) {
@@ -1094,6 +1119,8 @@ struct orc_sim {
         if (n == "rho") return &rho;
         if (n == "efi") return &efi;
         if (n == "efj") return &efj;
+        if (n == "bfi") return &bfi;
+        if (n == "bfj") return &bfj;
         if (spi < 0 || spi >= (int)species.size()) return nullptr;
         return species_field(*species[spi], n);
     }
@@ -1137,6 +1164,8 @@ orc_sim* orc_create(const orc_mesh_desc* d) {
     s->phi.init(m.ni, m.nj);
     s->rho.init(m.ni, m.nj);
     s->efi.init(m.ni, m.nj);
+    s->bfi.init(m.ni, m.nj);
+    s->bfj.init(m.ni, m.nj);
     s->efj.init(m.ni, m.nj);
     return s;
 }
@@ -1303,6 +1332,16 @@ void orc_javarandom_ints_bounded(int64_t seed, int bound, int n, int* out) {
 }
 
 }  // extern "C"
+
+// BoltzmannElectronsMaterial.electronModelBoltzmann (core/materials/BoltzmannElectronsMaterial.java:
+// 200-210): den = den0 * Math.exp((phi - phi0) / kTe0), then den < 1 -> 1
+extern "C" void orc_boltzmann_density(int64_t n, const double* phi, double den0, double kTe0, double phi0,
+                                      double* out) {
+    for (int64_t u = 0; u < n; u++) {
+        double d = den0 * std::exp((phi[u] - phi0) / kTe0);
+        out[u] = d < 1 ? 1 : d;
+    }
+}
 
 extern "C" void orc_set_rng_state(orc_sim* s, int64_t state) { s->rnd.seed = (uint64_t)state & ((1ULL << 48) - 1); }
 

@@ -84,14 +84,17 @@ SIGNATURES = {
     "sf_poisson_solve": (C.c_int, [P, DP, C.POINTER(SolveInfo)]),
     "sf_phi_set": (C.c_int, [P, DP]),
     "sf_phi_get": (C.c_int, [P, DP]),
+    "sf_boltzmann_density": (C.c_int, [P, C.c_double, C.c_double, C.c_double, DP]),
     "sf_efield_update": (C.c_int, [P]),
     "sf_efield_set": (C.c_int, [P, DP, DP]),
     "sf_efield_get": (C.c_int, [P, DP, DP]),
+    "sf_bfield_set": (C.c_int, [P, DP, DP]),
     "sf_operator_apply": (C.c_int, [P, DP, DP]),
     "sf_source_maxwellian": (C.c_int, [P, C.c_int, C.c_int, DP, C.c_int, C.c_double, C.c_double, C.c_double,
                                        IP]),
     "sf_sources_sample": (C.c_int, [P, C.POINTER(I64)]),
     "sf_last_timings": (C.c_int, [P, DP]),
+    "sf_last_push_timings": (C.c_int, [P, DP]),
     "sf_set_timing": (C.c_int, [P, C.c_int]),
     "sf_cnn_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]),
     "sf_cnn_destroy": (C.c_int, [P]),

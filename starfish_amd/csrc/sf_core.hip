@@ -114,6 +114,7 @@ static void free_mesh(sf_ctx* c) {
     dfree(c->d_efi);
     dfree(c->d_efj);
     dfree(c->d_ef32);
+    dfree(c->d_bf32);
 }
 
 int sf_ctx_destroy(sf_ctx* ctx) {
@@ -452,6 +453,13 @@ int sf_last_timings(sf_ctx* ctx, double* out4) {
     SF_CHECK_CTX(ctx);
     if (!out4) return SF_ERR_ARG;
     for (int k = 0; k < 4; k++) out4[k] = ctx->last_ms[k];
+    return SF_OK;
+}
+
+int sf_last_push_timings(sf_ctx* ctx, double* out3) {
+    SF_CHECK_CTX(ctx);
+    if (!out3) return SF_ERR_ARG;
+    for (int k = 0; k < 3; k++) out3[k] = ctx->last_push_ms[k];
     return SF_OK;
 }
 

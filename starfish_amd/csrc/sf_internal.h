@@ -69,11 +69,13 @@ struct MeshD {
     const DevSeg* seg;
     const uint8_t* node_hasseg;// [ni*nj] 1 if the node lists a DIRICHLET/SINK segment
     const float2* ef;          // [ni*nj] E field (i, j components)
+    const float2* bf;          // [ni*nj] B field (i, j components); null when no field is loaded
 };
 
 // Per-species push constants.
 struct PushC {
     double qm_dt;          // q/m * dt
+    double qm;             // q/m
     double dt;
     double kscale[NCH];    // fixed-point scales 2^k per channel
     double spwt;
@@ -134,6 +136,7 @@ struct sf_ctx {
     int device = 0;
     int ncu = 0;                    // compute units of the device (persistent-grid sizing)
     int grid_inplace = 0, grid_sort = 0;   // resident workgroups of the persistent tile kernels
+    int grid_inplace_b = 0, grid_sort_b = 0;   // the same with a magnetic field
     hipStream_t stream = nullptr;
     std::string err;
     double dt = 1e-7;
@@ -159,12 +162,14 @@ struct sf_ctx {
     double* d_efi = nullptr;
     double* d_efj = nullptr;
     float2* d_ef32 = nullptr;
+    float2* d_bf32 = nullptr;   // magnetic field, FP32 like E; allocated only while nonzero
     std::vector<sf::Species*> species;
     std::vector<sf::HostSource> sources;
     sf::PoissonState* poisson = nullptr;
     bool timing = false;
     hipEvent_t ev[6]{};
     double last_ms[4] = {0, 0, 0, 0};
+    double last_push_ms[3] = {0, 0, 0};   // tile kernel ms, tail kernel ms, tail slots
     sf::MeshD meshd() const {
         sf::MeshD m;
         m.face = d_face;
@@ -173,6 +178,7 @@ struct sf_ctx {
         m.seg = d_seg;
         m.node_hasseg = d_hasseg;
         m.ef = d_ef32;
+        m.bf = d_bf32;
         return m;
     }
 };

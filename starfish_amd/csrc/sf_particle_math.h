@@ -185,14 +185,48 @@ __device__ __forceinline__ bool process_boundary(const MeshP& m, const MeshD& md
 // Inputs: logical coordinate and velocity; outputs updated in place.  Returns alive.
 // HasSeg: (i, j) -> nonzero if node (i, j) lists a DIRICHLET/SINK segment (md.node_hasseg, or a
 // staged copy of it).
-template <class Load, class HasSeg>
+// UpdateVelocityBoris (KineticMaterial.java:847-893) for E = (ex, ey, 0) and B = (bx, by, 0):
+// half electric kick, rotation by t = (q/m) B dtp/2 and s = 2t / (1 + |t|^2), half kick.
+// The z components of t and s are zero, so their cross-product terms drop out.
+__device__ __forceinline__ void boris_velocity(double qm, double dtp, double ex, double ey, double bx, double by,
+                                               double& v0, double& v1, double& v2) {
+    const double t0 = qm * bx * 0.5 * dtp, t1 = qm * by * 0.5 * dtp;
+    const double f = 2 / (1 + (t0 * t0 + t1 * t1));
+    const double s0 = f * t0, s1 = f * t1;
+    const double hx = qm * ex * 0.5 * dtp, hy = qm * ey * 0.5 * dtp;
+    const double m0 = v0 + hx, m1 = v1 + hy, m2 = v2;
+    const double p0 = m0 - m2 * t1, p1 = m1 + m2 * t0, p2 = m2 + (m0 * t1 - m1 * t0);
+    v0 = m0 - p2 * s1 + hx;
+    v1 = m1 + p2 * s0 + hy;
+    v2 = m2 + (p0 * s1 - p1 * s0);
+}
+
+// Velocity update of ParticleMover.run / addParticle (KineticMaterial.java:336-353, 779-794):
+// the electric kick over dtp where the gathered B is zero, Boris elsewhere.  HAS_B = false is
+// the path with no magnetic field loaded anywhere.
+template <bool HAS_B, class Load>
+__device__ __forceinline__ void kick(const MeshP& m, const MeshD& md, double qm, double qm_dtp, double dtp, Load load,
+                                     double lc0, double lc1, double& v0, double& v1, double& v2) {
+    double ex, ey;
+    gather_e(m, lc0, lc1, load, ex, ey);
+    if (HAS_B) {
+        double bx, by;
+        auto load_b = [&](int i, int j) -> float2 { return md.bf[(size_t)j * m.ni + i]; };
+        gather_e(m, lc0, lc1, load_b, bx, by);
+        if (bx != 0 || by != 0) {
+            boris_velocity(qm, dtp, ex, ey, bx, by, v0, v1, v2);
+            return;
+        }
+    }
+    v0 += qm_dtp * ex;
+    v1 += qm_dtp * ey;
+}
+
+template <bool HAS_B = false, class Load, class HasSeg>
 __device__ __forceinline__ bool push_particle(const MeshP& m, const MeshD& md, const PushC& pc, Load load,
                                               HasSeg hasseg, double& lc0, double& lc1, double& v0, double& v1,
                                               double& v2) {
-    double ex, ey;
-    gather_e(m, lc0, lc1, load, ex, ey);
-    v0 += pc.qm_dt * ex;
-    v1 += pc.qm_dt * ey;
+    kick<HAS_B>(m, md, pc.qm, pc.qm_dt, pc.dt, load, lc0, lc1, v0, v1, v2);
     // Fast path: a single sub-move that stays inside the mesh and whose bounding box of nodes
     // lists no DIRICHLET/SINK segment.  ProcessBoundary then returns alive with no time left,
     // so this is the reference loop's first and only iteration.  Anything else takes the loop.

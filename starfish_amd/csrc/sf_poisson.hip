@@ -886,6 +886,42 @@ int sf_phi_get(sf_ctx* ctx, double* phi) {
     return SF_OK;
 }
 
+// BoltzmannElectronsMaterial.electronModelBoltzmann (BoltzmannElectronsMaterial.java:179-215):
+// n_e = den0 exp((phi - phi0) / kTe0) on every node, floored at 1 m^-3
+__global__ void boltzmann_density_kernel(int64_t nn, const double* __restrict__ phi, double den0, double inv_kte,
+                                         double phi0, double* __restrict__ den) {
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        double v = den0 * exp((phi[u] - phi0) * inv_kte);
+        den[u] = v < 1.0 ? 1.0 : v;
+    }
+}
+
+int sf_boltzmann_density(sf_ctx* ctx, double den0, double kTe0, double phi0, double* den) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready || !den) return sf_fail(ctx, SF_ERR_ARG, "sf_boltzmann_density: bad state or argument");
+    // values not given (<= 0) come from the potential solver, as the reference does
+    if (den0 <= 0 || kTe0 <= 0) {
+        if (!ctx->poisson)
+            return sf_fail(ctx, SF_ERR_STATE, "sf_boltzmann_density: den0/kTe0 not given and no solver set up");
+        den0 = ctx->poisson->desc.n0;
+        kTe0 = ctx->poisson->desc.Te0;
+        phi0 = ctx->poisson->desc.phi0;
+    }
+    if (!(kTe0 > 0)) return sf_fail(ctx, SF_ERR_ARG, "sf_boltzmann_density: non zero temperature kTe0 is needed");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t nn = (int64_t)ctx->mp.ni * ctx->mp.nj;
+    double* d_den = nullptr;
+    SF_HIP(ctx, hipMalloc((void**)&d_den, nn * 8));
+    hipLaunchKernelGGL(boltzmann_density_kernel, dim3(grid_elem(nn)), dim3(256), 0, ctx->stream, nn,
+                       (const double*)ctx->d_phi, den0, 1.0 / kTe0, phi0, d_den);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(den, d_den, nn * 8, hipMemcpyDeviceToHost);
+    hipFree(d_den);
+    if (e != hipSuccess) return sf_fail(ctx, SF_ERR_HIP, "sf_boltzmann_density: %s", hipGetErrorString(e));
+    return SF_OK;
+}
+
 int sf_efield_update(sf_ctx* ctx) {
     SF_CHECK_CTX(ctx);
     if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");

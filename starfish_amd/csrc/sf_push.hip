@@ -154,7 +154,7 @@ struct PReg {
 // thread loads its particle straight into registers, one window ahead.  Results go back through
 // LDS by slot and are stored coalesced: stores in the permuted order write partial cache lines
 // and run at about half the bandwidth of coalesced ones on this part.
-template <bool INPLACE>
+template <bool INPLACE, bool HAS_B>
 __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const MeshD& md, const PushC& pc,
                                                        const PState& s, const int32_t* __restrict__ offsets,
                                                        int32_t* __restrict__ keys,
@@ -247,7 +247,7 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
             double lc0 = cell_i(cur.c) + (double)cur.fx;
             double lc1 = cell_j(cur.c) + (double)cur.fy;
             double v0 = cur.vx, v1 = cur.vy, v2 = cur.vz;
-            bool alive = push_particle(m, md, pc, load_e, hasseg, lc0, lc1, v0, v1, v2);
+            bool alive = push_particle<HAS_B>(m, md, pc, load_e, hasseg, lc0, lc1, v0, v1, v2);
             if (alive) {
                 int ci, cj;
                 new_cell(m, lc0, lc1, ci, cj);
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile_inplace(Mes
                                                                            unsigned long long* __restrict__ acc,
                                                                            int32_t* __restrict__ counts,
                                                                            long long* stats) {
-    push_deposit_tile_body<true>(m, md, pc, s, offsets, nullptr, acc, counts, stats);
+    push_deposit_tile_body<true, false>(m, md, pc, s, offsets, nullptr, acc, counts, stats);
 }
 // Sort step (pass A): the key write needs a few more registers; no register cap, so no spills.
 __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile_sort(MeshP m, MeshD md, PushC pc, PState s,
@@ -355,13 +355,28 @@ __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile_sort(MeshP m, 
                                                                       int32_t* __restrict__ keys,
                                                                       unsigned long long* __restrict__ acc,
                                                                       int32_t* __restrict__ counts, long long* stats) {
-    push_deposit_tile_body<false>(m, md, pc, s, offsets, keys, acc, counts, stats);
+    push_deposit_tile_body<false, false>(m, md, pc, s, offsets, keys, acc, counts, stats);
+}
+// The same passes with a magnetic field loaded (Boris where the gathered B is nonzero).
+__global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile_inplace_b(MeshP m, MeshD md, PushC pc, PState s,
+                                                                          const int32_t* __restrict__ offsets,
+                                                                          unsigned long long* __restrict__ acc,
+                                                                          int32_t* __restrict__ counts,
+                                                                          long long* stats) {
+    push_deposit_tile_body<true, true>(m, md, pc, s, offsets, nullptr, acc, counts, stats);
+}
+__global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile_sort_b(MeshP m, MeshD md, PushC pc, PState s,
+                                                                        const int32_t* __restrict__ offsets,
+                                                                        int32_t* __restrict__ keys,
+                                                                        unsigned long long* __restrict__ acc,
+                                                                        int32_t* __restrict__ counts, long long* stats) {
+    push_deposit_tile_body<false, true>(m, md, pc, s, offsets, keys, acc, counts, stats);
 }
 
 // ----------------------------------------------------------------------------------------
 // pass A, unsorted tail (particles added since the last sort): global atomics
 // ----------------------------------------------------------------------------------------
-template <bool INPLACE>
+template <bool INPLACE, bool HAS_B>
 __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, PushC pc, PState s, int64_t p0,
                                                          int64_t p1, int32_t* __restrict__ keys,
                                                          unsigned long long* __restrict__ acc,
@@ -377,7 +392,7 @@ __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, Push
             continue;
         }
         PIn q = load_particle(s, p);
-        bool alive = push_particle(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
+        bool alive = push_particle<HAS_B>(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
         int32_t key = -1;
         if (alive) {
             int ci, cj;
@@ -526,7 +541,7 @@ __device__ __forceinline__ int64_t wave_rank(bool valid, int key, int32_t* curso
     return dst;
 }
 
-template <bool IDS>
+template <bool IDS, bool HAS_B>
 __global__ void __launch_bounds__(256) push_scatter(MeshP m, MeshD md, PushC pc, PState s, PState d, int64_t np,
                                                     const int32_t* __restrict__ keys, int32_t* __restrict__ cursor) {
     auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
@@ -542,7 +557,7 @@ __global__ void __launch_bounds__(256) push_scatter(MeshP m, MeshD md, PushC pc,
         int32_t idv = 0;
         if (valid) {
             PIn q = load_particle(s, p);
-            push_particle(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
+            push_particle<HAS_B>(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
             ci = cell_i(key);
             cj = cell_j(key);
             lkey = cj * m.nci + ci;
@@ -642,7 +657,8 @@ __global__ void __launch_bounds__(256) normalize_nodes(int64_t nn, unsigned long
 // particle addition (KineticMaterial.addParticle) and device preload
 // ----------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) add_particles_kernel(MeshP m, MeshD md, PState d, int64_t base, int64_t n,
-                                                            const double* __restrict__ in, double qm_half_dt,
+                                                            const double* __restrict__ in, double qm,
+                                                            double qm_half_dt, double half_dt,
                                                             int rewind, const int32_t* __restrict__ ids,
                                                             int id0, int with_ids) {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
@@ -652,11 +668,10 @@ __global__ void __launch_bounds__(256) add_particles_kernel(MeshP m, MeshD md, P
         if (rewind) {
             if (lc0 >= m.ni) lc0 = m.ni - 1;
             if (lc1 >= m.nj) lc1 = m.nj - 1;
-            double ex, ey;
+            // part.dt = -dt/2 (KineticMaterial.java:777-794)
             auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
-            gather_e(m, lc0, lc1, load_e, ex, ey);
-            u += qm_half_dt * ex;        // part.dt = -dt/2 (KineticMaterial.java:777-791)
-            v += qm_half_dt * ey;
+            if (md.bf) kick<true>(m, md, qm, qm_half_dt, half_dt, load_e, lc0, lc1, u, v, w);
+            else kick<false>(m, md, qm, qm_half_dt, half_dt, load_e, lc0, lc1, u, v, w);
         }
         int ci = (int)lc0, cj = (int)lc1;
         ci = min(max(ci, 0), m.nci - 1);
@@ -742,6 +757,7 @@ static int grid_for(int64_t n, int threads, int max_blocks = 65536) {
 static PushC make_pushc(sf_ctx* ctx, Species* sp) {
     PushC pc;
     pc.qm_dt = sp->qm * ctx->dt;
+    pc.qm = sp->qm;
     pc.dt = ctx->dt;
     for (int c = 0; c < NCH; c++) pc.kscale[c] = sp->kscale[c];
     pc.spwt = sp->desc.spwt;
@@ -796,14 +812,13 @@ int sort_species(sf_ctx* ctx, Species* sp) {
 
 // persistent grid of a tile kernel: the workgroups that are resident at once (occupancy x CUs),
 // at most one per tile
-static int tile_grid(sf_ctx* ctx, bool inplace) {
-    int& g = inplace ? ctx->grid_inplace : ctx->grid_sort;
+static int tile_grid(sf_ctx* ctx, bool inplace, bool has_b) {
+    int& g = has_b ? (inplace ? ctx->grid_inplace_b : ctx->grid_sort_b) : (inplace ? ctx->grid_inplace : ctx->grid_sort);
     if (g == 0) {
         int per_cu = 0;
-        hipError_t e = inplace ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, push_deposit_tile_inplace,
-                                                                                PUSH_THREADS, 0)
-                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, push_deposit_tile_sort,
-                                                                                PUSH_THREADS, 0);
+        const void* k = has_b ? (inplace ? (const void*)push_deposit_tile_inplace_b : (const void*)push_deposit_tile_sort_b)
+                              : (inplace ? (const void*)push_deposit_tile_inplace : (const void*)push_deposit_tile_sort);
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PUSH_THREADS, 0);
         if (e != hipSuccess || per_cu <= 0) per_cu = 2;
         g = per_cu * std::max(1, ctx->ncu);
     }
@@ -822,24 +837,48 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     // sort_interval-1 cells from their tile, inside the LDS halo for sub-cell steps)
     const bool sort_step = (sp->since_sort + 1 >= sp->sort_interval) || sp->nsorted == 0 ||
                            (sp->np - sp->nsorted) > sp->np / 8;
+    const int64_t np_before = sp->np, nsorted_before = sp->nsorted;
     SF_HIP(ctx, hipMemsetAsync(sp->stats, 0, 8 * sizeof(long long), ctx->stream));
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    const bool hb = md.bf != nullptr;
+    const dim3 tg(tile_grid(ctx, !sort_step, hb)), tb(PUSH_THREADS);
+    const dim3 g_tail(grid_for(sp->np - sp->nsorted, 256, 4096)), b256(256);
     if (sort_step) {
-        if (sp->nsorted > 0)
-            hipLaunchKernelGGL(push_deposit_tile_sort, dim3(tile_grid(ctx, false)), dim3(PUSH_THREADS), 0, ctx->stream, m, md,
-                               pc, s, sp->offsets, sp->key, sp->acc, sp->counts, sp->stats);
-        if (sp->np > sp->nsorted)
-            hipLaunchKernelGGL(push_deposit_tail<false>, dim3(grid_for(sp->np - sp->nsorted, 256, 4096)), dim3(256), 0,
-                               ctx->stream, m, md, pc, s, sp->nsorted, sp->np, sp->key, sp->acc, sp->counts,
-                               sp->stats);
+        if (sp->nsorted > 0) {
+            if (hb)
+                hipLaunchKernelGGL(push_deposit_tile_sort_b, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets, sp->key,
+                                   sp->acc, sp->counts, sp->stats);
+            else
+                hipLaunchKernelGGL(push_deposit_tile_sort, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets, sp->key,
+                                   sp->acc, sp->counts, sp->stats);
+        }
+        if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+        if (sp->np > sp->nsorted) {
+            if (hb)
+                hipLaunchKernelGGL((push_deposit_tail<false, true>), g_tail, b256, 0, ctx->stream, m, md, pc, s,
+                                   sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
+            else
+                hipLaunchKernelGGL((push_deposit_tail<false, false>), g_tail, b256, 0, ctx->stream, m, md, pc, s,
+                                   sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
+        }
     } else {
-        if (sp->nsorted > 0)
-            hipLaunchKernelGGL(push_deposit_tile_inplace, dim3(tile_grid(ctx, true)), dim3(PUSH_THREADS), 0, ctx->stream, m,
-                               md, pc, s, sp->offsets, sp->acc, sp->counts, sp->stats);
-        if (sp->np > sp->nsorted)
-            hipLaunchKernelGGL(push_deposit_tail<true>, dim3(grid_for(sp->np - sp->nsorted, 256, 4096)), dim3(256), 0,
-                               ctx->stream, m, md, pc, s, sp->nsorted, sp->np, sp->key, sp->acc, sp->counts,
-                               sp->stats);
+        if (sp->nsorted > 0) {
+            if (hb)
+                hipLaunchKernelGGL(push_deposit_tile_inplace_b, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets,
+                                   sp->acc, sp->counts, sp->stats);
+            else
+                hipLaunchKernelGGL(push_deposit_tile_inplace, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets,
+                                   sp->acc, sp->counts, sp->stats);
+        }
+        if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+        if (sp->np > sp->nsorted) {
+            if (hb)
+                hipLaunchKernelGGL((push_deposit_tail<true, true>), g_tail, b256, 0, ctx->stream, m, md, pc, s,
+                                   sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
+            else
+                hipLaunchKernelGGL((push_deposit_tail<true, false>), g_tail, b256, 0, ctx->stream, m, md, pc, s,
+                                   sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
+        }
     }
     SF_HIP(ctx, hipGetLastError());
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
@@ -852,12 +891,22 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     }
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     if (sort_step && sp->np > 0) {
-        if (sp->desc.track_ids)
-            hipLaunchKernelGGL(push_scatter<true>, dim3(grid_for(sp->np, 256, 1 << 20)), dim3(256), 0, ctx->stream, m,
-                               md, pc, s, d, sp->np, sp->key, sp->cursor);
-        else
-            hipLaunchKernelGGL(push_scatter<false>, dim3(grid_for(sp->np, 256, 1 << 20)), dim3(256), 0, ctx->stream, m,
-                               md, pc, s, d, sp->np, sp->key, sp->cursor);
+        const dim3 gs(grid_for(sp->np, 256, 1 << 20));
+        if (sp->desc.track_ids) {
+            if (hb)
+                hipLaunchKernelGGL((push_scatter<true, true>), gs, b256, 0, ctx->stream, m, md, pc, s, d, sp->np, sp->key,
+                                   sp->cursor);
+            else
+                hipLaunchKernelGGL((push_scatter<true, false>), gs, b256, 0, ctx->stream, m, md, pc, s, d, sp->np,
+                                   sp->key, sp->cursor);
+        } else {
+            if (hb)
+                hipLaunchKernelGGL((push_scatter<false, true>), gs, b256, 0, ctx->stream, m, md, pc, s, d, sp->np,
+                                   sp->key, sp->cursor);
+            else
+                hipLaunchKernelGGL((push_scatter<false, false>), gs, b256, 0, ctx->stream, m, md, pc, s, d, sp->np,
+                                   sp->key, sp->cursor);
+        }
         SF_HIP(ctx, hipGetLastError());
     }
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
@@ -881,8 +930,12 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     sp->stats_host[3] = (double)st[3] * inv_kv * sp->desc.spwt * sp->mass;
     sp->stats_host[4] = (double)st[4] * inv_kv * sp->desc.spwt * sp->mass;
     if (ctx->timing) {
-        float a, b, c, e;
+        float a, b, c, e, t0 = 0.f;
         hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+        hipEventElapsedTime(&t0, ctx->ev[0], ctx->ev[5]);
+        ctx->last_push_ms[0] = t0;
+        ctx->last_push_ms[1] = a - t0;
+        ctx->last_push_ms[2] = (double)(np_before - nsorted_before);
         hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
         hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
         hipEventElapsedTime(&e, ctx->ev[3], ctx->ev[4]);
@@ -948,8 +1001,8 @@ int add_particles_device(sf_ctx* ctx, Species* sp, int64_t n, const double* x, c
         SF_HIP(ctx, hipMemcpyAsync(d_ids, hid.data(), kept * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     }
     hipLaunchKernelGGL(add_particles_kernel, dim3(grid_for(kept, 256, 4096)), dim3(256), 0, ctx->stream, ctx->mp,
-                       ctx->meshd(), sp->buf[sp->cur], sp->np, kept, (const double*)d_in, -0.5 * ctx->dt * sp->qm,
-                       mode, (const int32_t*)d_ids, sp->id_counter, sp->desc.track_ids);
+                       ctx->meshd(), sp->buf[sp->cur], sp->np, kept, (const double*)d_in, sp->qm,
+                       -0.5 * ctx->dt * sp->qm, -0.5 * ctx->dt, mode, (const int32_t*)d_ids, sp->id_counter, sp->desc.track_ids);
     SF_HIP(ctx, hipGetLastError());
     SF_HIP(ctx, hipFreeAsync(d_in, ctx->stream));
     if (d_ids) SF_HIP(ctx, hipFreeAsync(d_ids, ctx->stream));
@@ -1009,4 +1062,39 @@ int ef32_refresh(sf_ctx* ctx) {
     return SF_OK;
 }
 
+// B field (DomainModule "bfi"/"bfj", DomainModule.java:174) stored as FP32 pairs like E.  A field
+// that is zero on every node frees the buffer: the mover then takes the electric kick everywhere,
+// which is what the reference does where the gathered B is zero.
+int bfield_set(sf_ctx* ctx, const double* bfi, const double* bfj) {
+    const int64_t nn = (int64_t)ctx->mp.ni * ctx->mp.nj;
+    bool any = false;
+    std::vector<float2> h;
+    if (bfi && bfj) {
+        h.resize(nn);
+        for (int64_t u = 0; u < nn; u++) {
+            if (!std::isfinite(bfi[u]) || !std::isfinite(bfj[u]))
+                return sf_fail(ctx, SF_ERR_ARG, "sf_bfield_set: non-finite value at node %lld", (long long)u);
+            h[u] = make_float2((float)bfi[u], (float)bfj[u]);
+            any = any || h[u].x != 0.0f || h[u].y != 0.0f;
+        }
+    }
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!any) {
+        if (ctx->d_bf32) SF_HIP(ctx, hipFree(ctx->d_bf32));
+        ctx->d_bf32 = nullptr;
+        return SF_OK;
+    }
+    if (!ctx->d_bf32) SF_HIP(ctx, hipMalloc((void**)&ctx->d_bf32, nn * sizeof(float2)));
+    SF_HIP(ctx, hipMemcpy(ctx->d_bf32, h.data(), nn * sizeof(float2), hipMemcpyHostToDevice));
+    return SF_OK;
+}
+
 }  // namespace sf
+
+extern "C" int sf_bfield_set(sf_ctx* ctx, const double* bfi, const double* bfj) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "sf_bfield_set: mesh not set up");
+    if ((bfi == nullptr) != (bfj == nullptr)) return sf_fail(ctx, SF_ERR_ARG, "sf_bfield_set: give both components or neither");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    return sf::bfield_set(ctx, bfi, bfj);
+}

@@ -184,6 +184,38 @@ class PoissonHipSolver:
         self.sim.ctx.call("sf_efield_update")
 
 
+class BoltzmannElectronsHip:
+    """`<material type="boltzmann_electrons_hip">`: BoltzmannElectronsMaterial with the BOLTZMANN
+    model (core/materials/BoltzmannElectronsMaterial.java:30-60, 179-215).  The density is
+    n0 exp((phi - phi0)/kTe0), floored at 1 m^-3, from the device-resident potential; n0/kTe0 <= 0
+    take n0, kTe0 and phi0 from the potential solver.  Like the reference it is not part of rho
+    (SolverModule.java:157).  The reference recomputes it every step for output; here it is
+    computed when read."""
+
+    def __init__(self, sim, spec, capacity=0, track_ids=False, vmax=0.0, n0=-1.0, kTe0=-1.0, phi0=0.0):
+        self.sim = sim
+        self.name = spec.name
+        self.spec = spec
+        self.den0, self.kTe0, self.phi0 = n0, kTe0, phi0
+
+    def update(self):
+        pass
+
+    def getNp(self):
+        return 0
+
+    def field(self, name):
+        if name == "nd":
+            out = np.zeros(self.sim.nn)
+            self.sim.ctx.call("sf_boltzmann_density", self.den0, self.kTe0, self.phi0, _dp(out))
+            return out
+        if name == "t":
+            from .deck import K_B, QE
+            kte = self.kTe0 if (self.den0 > 0 and self.kTe0 > 0) else self.sim.deck.solver.Te0
+            return np.full(self.sim.nn, kte * (QE / K_B))    # kTe0 * Constants.EVtoK
+        raise KeyError(name)
+
+
 class Registry:
     """Type-name registries of MaterialsModule and SolverModule."""
 
@@ -203,6 +235,7 @@ class HipPlugin:
 
     def register(self, registry: Registry):
         registry.registerMaterialType("KINETIC_HIP", KineticHipMaterial)
+        registry.registerMaterialType("BOLTZMANN_ELECTRONS_HIP", BoltzmannElectronsHip)
         registry.registerSolver("POISSON_HIP", PoissonHipSolver)
 
 
@@ -295,6 +328,14 @@ class Simulation:
     def set_efield(self, ei, ej):
         self.ctx.call("sf_efield_set", _dp(np.ascontiguousarray(ei, np.float64)),
                       _dp(np.ascontiguousarray(ej, np.float64)))
+
+    def set_bfield(self, bi, bj):
+        """Loads the domain's magnetic field (DomainModule "bfi"/"bfj"); None removes it."""
+        if bi is None:
+            self.ctx.call("sf_bfield_set", None, None)
+            return
+        self.ctx.call("sf_bfield_set", _dp(np.ascontiguousarray(bi, np.float64)),
+                      _dp(np.ascontiguousarray(bj, np.float64)))
 
     def close(self):
         self.ctx.close()

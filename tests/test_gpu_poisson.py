@@ -120,3 +120,31 @@ def test_potential_matches_tight_reference_solution(which):
         assert err < 1e-6
     finally:
         sim.close()
+
+
+@pytest.mark.gpu
+def test_boltzmann_electron_density_matches_oracle():
+    """The electron density of the BOLTZMANN model from the device potential equals the oracle's
+    on the same phi (the exponential may differ by an ulp), given explicitly and taken from the
+    solver's n0/kTe0/phi0; the electron temperature is kTe0 in kelvin."""
+    import oracle as O
+    from starfish_amd.plugin import BoltzmannElectronsHip, Simulation
+    deck = D.config_c1(steps=1)
+    sim = Simulation(deck, capacity=10000)
+    try:
+        sim.initial_solve()
+        phi = sim.phi()
+        e_solver = BoltzmannElectronsHip(sim, D.MaterialSpec("e-", "boltzmann_electrons_hip"))
+        e_given = BoltzmannElectronsHip(sim, D.MaterialSpec("e-", "boltzmann_electrons_hip"), n0=3e9, kTe0=1.5,
+                                        phi0=-2.0)
+        s = deck.solver
+        for e, (n0, kte, phi0) in ((e_solver, (s.n0, s.Te0, s.phi0)), (e_given, (3e9, 1.5, -2.0))):
+            nd = e.field("nd")
+            ref = O.boltzmann_density(phi, n0, kte, phi0)
+            print("nd range %.3e..%.3e, max rel diff %.2e" % (nd.min(), nd.max(), np.max(np.abs(nd - ref) / ref)),
+                  flush=True)
+            assert np.all(np.abs(nd - ref) <= 1e-14 * ref)
+            assert np.all(nd >= 1.0)
+        assert np.allclose(e_given.field("t"), 1.5 * QE / 1.3806503e-23, rtol=1e-15)
+    finally:
+        sim.close()

@@ -135,3 +135,59 @@ def test_deposit_is_bit_identical_run_to_run():
             sim.close()
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("interval", [1, 4])
+def test_boris_push_matches_oracle(interval):
+    """A magnetic field of up to 0.3 T that is zero on the left third of the domain: the
+    addParticle rewind and three pushes use Boris where the gathered B is nonzero and the
+    electric kick elsewhere (UpdateVelocityBoris, KineticMaterial.java:336-353, 779-794, 847-893).
+    The GPU keeps B in FP32, so the oracle gets the same FP32-rounded values; tolerance 1e-5."""
+    from starfish_amd.plugin import Simulation
+    n = 20000
+    deck = D.config_c1()
+    geo, st = geometry(deck)
+    m = deck.mesh
+    o = oracle_for(deck, geo, st)
+    o.initial_solve()
+    efi, efj = o.field("efi"), o.field("efj")
+    I, J = np.meshgrid(np.arange(m.ni), np.arange(m.nj))
+    on = (I > m.ni // 3).ravel()
+    bfi = (0.3 * np.cos(np.pi * I / (m.ni - 1))).ravel() * on
+    bfj = (0.2 * np.sin(np.pi * J / (m.nj - 1))).ravel() * on
+    bfi, bfj = bfi.astype(np.float32).astype(np.float64), bfj.astype(np.float32).astype(np.float64)
+    o.set_field("bfi", bfi)
+    o.set_field("bfj", bfj)
+    x, y, v, _, _ = open_cell_particles(deck, geo, n, seed=21)
+    ids = np.arange(n, dtype=np.int32)
+    o.load_particles(0, np.stack([x, y, np.zeros(n)], 1), v, ids=ids, rewind=True)
+    sim = Simulation(deck, capacity=4 * n, track_ids=True)
+    try:
+        sim.set_efield(efi, efj)
+        sim.set_bfield(bfi, bfj)
+        mat = sim.materials[0]
+        sim.ctx.call("sf_species_set_sort_interval", mat.index, interval)
+        mat.add_particles(x, y, v[:, 0], v[:, 1], v[:, 2], ids=ids, rewind=True)
+        mism, epos, evel = _compare_particles(deck, mat.particles(ids=True), o.particles(0))
+        print("after the rewind: velocity error %.2e" % evel, flush=True)
+        assert mism == 0 and evel < TOL
+        for step in range(3):
+            o.move()
+            o.update_species_fields()
+            mat.update()
+            gp, op = mat.particles(ids=True), o.particles(0)
+            mism, epos, evel = _compare_particles(deck, gp, op)
+            print("step %d: np %d / %d, position error %.2e, velocity error %.2e" %
+                  (step, len(gp["id"]), len(op["id"]), epos, evel), flush=True)
+            assert mism <= 2 and epos < TOL and evel < TOL
+            for name in ("nd", "u", "v", "w", "ww-sum"):
+                e = rel_err(mat.field(name), o.field(name, 0))
+                assert e < TOL, "%s error %g" % (name, e)
+        # Boris turned w for the particles in the field (the electric kick never changes w in XY)
+        w0 = v[op["id"], 2]
+        assert np.mean(op["vel"][:, 2] != w0) > 0.3
+        sim.set_bfield(None, None)
+        mat.update()          # the same species continues on the electric-only path
+    finally:
+        sim.close()

@@ -344,3 +344,85 @@ def test_maxwellian_source_rate_and_velocity_statistics():
         assert abs(v[:, c].std() - sigma) < 5 * sigma / math.sqrt(2 * n)
     y = p["pos"][:, 1] - v[:, 1] * 1e-7 * (1 + 1e-6)
     assert abs(y.mean() - 0.05) < 5 * 0.06 / math.sqrt(12 * n)
+
+
+def test_boltzmann_density_closed_forms():
+    """n = n0 exp((phi - phi0)/kTe) with the floor of 1 m^-3: phi = phi0 gives n0, one kTe ln 2
+    above gives 2 n0 (a swapped phi0 sign or a missing 1/kTe fails), deep negative phi hits the
+    floor, and the floor does not clip values just above 1."""
+    import oracle as O
+    n0, kte, phi0 = 1e10, 2.0, 0.5
+    phi = np.array([phi0, phi0 + kte * math.log(2.0), phi0 - kte * math.log(4.0), -100.0,
+                    phi0 - kte * math.log(n0 / 1.5)])
+    out = O.boltzmann_density(phi, n0, kte, phi0)
+    assert out[0] == n0
+    assert abs(out[1] - 2 * n0) <= 1e-15 * 2 * n0
+    assert abs(out[2] - n0 / 4) <= 1e-15 * n0
+    assert out[3] == 1.0
+    assert abs(out[4] - 1.5) < 1e-9
+
+
+# ---------------------------------------------------------------------------------------------
+# UpdateVelocityBoris (KineticMaterial.java:847-893)
+# ---------------------------------------------------------------------------------------------
+def _boris_case(E, B, vel, rewind=False, n=1):
+    deck = box_deck(ni=21, nj=11)
+    geo, _ = geometry(deck)
+    o = oracle_for(deck, geo)
+    nn = deck.mesh.ni * deck.mesh.nj
+    o.set_field("efi", np.full(nn, E[0]))
+    o.set_field("efj", np.full(nn, E[1]))
+    o.set_field("bfi", np.full(nn, B[0]))
+    o.set_field("bfj", np.full(nn, B[1]))
+    pos = np.tile([0.1, 0.05, 0.0], (n, 1))
+    vel = np.atleast_2d(np.asarray(vel, float))
+    o.load_particles(0, pos, vel, ids=np.arange(n, dtype=np.int32), rewind=rewind)
+    qm = QE / (16.0 * AMU)
+    return deck, o, qm
+
+
+@pytest.mark.parametrize("rewind", [False, True])
+def test_boris_satisfies_the_implicit_midpoint_rule(rewind):
+    """Boris is exactly the implicit midpoint rule (v1 - v0)/dt = (q/m)(E + (v0 + v1)/2 x B) with
+    B = (Bi, Bj, 0) and E = (Ei, Ej, 0).  A wrong cross-product sign, a swapped component or a
+    missing half kick violates it by O(1); the addParticle rewind is the same map over -dt/2."""
+    E, B = np.array([300.0, -1200.0, 0.0]), np.array([0.07, -0.11, 0.0])
+    v0 = np.array([5000.0, -800.0, 1500.0])
+    deck, o, qm = _boris_case(E, B, v0, rewind=rewind)
+    if rewind:
+        v1 = o.particles(0)["vel"][0]
+        dt = -0.5 * deck.dt
+    else:
+        o.move()
+        v1 = o.particles(0)["vel"][0]
+        dt = deck.dt
+    lhs = (v1 - v0) / dt
+    rhs = qm * (E + np.cross(0.5 * (v0 + v1), B))
+    assert np.abs(lhs - rhs).max() <= 1e-9 * np.abs(rhs).max()
+    assert np.abs(v1 - v0).max() > 1.0
+
+
+def test_boris_rotation_angle_in_a_pure_magnetic_field():
+    """E = 0, B along i: v_i is unchanged, |v| is conserved and (v_j, v_k) turns by
+    -2 atan(qB dt / 2m), the Boris phase of the gyration dv/dt = (q/m) v x B."""
+    Bx = 0.4
+    v0 = np.array([1200.0, 3000.0, -500.0])
+    deck, o, qm = _boris_case([0, 0, 0], [Bx, 0, 0], v0)
+    o.move()
+    v1 = o.particles(0)["vel"][0]
+    th = -2 * math.atan(qm * Bx * deck.dt / 2)
+    c, s = math.cos(th), math.sin(th)
+    exp_v = np.array([v0[0], c * v0[1] - s * v0[2], s * v0[1] + c * v0[2]])
+    assert np.allclose(v1, exp_v, rtol=0, atol=1e-9 * np.linalg.norm(v0))
+    assert abs(np.linalg.norm(v1) - np.linalg.norm(v0)) <= 1e-12 * np.linalg.norm(v0)
+
+
+def test_zero_magnetic_field_takes_the_electric_branch():
+    """Where the gathered B is exactly zero the mover uses v += (q/m) E dt, not Boris (which would
+    give the same velocity only up to rounding): the result is bit-identical to that formula."""
+    E = np.array([250.0, -40.0, 0.0])
+    v0 = np.array([100.0, 200.0, 300.0])
+    deck, o, qm = _boris_case(E, [0, 0, 0], v0)
+    o.move()
+    v1 = o.particles(0)["vel"][0]
+    assert v1[0] == v0[0] + qm * E[0] * deck.dt and v1[1] == v0[1] + qm * E[1] * deck.dt and v1[2] == v0[2]
