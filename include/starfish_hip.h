@@ -262,6 +262,45 @@ int sf_last_push_timings(sf_ctx* ctx, double* out3);
 int sf_set_timing(sf_ctx* ctx, int enabled);
 
 /* ------------------------------------------------------------------------------------------
+ * Monte Carlo collisions with a background material: MCC (collisions/MCC.java:45-221) with a
+ * constant cross-section (interactions/Sigma.java:108-113, <sigma>const</sigma>) and the CEX
+ * (MCC.java:287-297) or MEX/elastic (MCC.java:242-285) model; the IONIZATION model creates
+ * particles and is not provided.  The source species must track ids: Starfish.rnd() is replaced
+ * by Philox4x32-10 keyed by seed and counted by (particle id, step, tag), so results do not depend
+ * on the order particles are visited in.  The target material enters through its node fields
+ * (density, u-ave, v-ave, w-ave, temperature; Material.sampleMaxwellianVelocity,
+ * core/materials/Material.java:859-896), node order u = j*ni + i.
+ * ------------------------------------------------------------------------------------------ */
+#define SF_MCC_CEX 0
+#define SF_MCC_MEX 1
+typedef struct {
+    int source_species;      /* <source> (kinetic) */
+    int model;               /* SF_MCC_CEX or SF_MCC_MEX (<model>) */
+    double sigma;            /* constant cross-section, m^2 (<sigma_coeffs> of <sigma>const</sigma>) */
+    double target_molwt;     /* target material's molecular weight, amu */
+    int frequency;           /* <frequency>: steps between passes (>= 1) */
+    int stop_it;             /* <stop_it>: no passes after this step (<= 0: none) */
+    double max_target_temp;  /* <max_target_temp> (<= 0: no limit) */
+    uint64_t seed;           /* Philox key */
+    uint32_t tag;            /* distinguishes the streams of several interactions */
+} sf_mcc_desc;
+typedef struct sf_mcc sf_mcc;
+int sf_mcc_create(sf_ctx* ctx, const sf_mcc_desc* desc, sf_mcc** out);
+int sf_mcc_destroy(sf_mcc* mcc);
+/* target node fields; u, v, w, temp may be NULL (zero) */
+int sf_mcc_set_target(sf_mcc* mcc, const double* den, const double* u, const double* v, const double* w,
+                      const double* temp);
+/* MCC.perform (MCC.java:130-221) at step `it` and simulation time `time`: skipped after stop_it
+   and unless it % frequency == 0; velocities of collided particles are replaced in place */
+int sf_mcc_perform(sf_mcc* mcc, int it, double time, int64_t* collisions);
+/* which: 0 col-count-sum, 1 col-real-sum, 2 col-count (per pass), 3 nu (real sum per second per
+   node volume, MCC.java:147-160), 4 the target's energy source S (W/m^3, summed since the last
+   clear; E = 0.5 m |v| as MCC.java:205-214 writes it) */
+int sf_mcc_field(sf_mcc* mcc, int which, double* out);
+/* MCC.clearSamples (MCC.java:120-124); clear_source also zeroes S */
+int sf_mcc_clear_samples(sf_mcc* mcc, int clear_source);
+
+/* ------------------------------------------------------------------------------------------
  * Learned initial guess for the potential solve.  No counterpart in the reference: the
  * reference solvers start from the previous potential (LinearSolverPCG.java:33-37,
  * LinearSolverGS.java:41-53).  A 3-level encoder-decoder CNN (32/64/128 channels, 3x3

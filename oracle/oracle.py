@@ -69,6 +69,9 @@ def load():
         "orc_set_rng_state": (None, [P, C.c_int64]),
         "orc_round_state_fp32": (None, [P, C.c_int]),
         "orc_boltzmann_density": (None, [C.c_int64, DP, C.c_double, C.c_double, C.c_double, DP]),
+        "orc_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "orc_mcc_perform": (C.c_int64, [P, C.c_int, DP, DP, DP, DP, DP, C.c_double, C.c_double, C.c_int, C.c_double,
+                                        C.c_uint64, C.c_uint32, C.c_int, C.c_int, DP, DP, DP]),
         "orc_javarandom_doubles": (None, [C.c_int64, C.c_int, DP]),
         "orc_javarandom_ints": (None, [C.c_int64, C.c_int, IP]),
         "orc_javarandom_ints_bounded": (None, [C.c_int64, C.c_int, C.c_int, IP]),
@@ -166,6 +169,14 @@ class Oracle:
     def move(self):
         self.lib.orc_move(self.h)
 
+    def mcc_perform(self, sp, target, target_mass, sigma, model, max_T, seed, tag, it, frequency, acc):
+        """One MCC pass (collisions/MCC.java:167-221) over species sp.  target: dict of node fields
+        den, u, v, w, T; acc: dict of count_sum, real_sum, S arrays, accumulated in place."""
+        f = {k: np.ascontiguousarray(target[k], np.float64) for k in ("den", "u", "v", "w", "T")}
+        return self.lib.orc_mcc_perform(self.h, sp, _dp(f["den"]), _dp(f["u"]), _dp(f["v"]), _dp(f["w"]),
+                                        _dp(f["T"]), target_mass, sigma, model, max_T, seed, tag, it, frequency,
+                                        _dp(acc["count_sum"]), _dp(acc["real_sum"]), _dp(acc["S"]))
+
     def update_species_fields(self):
         self.lib.orc_update_species_fields(self.h)
 
@@ -215,6 +226,15 @@ class Oracle:
         out = np.zeros(self.nn, np.int32)
         self.lib.orc_fixed_nodes(self.h, _ip(out))
         return out
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 block (Salmon et al., SC'11) of a 4-word counter under a 2-word key."""
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    load().orc_philox4x32_10(c, k, o)
+    return list(o)
 
 
 def boltzmann_density(phi, den0, kTe0, phi0):

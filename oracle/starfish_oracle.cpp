@@ -1333,6 +1333,133 @@ void orc_javarandom_ints_bounded(int64_t seed, int bound, int n, int* out) {
 
 }  // extern "C"
 
+/* ---------------------------------------------------------------------------
+ * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+ * SC'11): the counter-based generator that replaces Starfish.rnd() in the MCC restatement, so
+ * that the draws of a particle depend only on (seed, particle id, step) and not on the order in
+ * which particles are visited.
+ * ------------------------------------------------------------------------- */
+static void philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr_in[0], ctr_in[1], ctr_in[2], ctr_in[3]};
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int round = 0; round < 10; round++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n1 = lo1, n2 = hi0 ^ c[3] ^ k1, n3 = lo0;
+        c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    for (int k = 0; k < 4; k++) out[k] = c[k];
+}
+extern "C" void orc_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    philox4x32_10(ctr, key, out);
+}
+
+/* draw k of a particle's stream: block k/2 of the counter (block, id, step, tag), 53 bits from
+   two 32-bit words, in [0, 1) like java.util.Random.nextDouble */
+struct CounterRng {
+    uint32_t key[2], id, step, tag;
+    double draw(uint32_t k) const {
+        uint32_t ctr[4] = {k >> 1, id, step, tag}, o[4];
+        philox4x32_10(ctr, key, o);
+        uint64_t bits = (k & 1) ? (((uint64_t)o[3] << 32) | o[2]) : (((uint64_t)o[1] << 32) | o[0]);
+        return (double)(bits >> 11) * 0x1.0p-53;
+    }
+};
+
+/* MCC.perform(Mesh) (collisions/MCC.java:167-221) over one kinetic source species with a
+   constant cross-section (Sigma.SigmaConst, interactions/Sigma.java:108-113) and the CEX
+   (MCC.java:287-297) or MEX/elastic (MCC.java:242-285) model.  The target material enters as its
+   node fields: density, stream velocity (u-ave, v-ave, w-ave) and temperature
+   (Material.sampleVelocity / sampleMaxwellianVelocity, Material.java:859-896; Utils.SampleMaxw3D
+   with its 1.2 factor, Utils.java:27-57).  Starfish.rnd() is replaced by the counter-based
+   stream: draw 0 is the collision test, draws 2..37 the target's Maxwellian (12 per component,
+   each component starting a new Philox block), draws 38 and 39 the elastic scattering angles.  Accumulates col-count-sum and col-real-sum at
+   node ((int)lc0, (int)lc1) and the target's energy source S (Field2D.scatter of
+   mpw (E1 - E2) / (dt vol), with E = 0.5 m |v| as the reference writes it).  Returns the number
+   of collisions. */
+extern "C" int64_t orc_mcc_perform(orc_sim* s, int spi, const double* den, const double* uave, const double* vave,
+                                   const double* wave, const double* temp, double target_mass, double sigma,
+                                   int model, double max_T, uint64_t seed, uint32_t tag, int it, int frequency,
+                                   double* count_sum, double* real_sum, double* S) {
+    Species& sp = *s->species[spi];
+    const Mesh& m = s->mesh;
+    Field fden, fu, fv, fw, fT, fS;
+    fden.init(m.ni, m.nj); fu.init(m.ni, m.nj); fv.init(m.ni, m.nj); fw.init(m.ni, m.nj); fT.init(m.ni, m.nj);
+    fS.init(m.ni, m.nj);
+    size_t nn = (size_t)m.ni * m.nj;
+    std::copy(den, den + nn, fden.d.begin());
+    std::copy(uave, uave + nn, fu.d.begin());
+    std::copy(vave, vave + nn, fv.d.begin());
+    std::copy(wave, wave + nn, fw.d.begin());
+    std::copy(temp, temp + nn, fT.d.begin());
+    std::copy(S, S + nn, fS.d.begin());
+    const double dt = frequency * s->dt;
+    CounterRng rng;
+    rng.key[0] = (uint32_t)seed;
+    rng.key[1] = (uint32_t)(seed >> 32);
+    rng.step = (uint32_t)it;
+    rng.tag = tag;
+    int64_t ncol = 0;
+    for (Particle& part : sp.parts) {
+        if (part.mpw <= 0) continue;
+        double den_a = gather(fden, part.lc[0], part.lc[1]);
+        if (den_a <= 0) continue;
+        double tv[3] = {gather(fu, part.lc[0], part.lc[1]), gather(fv, part.lc[0], part.lc[1]),
+                        gather(fw, part.lc[0], part.lc[1])};
+        double gv[3];
+        for (int i = 0; i < 3; i++) gv[i] = tv[i] - part.vel[i];
+        double g = std::sqrt(gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2]);
+        double P = 1 - std::exp(-sigma * g * dt * den_a);
+        rng.id = (uint32_t)part.id;
+        if (P < rng.draw(0)) continue;
+        /* target.sampleMaxwellianVelocity(mesh, lc, 0, max_T) */
+        double vt[3] = {tv[0], tv[1], tv[2]};
+        double T = gather(fT, part.lc[0], part.lc[1]);
+        if (T < 0) T = 0;
+        if (max_T > 0 && T > max_T) T = max_T;
+        double vth = std::sqrt(2 * K_B * T / target_mass);   /* Utils.computeVth */
+        for (int c = 0; c < 3; c++) {
+            double sum = 0;
+            for (int k = 0; k < 12; k++) sum += rng.draw(2 + 12 * c + k);
+            double fm = (sum - 0.5 * 12) * std::sqrt(12.0 / 12);
+            vt[c] += (vth * 1.2) * (std::sqrt(0.5) * 1 * fm);
+        }
+        double E1 = 0.5 * sp.mass * std::sqrt(part.vel[0] * part.vel[0] + part.vel[1] * part.vel[1] +
+                                              part.vel[2] * part.vel[2]);
+        if (model == 0) {
+            for (int c = 0; c < 3; c++) part.vel[c] = vt[c];
+        } else {
+            double rm1 = sp.mass / (sp.mass + target_mass), rm2 = target_mass / (sp.mass + target_mass);
+            double gg[3], vcm[3], vr[3];
+            for (int c = 0; c < 3; c++) gg[c] = part.vel[c] - vt[c];
+            double gmag = std::sqrt(gg[0] * gg[0] + gg[1] * gg[1] + gg[2] * gg[2]);
+            for (int c = 0; c < 3; c++) vcm[c] = rm1 * part.vel[c] + rm2 * vt[c];
+            double B = 2. * rng.draw(38) - 1.0;
+            double A = std::sqrt(1. - B * B);
+            vr[0] = B * gmag;
+            double C = 2. * PI * rng.draw(39);
+            vr[1] = A * std::cos(C) * gmag;
+            vr[2] = A * std::sin(C) * gmag;
+            for (int c = 0; c < 3; c++) part.vel[c] = vcm[c] + vr[c] * rm2;
+        }
+        double E2 = 0.5 * sp.mass * std::sqrt(part.vel[0] * part.vel[0] + part.vel[1] * part.vel[1] +
+                                              part.vel[2] * part.vel[2]);
+        double vol = m.node_volume(part.lc[0], part.lc[1]);
+        double dS = part.mpw * (E1 - E2) / (dt * vol);
+        scatter(m, fS, part.lc[0], part.lc[1], dS);
+        int i = (int)part.lc[0], j = (int)part.lc[1];
+        count_sum[(size_t)j * m.ni + i] += 1;
+        real_sum[(size_t)j * m.ni + i] += part.mpw;
+        ncol++;
+    }
+    std::copy(fS.d.begin(), fS.d.end(), S);
+    return ncol;
+}
+
 // BoltzmannElectronsMaterial.electronModelBoltzmann (core/materials/BoltzmannElectronsMaterial.java:
 // 200-210): den = den0 * Math.exp((phi - phi0) / kTe0), then den < 1 -> 1
 extern "C" void orc_boltzmann_density(int64_t n, const double* phi, double den0, double kTe0, double phi0,

@@ -45,6 +45,12 @@ class SolveInfo(C.Structure):
                 ("ref_norm", C.c_double), ("ms", C.c_double)]
 
 
+class MccDesc(C.Structure):
+    _fields_ = [("source_species", C.c_int), ("model", C.c_int), ("sigma", C.c_double), ("target_molwt", C.c_double),
+                ("frequency", C.c_int), ("stop_it", C.c_int), ("max_target_temp", C.c_double),
+                ("seed", C.c_uint64), ("tag", C.c_uint32)]
+
+
 P = C.c_void_p
 DP = C.POINTER(C.c_double)
 IP = C.POINTER(C.c_int)
@@ -89,6 +95,12 @@ SIGNATURES = {
     "sf_efield_set": (C.c_int, [P, DP, DP]),
     "sf_efield_get": (C.c_int, [P, DP, DP]),
     "sf_bfield_set": (C.c_int, [P, DP, DP]),
+    "sf_mcc_create": (C.c_int, [P, C.POINTER(MccDesc), C.POINTER(C.c_void_p)]),
+    "sf_mcc_destroy": (C.c_int, [P]),
+    "sf_mcc_set_target": (C.c_int, [P, DP, DP, DP, DP, DP]),
+    "sf_mcc_perform": (C.c_int, [P, C.c_int, C.c_double, C.POINTER(C.c_int64)]),
+    "sf_mcc_field": (C.c_int, [P, C.c_int, DP]),
+    "sf_mcc_clear_samples": (C.c_int, [P, C.c_int]),
     "sf_operator_apply": (C.c_int, [P, DP, DP]),
     "sf_source_maxwellian": (C.c_int, [P, C.c_int, C.c_int, DP, C.c_int, C.c_double, C.c_double, C.c_double,
                                        IP]),

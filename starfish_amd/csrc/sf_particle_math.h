@@ -309,6 +309,24 @@ __device__ __forceinline__ bool push_particle(const MeshP& m, const MeshD& md, c
     return alive;
 }
 
+// Corner weights of Field2D.scatter (core/domain/Field2D.java:244-295) for a particle at
+// cell (i,j) + offset (di,dj); RZ uses Ruyten's eq. 4.2 for the radial weight.
+__device__ __forceinline__ void scatter_weights(const MeshP& m, int i, int j, double di, double dj, double* w) {
+    if (m.domain == SF_DOMAIN_RZ) {
+        double fi = i + di;
+        double rp = m.x0[0] + (i + 1) * m.dh[0], rm = m.x0[0] + i * m.dh[0], r = m.x0[0] + fi * m.dh[0];
+        di = 1 - (0.5 * (rp - r) * (2 * rp + 3 * rm - r) / (rp * rp - rm * rm));
+    } else if (m.domain == SF_DOMAIN_ZR) {
+        double fj = j + dj;
+        double rp = m.x0[1] + (j + 1) * m.dh[1], rm = m.x0[1] + j * m.dh[1], r = m.x0[1] + fj * m.dh[1];
+        dj = 1 - (0.5 * (rp - r) * (2 * rp + 3 * rm - r) / (rp * rp - rm * rm));
+    }
+    w[0] = (1 - di) * (1 - dj);   // (i, j)
+    w[1] = di * (1 - dj);         // (i+1, j)
+    w[2] = di * dj;               // (i+1, j+1)
+    w[3] = (1 - di) * dj;         // (i, j+1)
+}
+
 __device__ __forceinline__ int cell_i(int32_t c) { return c & 0xFFFF; }
 __device__ __forceinline__ int cell_j(int32_t c) { return (c >> 16) & 0xFFFF; }
 __device__ __forceinline__ int32_t pack_cell(int i, int j) { return (j << 16) | i; }

@@ -1,0 +1,67 @@
+"""Volume interactions on the device: the `<mcc>` block of the reference's interactions module
+(collisions/MCC.java:45-124, InteractionsModule.performInteractions, interactions/
+InteractionsModule.java:115-130).  The collision pass runs in libstarfish_hip.so
+(sf_mcc_perform); this module holds the parameters and the target material's node fields."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+MODELS = {"CEX": 0, "MEX": 1, "ELASTIC": 1}
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+class MccHip:
+    """MCC of a kinetic GPU species with a background target given by node fields.
+
+    sim: plugin.Simulation; source: the KineticHipMaterial (created with track_ids=True);
+    model: "CEX" or "MEX"/"ELASTIC"; sigma: constant cross-section (m^2); target_molwt: amu;
+    frequency, stop_it, max_target_temp: the reference's keys; seed and tag key the per-particle
+    Philox stream."""
+
+    FIELDS = {"col-count-sum": 0, "col-real-sum": 1, "col-count": 2, "nu": 3, "S": 4}
+
+    def __init__(self, sim, source, model, sigma, target_molwt, frequency=1, stop_it=-1, max_target_temp=-1.0,
+                 seed=0, tag=0):
+        self.sim = sim
+        self.lib = _lib.load()
+        kind = model.upper()
+        if kind not in MODELS:
+            raise ValueError("unsupported MCC model %s (CEX and MEX/ELASTIC are provided)" % model)
+        d = _lib.MccDesc(source.index, MODELS[kind], sigma, target_molwt, frequency, stop_it, max_target_temp, seed,
+                         tag)
+        h = C.c_void_p()
+        sim.ctx.call("sf_mcc_create", C.byref(d), C.byref(h))
+        self.h = h
+
+    def _call(self, name, *args):
+        return _lib.check(self.sim.ctx.h, getattr(self.lib, name)(self.h, *args))
+
+    def set_target(self, den, u=None, v=None, w=None, temp=None):
+        """Target material's density, stream velocity and temperature on the nodes."""
+        arr = [None if a is None else np.ascontiguousarray(np.broadcast_to(a, (self.sim.nn,)), np.float64)
+               for a in (den, u, v, w, temp)]
+        self._call("sf_mcc_set_target", *[_dp(a) for a in arr])
+
+    def perform(self, it, time):
+        """MCC.perform at step it and time `time`; returns the number of collisions."""
+        n = C.c_int64()
+        self._call("sf_mcc_perform", it, time, C.byref(n))
+        return n.value
+
+    def field(self, name):
+        out = np.zeros(self.sim.nn)
+        self._call("sf_mcc_field", self.FIELDS[name], _dp(out))
+        return out
+
+    def clearSamples(self, clear_source=False):
+        self._call("sf_mcc_clear_samples", 1 if clear_source else 0)
+
+    def close(self):
+        if self.h:
+            self.lib.sf_mcc_destroy(self.h)
+            self.h = None
