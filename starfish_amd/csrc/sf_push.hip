@@ -106,6 +106,13 @@ __device__ void stat_flush(StatAcc s, long long* stats) {
     }
 }
 
+// Workgroup barrier that waits for the wave's LDS operations only.  __syncthreads() also waits
+// for all outstanding global loads and stores (vmcnt(0)); the window loop exchanges data through
+// LDS alone, so its prefetched loads and its stores may stay in flight across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // ----------------------------------------------------------------------------------------
 // pass A, sorted particles: one workgroup per tile of TILE_W x TILE_H cells
 // ----------------------------------------------------------------------------------------
@@ -120,7 +127,7 @@ struct TileLds {
     float o_fx[PUSH_THREADS], o_fy[PUSH_THREADS], o_vx[PUSH_THREADS], o_vy[PUSH_THREADS], o_vz[PUSH_THREADS];
 };
 
-// Particle state in registers (loaded one iteration ahead of its use).
+// Particle state of one window slot in registers.
 struct PReg {
     int64_t g;
     int32_t c;
@@ -133,7 +140,7 @@ struct PReg {
 // The tile's particles (its cell rows, contiguous after the last sort) are visited in windows of
 // PUSH_THREADS; thread t takes the window's particle (t * CHUNK_STRIDE) mod PUSH_THREADS, so the
 // lanes of a wave fall about one cell apart and their LDS atomics hit different nodes.  Each
-// thread loads its particle straight into registers, one window ahead.  Results go back through
+// thread loads its particle straight into registers.  Results go back through
 // LDS by slot and are stored coalesced: stores in the permuted order write partial cache lines
 // and run at about half the bandwidth of coalesced ones on this part.
 template <bool INPLACE, bool HAS_B>
@@ -203,25 +210,39 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
     };
     const int pt = (t * CHUNK_STRIDE) & (PUSH_THREADS - 1);
     // The tile's particles: TILE_H contiguous row segments, each visited in windows of
-    // PUSH_THREADS slots (the last window of a row may be partial).
-    for (int cr = 0; cr < TILE_H; cr++) {
-    const int rlen = L.seg_pre[cr + 1] - L.seg_pre[cr];
-    const int64_t rst = L.seg_start[cr];
-    for (int cb = 0; cb < rlen; cb += PUSH_THREADS) {
+    // PUSH_THREADS slots (the last window of a row may be partial).  The barriers between
+    // windows wait for LDS only, so the previous window's stores stay in flight across them.
+    // (Loading the next window into registers ahead of time measured slower: the extra
+    // registers cost more than the latency it hides at four workgroups per CU.)
+    auto row_len = [&](int r) { return L.seg_pre[r + 1] - L.seg_pre[r]; };
+    auto fetch = [&](int r, int b, PReg& q) {
+        if (b + pt < row_len(r)) {
+            q.g = L.seg_start[r] + b + pt;
+            q.c = s.cell[q.g];
+            q.fx = s.fx[q.g];
+            q.fy = s.fy[q.g];
+            q.vx = s.vx[q.g];
+            q.vy = s.vy[q.g];
+            q.vz = s.vz[q.g];
+        } else {
+            q.g = 0;
+            q.c = -1;
+            q.fx = q.fy = q.vx = q.vy = q.vz = 0.0f;
+        }
+    };
+    int cr = 0, cb = 0;
+    while (cr < TILE_H && row_len(cr) == 0) cr++;
+    while (cr < TILE_H) {
+        const int rlen = row_len(cr);
+        const int64_t rst = L.seg_start[cr];
         const bool ok = cb + pt < rlen;
         PReg cur;
-        if (ok) {
-            cur.g = rst + cb + pt;
-            cur.c = s.cell[cur.g];
-            cur.fx = s.fx[cur.g];
-            cur.fy = s.fy[cur.g];
-            cur.vx = s.vx[cur.g];
-            cur.vy = s.vy[cur.g];
-            cur.vz = s.vz[cur.g];
-        } else {
-            cur.g = 0;
-            cur.c = -1;
-            cur.fx = cur.fy = cur.vx = cur.vy = cur.vz = 0.0f;
+        fetch(cr, cb, cur);
+        int nr = cr, nb = cb + PUSH_THREADS;
+        if (nb >= rlen) {
+            nr = cr + 1;
+            nb = 0;
+            while (nr < TILE_H && row_len(nr) == 0) nr++;
         }
         int32_t key = -1;
         float fxn = cur.fx, fyn = cur.fy, vxn = cur.vx, vyn = cur.vy, vzn = cur.vz;
@@ -253,14 +274,25 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
                     if (li >= 0 && li + 1 < TILE_NW && lj >= 0 && lj + 1 < TILE_NH) {
                         double w4[4];
                         scatter_weights(m, ci, cj, di, dj, w4);
-                        const int off[4] = {0, 1, TILE_NW + 1, TILE_NW};
                         int l0 = lj * TILE_NW + li;
+                        // Lanes of one cell would all add to the same node in one instruction;
+                        // each lane starts at corner (lane mod 4), so those lanes spread over the
+                        // four corners and the same-address conflicts drop about fourfold.
+                        const int rot = t & 3;
+                        double wr[4];
+                        int offr[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const int cq = (q + rot) & 3;
+                            wr[q] = cq == 0 ? w4[0] : (cq == 1 ? w4[1] : (cq == 2 ? w4[2] : w4[3]));
+                            offr[q] = cq == 0 ? 0 : (cq == 1 ? 1 : (cq == 2 ? TILE_NW + 1 : TILE_NW));
+                        }
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
 #pragma unroll
                             for (int ch = 0; ch < NCH; ch++) {
-                                long long v = quant(w4[q], val[ch]);
-                                atomicAdd(&L.acc[ch][l0 + off[q]], (unsigned long long)v);
+                                long long v = quant(wr[q], val[ch]);
+                                atomicAdd(&L.acc[ch][l0 + offr[q]], (unsigned long long)v);
                             }
                         }
                     } else {
@@ -279,7 +311,7 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
                 L.o_vz[pt] = vzn;
             }
         }
-        __syncthreads();
+        lds_barrier();
         const int wlen = min(PUSH_THREADS, rlen - cb);
         if (t < wlen) {
             const int64_t g = rst + cb + t;
@@ -297,8 +329,9 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
                 keys[g] = kc;
             }
         }
-        __syncthreads();
-    }
+        lds_barrier();
+        cr = nr;
+        cb = nb;
     }
     __syncthreads();
     // flush LDS accumulators (nonzero entries) with 64-bit integer atomics
