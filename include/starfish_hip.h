@@ -301,6 +301,41 @@ int sf_mcc_field(sf_mcc* mcc, int which, double* out);
 int sf_mcc_clear_samples(sf_mcc* mcc, int clear_source);
 
 /* ------------------------------------------------------------------------------------------
+ * DSMC: Bird's no-time-counter collisions of a pair of kinetic species (collisions/DSMC.java:
+ * 45-430) with Boyd's weights for unequal specific weights, the elastic model (VHS scattering,
+ * or VSS when the pair's vss_alpha differs from 1) and a constant or Bird eq. 4.63 VHS
+ * cross-section (collisions/SigmaPlus.java:38-78, from the materials' <diam>, <ref_temp>,
+ * <visc_temp_index>).  Both species must track ids: a cell's list is ordered by particle id and
+ * its draws come from Philox4x32-10 counted by (cell, step, tag).  A species that was pushed in
+ * place since its last sort is sorted first.  At most 2048 particles of one species per cell
+ * (SF_ERR_UNSUPPORTED beyond).  The reference's steady-state clearSamples is left to the caller.
+ * ------------------------------------------------------------------------------------------ */
+#define SF_SIGMA_CONST 0
+#define SF_SIGMA_BIRD463 1
+typedef struct {
+    int species_a, species_b;        /* <pair>; the same index for a single species */
+    int sigma_type;                  /* SF_SIGMA_CONST or SF_SIGMA_BIRD463 */
+    double sigma_c0;                 /* constant cross-section, m^2 */
+    double diam_a, diam_b;           /* materials' <diam> */
+    double ref_temp_a, ref_temp_b;   /* <ref_temp> (reference default 275) */
+    double visc_temp_index_a, visc_temp_index_b;   /* <visc_temp_index> (default 0.85) */
+    double vss_alpha_a, vss_alpha_b; /* <vss_alpha> (default 1: VHS) */
+    double sig_cr_max0;              /* <sig_cr_max> initial value (reference default 1e-16) */
+    int frequency;                   /* <frequency> */
+    uint64_t seed;
+    uint32_t tag;
+} sf_dsmc_desc;
+typedef struct sf_dsmc sf_dsmc;
+int sf_dsmc_create(sf_ctx* ctx, const sf_dsmc_desc* desc, sf_dsmc** out);
+int sf_dsmc_destroy(sf_dsmc* dsmc);
+/* DSMC.perform (DSMC.java:163-190) at step `it`; skipped unless it % frequency == 0 */
+int sf_dsmc_perform(sf_dsmc* dsmc, int it, int64_t* collisions);
+/* which: 0 col-count-sum, 1 col-real-sum, 2 col-count, 3 nu (node fields at the cell's first
+   node), 4 sig_cr_max per cell (index j*(ni-1)+i) */
+int sf_dsmc_field(sf_dsmc* dsmc, int which, double* out);
+int sf_dsmc_clear_samples(sf_dsmc* dsmc);
+
+/* ------------------------------------------------------------------------------------------
  * Learned initial guess for the potential solve.  No counterpart in the reference: the
  * reference solvers start from the previous potential (LinearSolverPCG.java:33-37,
  * LinearSolverGS.java:41-53).  A 3-level encoder-decoder CNN (32/64/128 channels, 3x3

@@ -69,6 +69,10 @@ def load():
         "orc_set_rng_state": (None, [P, C.c_int64]),
         "orc_round_state_fp32": (None, [P, C.c_int]),
         "orc_boltzmann_density": (None, [C.c_int64, DP, C.c_double, C.c_double, C.c_double, DP]),
+        "orc_bird_gamma": (C.c_double, [C.c_double]),
+        "orc_dsmc_perform": (C.c_int64, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                         C.c_uint32, C.c_int, C.c_int, DP, DP, DP, DP]),
         "orc_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "orc_mcc_perform": (C.c_int64, [P, C.c_int, DP, DP, DP, DP, DP, C.c_double, C.c_double, C.c_int, C.c_double,
                                         C.c_uint64, C.c_uint32, C.c_int, C.c_int, DP, DP, DP]),
@@ -168,6 +172,15 @@ class Oracle:
 
     def move(self):
         self.lib.orc_move(self.h)
+
+    def dsmc_perform(self, spa, spb, pair, seed, tag, it, frequency, state, acc):
+        """One DSMC pass (collisions/DSMC.java:167-430).  pair: dict sigma_type, c0, diam, ref_temp,
+        visc_temp_index, vss_alpha (each of the last four a pair of values); state: dict of per-cell
+        sig_cr_max and rem (in/out); acc: dict of count_sum and real_sum node fields (accumulated)."""
+        d, t, w, a = pair["diam"], pair["ref_temp"], pair["visc_temp_index"], pair["vss_alpha"]
+        return self.lib.orc_dsmc_perform(self.h, spa, spb, pair["sigma_type"], pair.get("c0", 0.0), d[0], d[1], t[0], t[1],
+                                         w[0], w[1], a[0], a[1], seed, tag, it, frequency, _dp(state["sig_cr_max"]),
+                                         _dp(state["rem"]), _dp(acc["count_sum"]), _dp(acc["real_sum"]))
 
     def mcc_perform(self, sp, target, target_mass, sigma, model, max_T, seed, tag, it, frequency, acc):
         """One MCC pass (collisions/MCC.java:167-221) over species sp.  target: dict of node fields

@@ -1460,6 +1460,171 @@ extern "C" int64_t orc_mcc_perform(orc_sim* s, int spi, const double* den, const
     return ncol;
 }
 
+/* Utils.gamma (core/common/Utils.java:218-236): Bird's approximation of the Gamma function */
+static double bird_gamma(double X) {
+    double A = 1., Y = X;
+    if (Y < 1.0) {
+        A = A / Y;
+    } else {
+        do {
+            Y = Y - 1;
+            if (Y >= 1.) A = A * Y;
+        } while (Y >= 1.);
+    }
+    return A * (1. - 0.5748646 * Y + 0.9512363 * Y * Y - 0.6998588 * Y * Y * Y + 0.4245549 * Y * Y * Y * Y -
+                0.1010678 * Y * Y * Y * Y * Y);
+}
+
+extern "C" double orc_bird_gamma(double x) { return bird_gamma(x); }
+
+/* DSMC pair parameters: the cross-section (SigmaConst, or SigmaBird463 of
+   collisions/SigmaPlus.java:38-78 from the two materials' diam, ref_temp, visc_temp_index) and the
+   VSS exponent vss_inv = (1/alpha1 + 1/alpha2)/2 (DSMC.java:83) */
+struct DsmcPair {
+    int sigma_type;   /* 0 const, 1 Bird463 */
+    double c0, sigma0, ref_temp, visc_temp_index, reduced_mass, gamma, vss_inv;
+    double eval(double g) const {
+        if (sigma_type == 0) return c0;
+        return sigma0 * std::pow(2. * K_B * ref_temp / (reduced_mass * g * g), visc_temp_index - 0.5) / gamma;
+    }
+};
+
+/* DSMC.perform(Mesh) + collideCell + ModelElastic (collisions/DSMC.java:167-430) for species
+   spa, spb (the same index for a single-species pair), one pass at step `it`.  Cell lists hold the
+   particles with (int)lc inside the cells, ordered by particle id; Starfish.rnd() is replaced by
+   the counter-based stream of the cell (draw k of counter (k/2, cell, step, tag)), consumed in
+   the reference's order.  sig_cr_max and rem are the per-cell state (cell index j*(ni-1)+i),
+   count_sum / real_sum the node fields at node (i, j).  Returns the number of collisions. */
+extern "C" int64_t orc_dsmc_perform(orc_sim* s, int spa, int spb, int sigma_type, double c0, double diam_a,
+                                    double diam_b, double ref_temp_a, double ref_temp_b, double vti_a, double vti_b,
+                                    double vss_alpha_a, double vss_alpha_b, uint64_t seed, uint32_t tag, int it,
+                                    int frequency, double* sig_cr_max, double* rem, double* count_sum,
+                                    double* real_sum) {
+    const Mesh& m = s->mesh;
+    Species& A = *s->species[spa];
+    Species& B = *s->species[spb];
+    const bool same = spa == spb;
+    DsmcPair pr;
+    pr.sigma_type = sigma_type;
+    pr.c0 = c0;
+    double d_ref = diam_a + diam_b;
+    pr.sigma0 = 0.25 * PI * d_ref * d_ref;
+    pr.ref_temp = 0.5 * (ref_temp_a + ref_temp_b);
+    pr.visc_temp_index = 0.5 * (vti_a + vti_b);
+    pr.reduced_mass = A.mass * B.mass / (A.mass + B.mass);
+    pr.gamma = bird_gamma(2.5 - pr.visc_temp_index);
+    pr.vss_inv = 0.5 * (1.0 / vss_alpha_a + 1.0 / vss_alpha_b);
+    const int nci = m.ni - 1, ncj = m.nj - 1;
+    std::vector<std::vector<Particle*>> la((size_t)nci * ncj), lb(same ? 0 : (size_t)nci * ncj);
+    auto bin = [&](Species& sp, std::vector<std::vector<Particle*>>& L) {
+        for (Particle& p : sp.parts) {
+            if (p.mpw <= 0) continue;
+            int i = (int)p.lc[0], j = (int)p.lc[1];
+            if (i >= m.ni - 1 || j >= m.nj - 1) continue;
+            L[(size_t)j * nci + i].push_back(&p);
+        }
+        for (auto& c : L) std::sort(c.begin(), c.end(), [](const Particle* x, const Particle* y) { return x->id < y->id; });
+    };
+    bin(A, la);
+    if (!same) bin(B, lb);
+    const double delta_t = frequency * s->dt;
+    const double spwt1 = A.spwt, spwt2 = B.spwt;
+    CounterRng rng;
+    rng.key[0] = (uint32_t)seed;
+    rng.key[1] = (uint32_t)(seed >> 32);
+    rng.step = (uint32_t)it;
+    rng.tag = tag;
+    int64_t ntot = 0;
+    for (int j = 0; j < ncj; j++)
+        for (int i = 0; i < nci; i++) {
+            const size_t c = (size_t)j * nci + i;
+            std::vector<Particle*>& l1 = la[c];
+            std::vector<Particle*>& l2 = same ? la[c] : lb[c];
+            double np1 = (double)l1.size(), np2 = (double)l2.size();
+            double cell_volume = m.node_volume(i + 0.5, j + 0.5);   /* Mesh.cellVol */
+            double Pab = spwt2 / spwt1, Pba = spwt1 / spwt2;
+            if (Pab > 1) Pab = 1;
+            if (Pba > 1) Pba = 1;
+            double nsel_f = ((np1 * spwt1 / cell_volume) * np2 * delta_t * sig_cr_max[c]) / (Pab + (spwt2 / spwt1) * Pba);
+            if (!same) nsel_f *= 2.0;
+            nsel_f += rem[c];
+            int nsel = (int)nsel_f;
+            if ((same && (np1 < 2 || np2 < 2)) || np1 < 1 || np2 < 1) continue;
+            rem[c] = nsel_f - nsel;
+            if (nsel > np1 * np2) {
+                nsel = (int)(np1 * np2);
+                rem[c] = 0;
+            }
+            rng.id = (uint32_t)c;
+            uint32_t k = 0;
+            double scm = 0, sums0 = 0, sums1 = 0;
+            for (int q = 0; q < nsel; q++) {
+                int p1 = (int)(rng.draw(k++) * np1), p2;
+                do {
+                    p2 = (int)(rng.draw(k++) * np2);
+                } while (same && p1 == p2);
+                Particle& a = *l1[p1];
+                Particle& b = *l2[p2];
+                double cr[3] = {a.vel[0] - b.vel[0], a.vel[1] - b.vel[1], a.vel[2] - b.vel[2]};
+                double cr_mag = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+                double sigma_cr = cr_mag * pr.eval(cr_mag);
+                if (sigma_cr > scm) scm = sigma_cr;
+                double P = sigma_cr / sig_cr_max[c];
+                if (rng.draw(k++) < P) {
+                    /* ModelElastic (DSMC.java:360-430) */
+                    double ma = A.mass, mb = B.mass;
+                    double rm1 = ma / (ma + mb), rm2 = mb / (ma + mb);
+                    double g[3] = {a.vel[0] - b.vel[0], a.vel[1] - b.vel[1], a.vel[2] - b.vel[2]};
+                    double g_mag = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+                    double vc[3], vr[3];
+                    for (int d = 0; d < 3; d++) vc[d] = rm1 * a.vel[d] + rm2 * b.vel[d];
+                    if (std::fabs(pr.vss_inv - 1.) < 1.e-4) {
+                        double Bc = 2. * rng.draw(k++) - 1.0;
+                        double Ac = std::sqrt(1. - Bc * Bc);
+                        vr[0] = Bc * g_mag;
+                        double Cc = 2. * PI * rng.draw(k++);
+                        vr[1] = Ac * std::cos(Cc) * g_mag;
+                        vr[2] = Ac * std::sin(Cc) * g_mag;
+                    } else {
+                        double Bc = 2. * (std::pow(rng.draw(k++), pr.vss_inv)) - 1.;
+                        double Ac = std::sqrt(1. - Bc * Bc);
+                        double Cc = 2. * PI * rng.draw(k++);
+                        double OC = std::cos(Cc), SC = std::sin(Cc);
+                        double D = std::sqrt(g[1] * g[1] + g[2] * g[2]);
+                        if (D > 1.e-6) {
+                            vr[0] = Bc * g[0] + Ac * SC * D;
+                            vr[1] = Bc * g[1] + Ac * (g_mag * g[2] * OC - g[0] * g[1] * SC) / D;
+                            vr[2] = Bc * g[2] - Ac * (g_mag * g[1] * OC + g[0] * g[2] * SC) / D;
+                        } else {
+                            vr[0] = Bc * g[1];
+                            vr[1] = Ac * OC * g[1];
+                            vr[2] = Ac * SC * g[1];
+                        }
+                    }
+                    if (a.mpw == b.mpw) {
+                        for (int d = 0; d < 3; d++) {
+                            a.vel[d] = vc[d] + vr[d] * rm2;
+                            b.vel[d] = vc[d] - vr[d] * rm1;
+                        }
+                    } else {
+                        double pab = b.mpw / a.mpw, pba = a.mpw / b.mpw;
+                        if (rng.draw(k++) < pab)
+                            for (int d = 0; d < 3; d++) a.vel[d] = vc[d] + rm2 * vr[d];
+                        if (rng.draw(k++) < pba)
+                            for (int d = 0; d < 3; d++) b.vel[d] = vc[d] - rm1 * vr[d];
+                    }
+                    sums0 += 1.0;
+                    sums1 += 0.5 * (a.mpw + b.mpw);
+                }
+            }
+            if (scm > 0) sig_cr_max[c] = scm;
+            count_sum[(size_t)j * m.ni + i] += sums0;
+            real_sum[(size_t)j * m.ni + i] += sums1;
+            ntot += (int64_t)sums0;
+        }
+    return ntot;
+}
+
 // BoltzmannElectronsMaterial.electronModelBoltzmann (core/materials/BoltzmannElectronsMaterial.java:
 // 200-210): den = den0 * Math.exp((phi - phi0) / kTe0), then den < 1 -> 1
 extern "C" void orc_boltzmann_density(int64_t n, const double* phi, double den0, double kTe0, double phi0,

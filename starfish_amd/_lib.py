@@ -51,6 +51,14 @@ class MccDesc(C.Structure):
                 ("seed", C.c_uint64), ("tag", C.c_uint32)]
 
 
+class DsmcDesc(C.Structure):
+    _fields_ = [("species_a", C.c_int), ("species_b", C.c_int), ("sigma_type", C.c_int), ("sigma_c0", C.c_double),
+                ("diam_a", C.c_double), ("diam_b", C.c_double), ("ref_temp_a", C.c_double), ("ref_temp_b", C.c_double),
+                ("visc_temp_index_a", C.c_double), ("visc_temp_index_b", C.c_double), ("vss_alpha_a", C.c_double),
+                ("vss_alpha_b", C.c_double), ("sig_cr_max0", C.c_double), ("frequency", C.c_int),
+                ("seed", C.c_uint64), ("tag", C.c_uint32)]
+
+
 P = C.c_void_p
 DP = C.POINTER(C.c_double)
 IP = C.POINTER(C.c_int)
@@ -101,6 +109,11 @@ SIGNATURES = {
     "sf_mcc_perform": (C.c_int, [P, C.c_int, C.c_double, C.POINTER(C.c_int64)]),
     "sf_mcc_field": (C.c_int, [P, C.c_int, DP]),
     "sf_mcc_clear_samples": (C.c_int, [P, C.c_int]),
+    "sf_dsmc_create": (C.c_int, [P, C.POINTER(DsmcDesc), C.POINTER(C.c_void_p)]),
+    "sf_dsmc_destroy": (C.c_int, [P]),
+    "sf_dsmc_perform": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64)]),
+    "sf_dsmc_field": (C.c_int, [P, C.c_int, DP]),
+    "sf_dsmc_clear_samples": (C.c_int, [P]),
     "sf_operator_apply": (C.c_int, [P, DP, DP]),
     "sf_source_maxwellian": (C.c_int, [P, C.c_int, C.c_int, DP, C.c_int, C.c_double, C.c_double, C.c_double,
                                        IP]),

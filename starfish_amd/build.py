@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstarfish_hip.so")
-SOURCES = ["sf_core.hip", "sf_push.hip", "sf_poisson.hip", "sf_cnn.hip", "sf_mcc.hip", "sf_host.cpp"]
+SOURCES = ["sf_core.hip", "sf_push.hip", "sf_poisson.hip", "sf_cnn.hip", "sf_mcc.hip", "sf_dsmc.hip", "sf_host.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

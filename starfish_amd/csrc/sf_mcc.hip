@@ -8,48 +8,9 @@
 #include <vector>
 
 #include "sf_particle_math.h"
+#include "sf_rng.h"
 
 namespace sf {
-
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of two 32x32->64 multiplies
-// with the key bumped by the Weyl constants between rounds.
-__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
-        uint4 n;
-        n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
-        n.y = (uint32_t)p1;
-        n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
-        n.w = (uint32_t)p0;
-        c = n;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    return c;
-}
-
-// A particle's stream: draw k uses half k&1 of block k>>1 of the counter (block, id, step, tag),
-// 53 bits in [0, 1).
-struct ParticleRng {
-    uint32_t k0, k1, id, step, tag;
-    __device__ __forceinline__ double draw(uint32_t k) const {
-        uint4 o = philox4x32_10(make_uint4(k >> 1, id, step, tag), k0, k1);
-        uint64_t bits = (k & 1) ? (((uint64_t)o.w << 32) | o.z) : (((uint64_t)o.y << 32) | o.x);
-        return (double)(bits >> 11) * 0x1.0p-53;
-    }
-    // twelve consecutive draws summed in order (Utils.SampleMaxw1D), six Philox blocks
-    __device__ __forceinline__ double sum12(uint32_t k) const {
-        double s = 0;
-        for (uint32_t b = 0; b < 6; b++) {
-            uint4 o = philox4x32_10(make_uint4((k >> 1) + b, id, step, tag), k0, k1);
-            s += (double)(((((uint64_t)o.y << 32) | o.x)) >> 11) * 0x1.0p-53;
-            s += (double)(((((uint64_t)o.w << 32) | o.z)) >> 11) * 0x1.0p-53;
-        }
-        return s;
-    }
-};
 
 // Field2D.gather with the gather_safe fallback (Field2D.java:300-390) on an FP64 node field.
 __device__ __forceinline__ double gather_node(const MeshP& m, const double* __restrict__ f, double fi, double fj) {
@@ -72,25 +33,6 @@ __device__ __forceinline__ double gather_node(const MeshP& m, const double* __re
     v += di * dj * at(i + 1, j + 1);
     v += (1 - di) * dj * at(i, j + 1);
     return v;
-}
-
-// Mesh.nodeVol(i0, j0) (Mesh.java:998-1031) at a fractional position on a single uniform mesh:
-// the control volume of half-width 1/2 clipped to the mesh (controlVolumeLCs, Mesh.java:656-716),
-// depth 1 in XY, 2 pi A r in RZ/ZR with r moved a quarter cell inwards on the first/last line.
-__device__ __forceinline__ double node_vol_at(const MeshP& m, double i0, double j0) {
-    const double ia = fmax(i0 - 0.5, 0.0), ib = fmin(i0 + 0.5, (double)(m.ni - 1));
-    const double ja = fmax(j0 - 0.5, 0.0), jb = fmin(j0 + 0.5, (double)(m.nj - 1));
-    const double A = (ib - ia) * m.dh[0] * ((jb - ja) * m.dh[1]);
-    if (m.domain == SF_DOMAIN_XY) return A;
-    double r;
-    if (m.domain == SF_DOMAIN_RZ) {
-        double ir = i0 == 0 ? 0.25 : (i0 == m.ni - 1 ? i0 - 0.25 : i0);
-        r = m.x0[0] + ir * m.dh[0];
-    } else {
-        double jr = j0 == 0 ? 0.25 : (j0 == m.nj - 1 ? j0 - 0.25 : j0);
-        r = m.x0[1] + jr * m.dh[1];
-    }
-    return 2 * M_PI * A * r;
 }
 
 struct MccTarget {

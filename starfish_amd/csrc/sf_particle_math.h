@@ -327,6 +327,41 @@ __device__ __forceinline__ void scatter_weights(const MeshP& m, int i, int j, do
     w[3] = (1 - di) * dj;         // (i, j+1)
 }
 
+// Mesh.nodeVol(i0, j0) (Mesh.java:998-1031) at a fractional position on a single uniform mesh:
+// the corners (i0 -/+ 1/2, j0 -/+ 1/2) clamped to the mesh (controlVolumeLCs, Mesh.java:656-716),
+// their positions, and the area of the quadrilateral as the sum of two triangles (Mesh.area,
+// Mesh.java:965-985), evaluated in the reference's order without fused multiply-adds so that it
+// rounds as the reference does (a DSMC pair count is the integer part of a product with it);
+// depth 1 in XY, 2 pi A r in RZ/ZR with r moved a quarter cell inwards on the first/last line.
+__device__ __forceinline__ double node_vol_at(const MeshP& m, double i0, double j0) {
+#pragma clang fp contract(off)
+    double V[4][2];
+    const double di[4] = {-0.5, 0.5, 0.5, -0.5}, dj[4] = {-0.5, -0.5, 0.5, 0.5};
+    for (int k = 0; k < 4; k++) {
+        double i = i0 + di[k], j = j0 + dj[k];
+        if (i < 0) i = 0;
+        if (i > m.ni - 1) i = m.ni - 1;
+        if (j < 0) j = 0;
+        if (j > m.nj - 1) j = m.nj - 1;
+        V[k][0] = m.x0[0] + i * m.dh[0];
+        V[k][1] = m.x0[1] + j * m.dh[1];
+    }
+    double A = 0.5 * fabs(V[0][0] * V[1][1] - V[0][0] * V[2][1] + V[1][0] * V[2][1] - V[1][0] * V[0][1] +
+                          V[2][0] * V[0][1] - V[2][0] * V[1][1]);
+    A += 0.5 * fabs(V[0][0] * V[2][1] - V[0][0] * V[3][1] + V[2][0] * V[3][1] - V[2][0] * V[0][1] +
+                    V[3][0] * V[0][1] - V[3][0] * V[2][1]);
+    if (m.domain == SF_DOMAIN_XY) return 1.0 * A;
+    double r;
+    if (m.domain == SF_DOMAIN_RZ) {
+        const double ir = i0 == 0 ? i0 + 0.25 : (i0 == m.ni - 1 ? i0 - 0.25 : i0);
+        r = m.x0[0] + ir * m.dh[0];
+    } else {
+        const double jr = j0 == 0 ? j0 + 0.25 : (j0 == m.nj - 1 ? j0 - 0.25 : j0);
+        r = m.x0[1] + jr * m.dh[1];
+    }
+    return 2 * M_PI * A * r;
+}
+
 __device__ __forceinline__ int cell_i(int32_t c) { return c & 0xFFFF; }
 __device__ __forceinline__ int cell_j(int32_t c) { return (c >> 16) & 0xFFFF; }
 __device__ __forceinline__ int32_t pack_cell(int i, int j) { return (j << 16) | i; }

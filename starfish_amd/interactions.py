@@ -65,3 +65,57 @@ class MccHip:
         if self.h:
             self.lib.sf_mcc_destroy(self.h)
             self.h = None
+
+
+class DsmcHip:
+    """`<dsmc>` of the reference's collisions plugin (collisions/DSMC.java:45-124) between two
+    GPU species (the same one for a single-species gas), both created with track_ids=True.
+
+    sigma: "const" (sigma_c0 in m^2) or "bird463" (VHS from each material's diam, ref_temp and
+    visc_temp_index, given as pairs); vss_alpha: pair of the materials' <vss_alpha>;
+    sig_cr_max: initial value; frequency: the reference's key; seed and tag key the per-cell
+    Philox streams."""
+
+    FIELDS = {"col-count-sum": 0, "col-real-sum": 1, "col-count": 2, "nu": 3}
+    SIGMA = {"CONST": 0, "BIRD463": 1}
+
+    def __init__(self, sim, mat_a, mat_b, sigma="bird463", sigma_c0=0.0, diam=(0.0, 0.0), ref_temp=(275.0, 275.0),
+                 visc_temp_index=(0.85, 0.85), vss_alpha=(1.0, 1.0), sig_cr_max=1e-16, frequency=1, seed=0, tag=0):
+        self.sim = sim
+        self.lib = _lib.load()
+        kind = sigma.upper()
+        if kind not in self.SIGMA:
+            raise ValueError("unsupported DSMC cross-section %s (const and bird463 are provided)" % sigma)
+        d = _lib.DsmcDesc(mat_a.index, mat_b.index, self.SIGMA[kind], sigma_c0, diam[0], diam[1], ref_temp[0],
+                          ref_temp[1], visc_temp_index[0], visc_temp_index[1], vss_alpha[0], vss_alpha[1], sig_cr_max,
+                          frequency, seed, tag)
+        h = C.c_void_p()
+        sim.ctx.call("sf_dsmc_create", C.byref(d), C.byref(h))
+        self.h = h
+
+    def _call(self, name, *args):
+        return _lib.check(self.sim.ctx.h, getattr(self.lib, name)(self.h, *args))
+
+    def perform(self, it):
+        n = C.c_int64()
+        self._call("sf_dsmc_perform", it, C.byref(n))
+        return n.value
+
+    def field(self, name):
+        out = np.zeros(self.sim.nn)
+        self._call("sf_dsmc_field", self.FIELDS[name], _dp(out))
+        return out
+
+    def sig_cr_max(self):
+        m = self.sim.deck.mesh
+        out = np.zeros((m.ni - 1) * (m.nj - 1))
+        self._call("sf_dsmc_field", 4, _dp(out))
+        return out
+
+    def clearSamples(self):
+        self._call("sf_dsmc_clear_samples")
+
+    def close(self):
+        if self.h:
+            self.lib.sf_dsmc_destroy(self.h)
+            self.h = None
