@@ -154,6 +154,18 @@ int sf_particles_count(sf_ctx* ctx, int species, int64_t* n);
    right after a counting sort). */
 int sf_particles_download(sf_ctx* ctx, int species, double* x, double* y, double* lc0, double* lc1, double* u,
                           double* v, double* w, int* ids);
+/* Restart block of a kinetic species, the bytes KineticMaterial.saveRestartData writes to
+   restart.bin (core/materials/KineticMaterial.java:904-951, big-endian as java.io.DataOutputStream;
+   RestartModule writes the step and the material count before the blocks,
+   core/common/RestartModule.java:120-140): num_samples, np, per particle pos/vel interleaved,
+   lc, dt, mpw, mass, born_it, id, then den, den-ave, T, U, V, W, u-ave, v-ave, w-ave as
+   Field2D.binaryWrite (i-major).  pos[2], dt and born_it are written as 0 (not kept on the
+   device).  Reading follows loadRestartData (KineticMaterial.java:953-1000): every particle goes
+   through addParticle (velocity rewind with the current E, new id), num_samples is restored and
+   the fields are skipped (the next step recomputes them). */
+int sf_species_restart_size(sf_ctx* ctx, int species, int64_t* bytes);
+int sf_species_restart_write(sf_ctx* ctx, int species, uint8_t* buf, int64_t bytes);
+int sf_species_restart_read(sf_ctx* ctx, int species, const uint8_t* buf, int64_t bytes, int64_t* consumed);
 /* Device-side preload: n particles uniformly over the mesh cells whose four nodes are OPEN,
    velocities drifting Maxwellian (drift vector + temperature in K, Utils.SampleMaxw3D's 1.2
    widening kept, core/common/Utils.java:49-57), counter-based generator keyed by seed.

@@ -85,6 +85,9 @@ SIGNATURES = {
     "sf_particles_add": (C.c_int, [P, C.c_int, I64, DP, DP, DP, DP, DP, IP, C.c_int]),
     "sf_particles_count": (C.c_int, [P, C.c_int, C.POINTER(I64)]),
     "sf_particles_download": (C.c_int, [P, C.c_int, DP, DP, DP, DP, DP, DP, DP, IP]),
+    "sf_species_restart_size": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64)]),
+    "sf_species_restart_write": (C.c_int, [P, C.c_int, C.c_void_p, C.c_int64]),
+    "sf_species_restart_read": (C.c_int, [P, C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "sf_particles_preload": (C.c_int, [P, C.c_int, I64, DP, C.c_double, C.c_uint64]),
     "sf_push_deposit": (C.c_int, [P, C.c_int]),
     "sf_species_set_sort_interval": (C.c_int, [P, C.c_int, C.c_int]),

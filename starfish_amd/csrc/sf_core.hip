@@ -393,6 +393,117 @@ int sf_particles_download(sf_ctx* ctx, int species, double* x, double* y, double
     return SF_OK;
 }
 
+// ---- restart block of a kinetic species (KineticMaterial.saveRestartData / loadRestartData,
+// core/materials/KineticMaterial.java:904-1000), in java.io.DataOutputStream's big-endian encoding
+static void be_put(uint8_t*& p, uint64_t bits, int nbytes) {
+    for (int k = nbytes - 1; k >= 0; k--) *p++ = (uint8_t)(bits >> (8 * k));
+}
+static void be_double(uint8_t*& p, double v) {
+    uint64_t b;
+    std::memcpy(&b, &v, 8);
+    be_put(p, b, 8);
+}
+static uint64_t be_get(const uint8_t*& p, int nbytes) {
+    uint64_t b = 0;
+    for (int k = 0; k < nbytes; k++) b = (b << 8) | *p++;
+    return b;
+}
+static double be_get_double(const uint8_t*& p) {
+    uint64_t b = be_get(p, 8);
+    double v;
+    std::memcpy(&v, &b, 8);
+    return v;
+}
+static const int RESTART_FIELDS[9] = {0, 16, 12, 1, 2, 3, 17, 18, 19};   // den, den-ave, T, U, V, W, u/v/w-ave
+static const int64_t RESTART_PART_BYTES = 11 * 8 + 4 + 4;
+
+int sf_species_restart_size(sf_ctx* ctx, int species, int64_t* bytes) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp || !bytes) return sf_fail(ctx, SF_ERR_ARG, "sf_species_restart_size: bad argument");
+    *bytes = 4 + 8 + sp->nalive * RESTART_PART_BYTES + 9 * (int64_t)ctx->mp.ni * ctx->mp.nj * 8;
+    return SF_OK;
+}
+
+int sf_species_restart_write(sf_ctx* ctx, int species, uint8_t* buf, int64_t bytes) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    int64_t need = 0;
+    if (!sp || !buf || sf_species_restart_size(ctx, species, &need) != SF_OK || bytes < need)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_species_restart_write: bad argument or buffer shorter than %lld bytes",
+                       (long long)need);
+    const int64_t n = sp->nalive;
+    std::vector<double> x(n), y(n), l0(n), l1(n), u(n), v(n), w(n);
+    std::vector<int> id(sp->desc.track_ids ? n : 0);
+    int rc = sf_particles_download(ctx, species, x.data(), y.data(), l0.data(), l1.data(), u.data(), v.data(), w.data(),
+                                   sp->desc.track_ids ? id.data() : nullptr);
+    if (rc) return rc;
+    uint8_t* p = buf;
+    be_put(p, (uint32_t)sp->num_samples, 4);
+    be_put(p, (uint64_t)n, 8);
+    for (int64_t k = 0; k < n; k++) {
+        // pos[i], vel[i] interleaved; pos[2] (z in XY, theta in RZ) is not kept on the device
+        be_double(p, x[k]);
+        be_double(p, u[k]);
+        be_double(p, y[k]);
+        be_double(p, v[k]);
+        be_double(p, 0.0);
+        be_double(p, w[k]);
+        be_double(p, l0[k]);
+        be_double(p, l1[k]);
+        be_double(p, 0.0);                 // dt: no time left after a completed push
+        be_double(p, sp->desc.spwt);       // mpw
+        be_double(p, sp->mass);
+        be_put(p, 0u, 4);                  // born_it is not tracked
+        be_put(p, (uint32_t)(sp->desc.track_ids ? id[k] : (int)k), 4);
+    }
+    const MeshP& m = ctx->mp;
+    const size_t nn = (size_t)m.ni * m.nj;
+    std::vector<double> f(nn);
+    for (int q = 0; q < 9; q++) {
+        rc = species_field(ctx, sp, RESTART_FIELDS[q], f.data());
+        if (rc) return rc;
+        for (int i = 0; i < m.ni; i++)          // Field2D.binaryWrite: data[i][j], j fastest
+            for (int j = 0; j < m.nj; j++) be_double(p, f[(size_t)j * m.ni + i]);
+    }
+    return SF_OK;
+}
+
+int sf_species_restart_read(sf_ctx* ctx, int species, const uint8_t* buf, int64_t bytes, int64_t* consumed) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp || !buf || bytes < 12) return sf_fail(ctx, SF_ERR_ARG, "sf_species_restart_read: bad argument");
+    const uint8_t* p = buf;
+    const int num_samples = (int)(int32_t)be_get(p, 4);
+    const int64_t n = (int64_t)be_get(p, 8);
+    const MeshP& m = ctx->mp;
+    const int64_t total = 12 + n * RESTART_PART_BYTES + 9 * (int64_t)m.ni * m.nj * 8;
+    if (n < 0 || bytes < total)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_species_restart_read: block of %lld particles needs %lld bytes, got %lld",
+                       (long long)n, (long long)total, (long long)bytes);
+    std::vector<double> x(n), y(n), u(n), v(n), w(n);
+    for (int64_t k = 0; k < n; k++) {
+        x[k] = be_get_double(p);
+        u[k] = be_get_double(p);
+        y[k] = be_get_double(p);
+        v[k] = be_get_double(p);
+        be_get_double(p);                  // pos[2]
+        w[k] = be_get_double(p);
+        p += 8 * 5 + 4 + 4;                // lc (recomputed from pos), dt, mpw, mass, born_it, id
+    }
+    // the fields are overwritten by the next push and sampling; they are skipped
+    p += 9 * (size_t)m.ni * m.nj * 8;
+    // KineticMaterial.loadRestartData adds every particle with addParticle: velocity rewind with
+    // the current E and a new id (KineticMaterial.java:759-802)
+    if (n > 0) {
+        int rc = sf_particles_add(ctx, species, n, x.data(), y.data(), u.data(), v.data(), w.data(), nullptr, 1);
+        if (rc < 0) return rc;
+    }
+    sp->num_samples = num_samples;
+    if (consumed) *consumed = (int64_t)(p - buf);
+    return SF_OK;
+}
+
 int sf_particles_preload(sf_ctx* ctx, int species, int64_t n, const double* drift3, double temp, uint64_t seed) {
     SF_CHECK_CTX(ctx);
     Species* sp = get_species(ctx, species);

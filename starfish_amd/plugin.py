@@ -153,6 +153,21 @@ class KineticHipMaterial:
         self.sim.ctx.call("sf_species_stats", self.index, _dp(out))
         return out
 
+    def saveRestartData(self, out):
+        """KineticMaterial.saveRestartData: appends this species' restart block to a binary file."""
+        n = C.c_int64()
+        self.sim.ctx.call("sf_species_restart_size", self.index, C.byref(n))
+        buf = (C.c_uint8 * n.value)()
+        self.sim.ctx.call("sf_species_restart_write", self.index, buf, n.value)
+        out.write(bytes(buf))
+
+    def loadRestartData(self, data, offset):
+        """KineticMaterial.loadRestartData from data[offset:]; returns the offset after the block."""
+        view = (C.c_uint8 * (len(data) - offset)).from_buffer_copy(data, offset)
+        used = C.c_int64()
+        self.sim.ctx.call("sf_species_restart_read", self.index, view, len(data) - offset, C.byref(used))
+        return offset + used.value
+
     def clear_samples(self):
         self.sim.ctx.call("sf_species_clear_samples", self.index)
 
@@ -328,6 +343,32 @@ class Simulation:
     def set_efield(self, ei, ej):
         self.ctx.call("sf_efield_set", _dp(np.ascontiguousarray(ei, np.float64)),
                       _dp(np.ascontiguousarray(ej, np.float64)))
+
+    def save_restart(self, path):
+        """RestartModule.saveRestartData (core/common/RestartModule.java:120-140): the step, the
+        material count, then each material's block (kinetic GPU species; other materials write
+        none, as the reference's Material base class)."""
+        import struct
+        with open(path, "wb") as f:
+            f.write(struct.pack(">ii", self.it, len(self.deck.materials)))
+            for m in self.materials:
+                if hasattr(m, "saveRestartData"):
+                    m.saveRestartData(f)
+
+    def load_restart(self, path):
+        """RestartModule.loadRestartData (RestartModule.java:145-165): restores the step and the
+        kinetic species; refuses a file written for another material count."""
+        import struct
+        data = open(path, "rb").read()
+        it, nmat = struct.unpack(">ii", data[:8])
+        if nmat != len(self.deck.materials):
+            raise ValueError("incompatible restart file: %d materials, this deck has %d" % (nmat, len(self.deck.materials)))
+        off = 8
+        for m in self.materials:
+            if hasattr(m, "loadRestartData"):
+                off = m.loadRestartData(data, off)
+        self.it = it
+        return it
 
     def set_bfield(self, bi, bj):
         """Loads the domain's magnetic field (DomainModule "bfi"/"bfj"); None removes it."""
