@@ -57,6 +57,9 @@ def parse():
                     help="C2-style runs (random cylinder radius / bias) that provide training snapshots")
     ap.add_argument("--learned-train-steps", type=int, default=600)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--host-sources", action="store_true",
+                    help="sample the boundary injection on the host with the reference's random stream "
+                         "(default: sampled on the device)")
     return ap.parse_args()
 
 
@@ -84,8 +87,7 @@ def learned_guess(sim, deck, args):
         rows = []
         mat = sim.materials[0]
         for _ in range(4):
-            inj = C.c_int64()
-            sim.ctx.call("sf_sources_sample", C.byref(inj))
+            sim.sample_sources()
             mat.update()
             row = {}
             for start in ("zero", "current", "learned"):
@@ -162,7 +164,7 @@ def main():
 
     npart = int(args.particles)
     deck = D.config_c2(steps=args.steps, particles=npart)
-    sim = Simulation(deck, capacity=int(npart * 1.6) + 1_000_000)
+    sim = Simulation(deck, capacity=int(npart * 1.6) + 1_000_000, device_sources=not args.host_sources)
     mat = sim.materials[0]
     nparts, drift, temp = deck.preload
     mat.preload(nparts, drift, temp, seed=1)
@@ -186,8 +188,7 @@ def main():
     t_start = time.perf_counter()
     for _ in range(args.steps):
         npre = mat.getNp()
-        inj = C.c_int64()
-        sim.ctx.call("sf_sources_sample", C.byref(inj))
+        inj = C.c_int64(sim.sample_sources())
         mat.update()
         tm = np.zeros(4)
         sim.ctx.call("sf_last_timings", tm.ctypes.data_as(C.POINTER(C.c_double)))
@@ -261,6 +262,8 @@ def main():
         "config": {"workload": "ES-PIC O+ past a -100 V cylinder, XY 1024x512 nodes dx=1e-3 m, "
                                "%d particles preloaded, Boltzmann electrons n0=1e10 kTe=2 eV, dt=1e-7 s" % npart,
                    "particles_final": mat.getNp(), "state": "int32 cell + f32 offsets + f32 velocity (24 B)",
+                   "injection": ("host, java.util.Random replay" if args.host_sources else
+                                 "sampled on the device (Philox4x32-10)"),
                    "deposit": "int64 fixed point, 7 moment channels", "solver": "Newton + FP32 Jacobi-PCG "
                    "with FP64 residual refinement, rel. residual 1e-8 every step"},
         "roofline": roofline,

@@ -260,6 +260,13 @@ int sf_source_maxwellian(sf_ctx* ctx, int species, int npts, const double* pts, 
                          double v_drift, double temp, int* source);
 /* SourceModule.sampleSources (core/source/SourceModule.java:231-283): every source injects */
 int sf_sources_sample(sf_ctx* ctx, int64_t* injected);
+/* The same injection with the particles sampled on the device (no host sampling or upload):
+   identical particle counts per step, positions along the boundary and drifting-Maxwellian
+   velocities (with Utils.SampleMaxw3D's 1.2 factor and the inward-velocity rejection of
+   MaxwellianSource.sampleParticle) drawn from Philox4x32-10 streams keyed by seed and counted by
+   (particle of the batch, batch, source).  The shared java.util.Random stream is not consumed, so
+   the draws match the reference's in distribution, not value by value. */
+int sf_sources_sample_device(sf_ctx* ctx, uint64_t seed, int64_t* injected);
 
 /* ------------------------------------------------------------------------------------------
  * Timing of the last sf_push_deposit (device ms, with sf_set_timing(ctx, 1)): out4[0] pass A

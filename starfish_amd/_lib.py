@@ -121,6 +121,7 @@ SIGNATURES = {
     "sf_source_maxwellian": (C.c_int, [P, C.c_int, C.c_int, DP, C.c_int, C.c_double, C.c_double, C.c_double,
                                        IP]),
     "sf_sources_sample": (C.c_int, [P, C.POINTER(I64)]),
+    "sf_sources_sample_device": (C.c_int, [P, C.c_uint64, C.POINTER(C.c_int64)]),
     "sf_last_timings": (C.c_int, [P, DP]),
     "sf_last_push_timings": (C.c_int, [P, DP]),
     "sf_set_timing": (C.c_int, [P, C.c_int]),

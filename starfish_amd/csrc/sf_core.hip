@@ -103,6 +103,7 @@ static void free_species(Species* sp) {
 }
 
 static void free_mesh(sf_ctx* c) {
+    for (auto& src : c->sources) dfree(src.d_geom);
     dfree(c->d_face);
     dfree(c->d_nseg_ptr);
     dfree(c->d_nseg_idx);

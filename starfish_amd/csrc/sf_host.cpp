@@ -485,6 +485,30 @@ int sf_source_maxwellian(sf_ctx* ctx, int species, int npts, const double* pts, 
     return SF_OK;
 }
 
+// SourceModule.sampleSources with the particles sampled on the device: the same particle counts
+// (mdot dt / (m spwt) plus the carried remainder), positions and velocities drawn from
+// per-particle Philox streams keyed by seed instead of the shared generator
+int sf_sources_sample_device(sf_ctx* ctx, uint64_t seed, int64_t* injected) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    int64_t total = 0;
+    const uint32_t step = ctx->inject_step++;
+    for (size_t q = 0; q < ctx->sources.size(); q++) {
+        HostSource& s = ctx->sources[q];
+        Species* sp = ctx->species[s.species];
+        double mp = (s.mdot * ctx->dt) / (sp->mass * sp->desc.spwt) + s.rem;
+        int num = (int)mp;
+        s.rem = mp - num;
+        int64_t added = 0;
+        int rc = inject_maxwellian_device(ctx, s, num, seed, step, (uint32_t)q, &added);
+        if (rc) return rc;
+        total += added;
+    }
+    if (injected) *injected = total;
+    return SF_OK;
+}
+
 // SourceModule.sampleSources -> Source.regenerate / sampleKinetic / MaxwellianSource.sampleParticle
 int sf_sources_sample(sf_ctx* ctx, int64_t* injected) {
     SF_CHECK_CTX(ctx);

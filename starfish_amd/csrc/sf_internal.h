@@ -125,6 +125,7 @@ struct HostSource {
     std::vector<double> seg;       // x1,y1,x2,y2 per linear segment
     std::vector<double> cum;
     double area = 0, mdot = 0, v_drift = 0, v_th = 0, rem = 0;
+    double* d_geom = nullptr;      // device copy of seg then cum, for sampling on the device
 };
 
 struct PoissonState;
@@ -165,6 +166,7 @@ struct sf_ctx {
     float2* d_bf32 = nullptr;   // magnetic field, FP32 like E; allocated only while nonzero
     std::vector<sf::Species*> species;
     std::vector<sf::HostSource> sources;
+    uint32_t inject_step = 0;      // batches sampled on the device (counter of their Philox streams)
     sf::PoissonState* poisson = nullptr;
     bool timing = false;
     hipEvent_t ev[6]{};
@@ -211,4 +213,6 @@ int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* d);
 int poisson_solve(sf_ctx* ctx, sf_solve_info* info);
 int operator_apply(sf_ctx* ctx, const double* x, double* y);
 int preload(sf_ctx* ctx, Species* sp, int64_t n, const double* drift, double temp, uint64_t seed);
+int inject_maxwellian_device(sf_ctx* ctx, HostSource& s, int num, uint64_t seed, uint32_t step, uint32_t tag,
+                             int64_t* added);
 }  // namespace sf

@@ -12,6 +12,7 @@
 // Particles are read coalesced in cell-sorted order.  The deposit and the sums are 64-bit
 // fixed point, so every density and moment is independent of the order of the additions.
 #include "sf_particle_math.h"
+#include "sf_rng.h"
 
 namespace sf {
 
@@ -700,6 +701,123 @@ __global__ void __launch_bounds__(256) add_particles_kernel(MeshP m, MeshD md, P
         d.vz[p] = (float)w;
         if (with_ids) d.id[p] = ids ? ids[k] : id0 + (int)k;
     }
+}
+
+// MaxwellianSource.sampleParticle + Source.sampleKinetic + KineticMaterial.addParticle for one
+// batch of a boundary source, sampled on the device (sources/MaxwellianSource.java:93-125,
+// core/source/Source.java:123-198, Spline.randomT for a polyline, Utils.SampleMaxw3D with its
+// 1.2 factor).  Particle k of the batch draws from its own Philox stream (id k, the injection
+// step, the source tag): draw 0 places it along the boundary; attempt a of the velocity uses
+// draws 2 + 38 a .. 37 + 38 a (12 per component), repeated while the velocity does not point
+// into the domain.  Particles outside the mesh become holes (cell -1).
+__global__ void __launch_bounds__(256) inject_maxwellian_kernel(MeshP m, MeshD md, PState d, int64_t base, int n,
+                                                                const double* __restrict__ geom, int nseg, double area,
+                                                                double v_drift, double v_th, double qm,
+                                                                double qm_half_dt, double half_dt, double dt,
+                                                                uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
+                                                                int with_ids, int id0,
+                                                                unsigned long long* __restrict__ accepted) {
+    const double* seg = geom;
+    const double* cum = geom + 4 * nseg;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        ParticleRng rng{k0, k1, (uint32_t)k, step, tag};
+        // Spline.randomT (XY): uniform in length, binary search over the cumulative lengths
+        const double A1 = rng.draw(0) * area;
+        int i1 = 0, i2 = nseg + 1, si;
+        if (A1 < cum[0]) si = -1;
+        else if (A1 > cum[nseg]) si = nseg + 1;
+        else {
+            while (true) {
+                const int mid = (int)(0.5 * (i1 + i2));
+                if (A1 < cum[mid]) i2 = mid;
+                else if (A1 > cum[mid]) i1 = mid;
+                else { i1 = mid; break; }
+                if ((i2 - i1) <= 1) break;
+            }
+            si = i1;
+        }
+        si = min(max(si, 0), nseg - 1);
+        const double frac = (A1 - cum[si]) / (cum[si + 1] - cum[si]);
+        double t = si + frac;
+        int ks = (int)t;
+        double st = t - ks;
+        if (ks > nseg - 1) { ks = nseg - 1; st = 1.0; }
+        const double* g = seg + 4 * ks;
+        const double px = g[0] + st * (g[2] - g[0]), py = g[1] + st * (g[3] - g[1]);
+        const double ddx = g[2] - g[0], ddy = g[3] - g[1], len = sqrt(ddx * ddx + ddy * ddy);
+        const double n0 = -ddy / len, n1 = ddx / len;
+        double vel[3] = {0, 0, 0};
+        const double sh = sqrt(0.5);
+        for (int a = 0; a < 64; a++) {
+            const uint32_t q = 2 + 38 * a;
+            double vm[3];
+            for (int c = 0; c < 3; c++) vm[c] = (v_th * 1.2) * (sh * 1.0 * ((rng.sum12(q + 12 * c) - 0.5 * 12) * 1.0));
+            vel[0] = vm[0] + n0 * v_drift;
+            vel[1] = vm[1] + n1 * v_drift;
+            vel[2] = vm[2];
+            if (n0 * vel[0] + n1 * vel[1] > 0) break;
+        }
+        // Source.sampleKinetic nudge, then KineticMaterial.addParticle
+        const double x = px + vel[0] * 1e-6 * dt, y = py + vel[1] * 1e-6 * dt;
+        const int64_t p = base + k;
+        double lc0 = (x - m.x0[0]) * m.inv_dh[0];
+        double lc1 = (y - m.x0[1]) * m.inv_dh[1];
+        const bool inside = !(lc0 < -FLT_EPS || lc1 < -FLT_EPS || lc0 > (m.ni - 1 + FLT_EPS) || lc1 > (m.nj - 1 + FLT_EPS));
+        if (!inside || !(isfinite(vel[0]) && isfinite(vel[1]) && isfinite(vel[2]))) {
+            d.cell[p] = -1;
+            continue;
+        }
+        if (lc0 >= m.ni) lc0 = m.ni - 1;
+        if (lc1 >= m.nj) lc1 = m.nj - 1;
+        auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
+        if (md.bf) kick<true>(m, md, qm, qm_half_dt, half_dt, load_e, lc0, lc1, vel[0], vel[1], vel[2]);
+        else kick<false>(m, md, qm, qm_half_dt, half_dt, load_e, lc0, lc1, vel[0], vel[1], vel[2]);
+        int ci = (int)lc0, cj = (int)lc1;
+        ci = min(max(ci, 0), m.nci - 1);
+        cj = min(max(cj, 0), m.ncj - 1);
+        d.cell[p] = pack_cell(ci, cj);
+        d.fx[p] = (float)(lc0 - ci);
+        d.fy[p] = (float)(lc1 - cj);
+        d.vx[p] = (float)vel[0];
+        d.vy[p] = (float)vel[1];
+        d.vz[p] = (float)vel[2];
+        if (with_ids) d.id[p] = id0 + k;
+        atomicAdd(accepted, 1ull);
+    }
+}
+
+int inject_maxwellian_device(sf_ctx* ctx, HostSource& s, int num, uint64_t seed, uint32_t step, uint32_t tag,
+                             int64_t* added) {
+    *added = 0;
+    if (num <= 0) return SF_OK;
+    Species* sp = ctx->species[s.species];
+    if (sp->np + num > sp->capacity)
+        return sf_fail(ctx, SF_ERR_OOM, "species capacity %lld exceeded (have %lld, injecting %d)",
+                       (long long)sp->capacity, (long long)sp->np, num);
+    const int nseg = (int)s.seg.size() / 4;
+    if (!s.d_geom) {
+        std::vector<double> g(s.seg);
+        g.insert(g.end(), s.cum.begin(), s.cum.end());
+        SF_HIP(ctx, hipMalloc((void**)&s.d_geom, g.size() * sizeof(double)));
+        SF_HIP(ctx, hipMemcpy(s.d_geom, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    unsigned long long* d_acc = nullptr;
+    SF_HIP(ctx, hipMallocAsync((void**)&d_acc, sizeof(unsigned long long), ctx->stream));
+    SF_HIP(ctx, hipMemsetAsync(d_acc, 0, sizeof(unsigned long long), ctx->stream));
+    hipLaunchKernelGGL(inject_maxwellian_kernel, dim3(std::min(4096, (num + 255) / 256)), dim3(256), 0, ctx->stream, ctx->mp,
+                       ctx->meshd(), sp->buf[sp->cur], sp->np, num, (const double*)s.d_geom, nseg, s.area, s.v_drift,
+                       s.v_th, sp->qm, -0.5 * ctx->dt * sp->qm, -0.5 * ctx->dt, ctx->dt, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), step, tag, sp->desc.track_ids, sp->id_counter, d_acc);
+    SF_HIP(ctx, hipGetLastError());
+    unsigned long long acc = 0;
+    SF_HIP(ctx, hipMemcpyAsync(&acc, d_acc, sizeof(acc), hipMemcpyDeviceToHost, ctx->stream));
+    SF_HIP(ctx, hipFreeAsync(d_acc, ctx->stream));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    sp->id_counter += num;
+    sp->np += num;             // rejected slots are holes until the next sort
+    sp->nalive += (int64_t)acc;
+    *added = (int64_t)acc;
+    return SF_OK;
 }
 
 // counter-based generator for the preload: splitmix64 finaliser over (seed, particle, draw)

@@ -257,8 +257,12 @@ class HipPlugin:
 class Simulation:
     """Starfish.start + MainLoop over the registered GPU types, driven from a Deck."""
 
-    def __init__(self, deck: Deck, device=0, capacity=None, track_ids=False, vmax=0.0, plugins=None):
+    def __init__(self, deck: Deck, device=0, capacity=None, track_ids=False, vmax=0.0, plugins=None,
+                 device_sources=False):
+        # device_sources: sample boundary injection on the device (sf_sources_sample_device) instead
+        # of replaying the reference's java.util.Random stream on the host
         self.deck = deck
+        self.device_sources = device_sources
         self.registry = Registry()
         for p in (plugins or [HipPlugin()]):
             p.register(self.registry)
@@ -313,9 +317,16 @@ class Simulation:
     def initial_solve(self):
         return self.update_fields()
 
-    def step(self):                                 # Starfish.MainLoop body
+    def sample_sources(self):                       # SourceModule.sampleSources
         injected = C.c_int64()
-        self.ctx.call("sf_sources_sample", C.byref(injected))
+        if self.device_sources:
+            self.ctx.call("sf_sources_sample_device", C.c_uint64(self.deck.seed), C.byref(injected))
+        else:
+            self.ctx.call("sf_sources_sample", C.byref(injected))
+        return injected.value
+
+    def step(self):                                 # Starfish.MainLoop body
+        self.sample_sources()
         for m in self.materials:
             m.update()
         self.update_fields()
