@@ -264,7 +264,7 @@ def main():
                    "particles_final": mat.getNp(), "state": "int32 cell + f32 offsets + f32 velocity (24 B)",
                    "injection": ("host, java.util.Random replay" if args.host_sources else
                                  "sampled on the device (Philox4x32-10)"),
-                   "deposit": "int64 fixed point, 7 moment channels", "solver": "Newton + FP32 Jacobi-PCG "
+                   "deposit": "int64 fixed point, 7 moment channels", "solver": "Newton + FP32 PCG (block red-black SGS preconditioner) "
                    "with FP64 residual refinement, rel. residual 1e-8 every step"},
         "roofline": roofline,
         "cpu_baseline": cpu,
