@@ -87,7 +87,7 @@ struct Species {
     int64_t capacity = 0;
     int64_t np = 0, nsorted = 0;   // slots in use (incl. dead holes between sorts), sorted prefix
     int64_t nalive = 0;            // particles alive after the last push or add
-    int sort_interval = 4;         // steps between counting sorts; other steps push in place
+    int sort_interval = 3;         // steps between counting sorts; other steps push in place
     int since_sort = 0;
     int cur = 0;
     PState buf[2]{};
