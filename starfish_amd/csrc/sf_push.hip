@@ -709,7 +709,8 @@ __global__ void __launch_bounds__(256) add_particles_kernel(MeshP m, MeshD md, P
 // 1.2 factor).  Particle k of the batch draws from its own Philox stream (id k, the injection
 // step, the source tag): draw 0 places it along the boundary; attempt a of the velocity uses
 // draws 2 + 38 a .. 37 + 38 a (12 per component), repeated while the velocity does not point
-// into the domain.  Particles outside the mesh become holes (cell -1).
+// into the domain.  Particles outside the mesh, or still without an inward velocity after 64
+// attempts (a source whose drift points out of the domain), become holes (cell -1).
 __global__ void __launch_bounds__(256) inject_maxwellian_kernel(MeshP m, MeshD md, PState d, int64_t base, int n,
                                                                 const double* __restrict__ geom, int nseg, double area,
                                                                 double v_drift, double v_th, double qm,
@@ -748,14 +749,15 @@ __global__ void __launch_bounds__(256) inject_maxwellian_kernel(MeshP m, MeshD m
         const double n0 = -ddy / len, n1 = ddx / len;
         double vel[3] = {0, 0, 0};
         const double sh = sqrt(0.5);
-        for (int a = 0; a < 64; a++) {
+        bool inward = false;
+        for (int a = 0; a < 64 && !inward; a++) {   // the reference repeats without a limit
             const uint32_t q = 2 + 38 * a;
             double vm[3];
             for (int c = 0; c < 3; c++) vm[c] = (v_th * 1.2) * (sh * 1.0 * ((rng.sum12(q + 12 * c) - 0.5 * 12) * 1.0));
             vel[0] = vm[0] + n0 * v_drift;
             vel[1] = vm[1] + n1 * v_drift;
             vel[2] = vm[2];
-            if (n0 * vel[0] + n1 * vel[1] > 0) break;
+            inward = n0 * vel[0] + n1 * vel[1] > 0;
         }
         // Source.sampleKinetic nudge, then KineticMaterial.addParticle
         const double x = px + vel[0] * 1e-6 * dt, y = py + vel[1] * 1e-6 * dt;
@@ -763,7 +765,7 @@ __global__ void __launch_bounds__(256) inject_maxwellian_kernel(MeshP m, MeshD m
         double lc0 = (x - m.x0[0]) * m.inv_dh[0];
         double lc1 = (y - m.x0[1]) * m.inv_dh[1];
         const bool inside = !(lc0 < -FLT_EPS || lc1 < -FLT_EPS || lc0 > (m.ni - 1 + FLT_EPS) || lc1 > (m.nj - 1 + FLT_EPS));
-        if (!inside || !(isfinite(vel[0]) && isfinite(vel[1]) && isfinite(vel[2]))) {
+        if (!inward || !inside || !(isfinite(vel[0]) && isfinite(vel[1]) && isfinite(vel[2]))) {
             d.cell[p] = -1;
             continue;
         }
