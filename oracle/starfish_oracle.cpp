@@ -11,7 +11,16 @@
  *     only the summation order of a row, i.e. the last bits),
  *   - the threaded GS sweep is run with its chunks in order (the reference's
  *     unsynchronised chunks make its iterates timing dependent; the discrete
- *     solution is the parity target).
+ *     solution is the parity target),
+ *   - MCC and DSMC draw from Philox4x32-10 streams (per particle id, per cell)
+ *     instead of Starfish.rnd(), and DSMC cell lists are ordered by particle
+ *     id; the generator is pinned by its published known-answer vectors and
+ *     the collision passes by closed forms (tests/test_oracle_mcc.py,
+ *     tests/test_oracle_dsmc.py).
+ * Pins of every restated function: tests/test_oracle_*.py.  Parity unpinned:
+ * the glibc/JDK math functions above and the Monte-Carlo interface node
+ * volumes (reproduced draw for draw from the replica generator, never run
+ * against Java).
  */
 #include "starfish_oracle.h"
 
