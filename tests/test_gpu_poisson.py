@@ -54,7 +54,7 @@ def tight_solution(o, geo, rho, solver):
 @pytest.mark.gpu
 def test_operator_and_gradient_match_assembled_matrices():
     from starfish_amd.plugin import Simulation, _dp
-    for deck in (D.config_c1(), _small_rz_deck()):
+    for deck in (D.config_c1(), _small_rz_deck(), _small_zr_deck()):
         geo, st = geometry(deck)
         o = oracle_for(deck, geo, st)
         A = o.operator(0)
@@ -88,19 +88,31 @@ def _small_rz_deck():
                   solver=D.SolverSpec("poisson_hip", "GS", False, 1e12, 2.0, 0.0), num_it=1, dt=1e-8, seed=1)
 
 
+def _small_zr_deck():
+    """The RZ deck with the axes exchanged: z along i, r along j (axis on the bottom face)."""
+    mesh = D.MeshSpec("ZR", 49, 65, (0.0, 0.0), (1e-3, 1e-3),
+                      {"BOTTOM": ("SYMMETRY", 0.0), "LEFT": ("DIRICHLET", 0.0), "RIGHT": ("DIRICHLET", -50.0)})
+    pts = [(0.02, 0.02), (0.03, 0.02), (0.03, 0.025), (0.02, 0.025), (0.02, 0.02)]
+    return D.Deck(mesh,
+                  boundaries=[D.BoundarySpec("electrode", "DIRICHLET", -20.0, [(y, x) for x, y in pts][::-1], True,
+                                             "SS")],
+                  materials=[D.MaterialSpec("Xe+", "kinetic_hip", 131.0, 1.0, 1e4, 0.0)],
+                  solver=D.SolverSpec("poisson_hip", "GS", False, 1e12, 2.0, 0.0), num_it=1, dt=1e-8, seed=1)
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", ["c1", "rz"])
+@pytest.mark.parametrize("which", ["c1", "rz", "zr"])
 def test_potential_matches_tight_reference_solution(which):
     from starfish_amd.plugin import Simulation, _dp
-    deck = D.config_c1() if which == "c1" else _small_rz_deck()
+    deck = {"c1": D.config_c1, "rz": _small_rz_deck, "zr": _small_zr_deck}[which]()
     geo, st = geometry(deck)
     o = oracle_for(deck, geo, st)
     m = deck.mesh
-    # ion blob of up to 1e11 m^-3 (XY) / 1e13 (RZ) downstream of the object: plain Newton from
+    # ion blob of up to 1e11 m^-3 (XY) / 1e13 (RZ, ZR) downstream of the object: plain Newton from
     # phi = 0 converges for these (the reference's solver has no damping)
     X = m.x0[0] + (np.arange(m.ni * m.nj) % m.ni) * m.dh[0]
     Y = m.x0[1] + (np.arange(m.ni * m.nj) // m.ni) * m.dh[1]
-    cx, cy = (0.25, 0.1) if which == "c1" else (0.04, 0.03)
+    cx, cy = {"c1": (0.25, 0.1), "rz": (0.04, 0.03), "zr": (0.03, 0.04)}[which]
     w = 0.1 if which == "c1" else 0.01
     peak = 1e11 if which == "c1" else 1e13
     rho = QE * peak * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / w ** 2)

@@ -9,21 +9,27 @@ from starfish_amd import deck as D
 TOL = 1e-5
 
 
-def rz_deck():
-    mesh = D.MeshSpec("RZ", 129, 97, (0.0, 0.0), (5e-4, 5e-4),
-                      {"LEFT": ("SYMMETRY", 0.0), "BOTTOM": ("DIRICHLET", 0.0), "TOP": ("DIRICHLET", -40.0)})
+def rz_deck(domain="RZ"):
+    """RZ: r along i (axis on the left face); ZR: the same geometry with the axes exchanged."""
+    pts = [(0.03, 0.02), (0.04, 0.02), (0.04, 0.025), (0.03, 0.025), (0.03, 0.02)]
+    if domain == "RZ":
+        mesh = D.MeshSpec("RZ", 129, 97, (0.0, 0.0), (5e-4, 5e-4),
+                          {"LEFT": ("SYMMETRY", 0.0), "BOTTOM": ("DIRICHLET", 0.0), "TOP": ("DIRICHLET", -40.0)})
+    else:
+        mesh = D.MeshSpec("ZR", 97, 129, (0.0, 0.0), (5e-4, 5e-4),
+                          {"BOTTOM": ("SYMMETRY", 0.0), "LEFT": ("DIRICHLET", 0.0), "RIGHT": ("DIRICHLET", -40.0)})
+        pts = [(y, x) for x, y in pts][::-1]
     return D.Deck(mesh,
-                  boundaries=[D.BoundarySpec("electrode", "DIRICHLET", -20.0,
-                                             [(0.03, 0.02), (0.04, 0.02), (0.04, 0.025), (0.03, 0.025),
-                                              (0.03, 0.02)], True, "SS")],
+                  boundaries=[D.BoundarySpec("electrode", "DIRICHLET", -20.0, pts, True, "SS")],
                   materials=[D.MaterialSpec("Xe+", "kinetic_hip", 131.0, 1.0, 1e5, 0.0)],
                   solver=D.SolverSpec("poisson_hip", "GS", False, 1e12, 2.0, 0.0), num_it=1, dt=2e-8, seed=1)
 
 
 @pytest.mark.gpu
-def test_rz_push_deposit_matches_oracle():
+@pytest.mark.parametrize("domain", ["RZ", "ZR"])
+def test_rz_push_deposit_matches_oracle(domain):
     from starfish_amd.plugin import Simulation
-    deck = rz_deck()
+    deck = rz_deck(domain)
     geo, st = geometry(deck)
     o = oracle_for(deck, geo, st)
     o.initial_solve()

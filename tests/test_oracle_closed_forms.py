@@ -133,6 +133,57 @@ def test_rz_rotation_conserves_speed_and_radius_formula():
     assert np.isclose(p["pos"][0, 1], z0 + v0[1] * dt, rtol=1e-14)
 
 
+def test_zr_is_the_transposed_rz_case():
+    """ZR (z along i, r along j) is RZ with the axes exchanged: on mirrored decks the node
+    volumes, node types, potential, field, pushed particles (velocity components exchanged, the
+    azimuthal one kept) and deposited densities are the transposes of the RZ ones.  A ZR branch
+    that rotates or weighs by the wrong coordinate fails this."""
+    pts = [(0.02, 0.02), (0.03, 0.02), (0.03, 0.025), (0.02, 0.025), (0.02, 0.02)]
+    decks = {
+        "RZ": box_deck("RZ", ni=41, nj=29, dx=1e-3, dy=1e-3, bc={"LEFT": ("SYMMETRY", 0), "TOP": ("DIRICHLET", -30.0)},
+                       boundaries=[D.BoundarySpec("el", "DIRICHLET", -20.0, pts, True, "SS")], dt=2e-8),
+        "ZR": box_deck("ZR", ni=29, nj=41, dx=1e-3, dy=1e-3,
+                       bc={"BOTTOM": ("SYMMETRY", 0), "RIGHT": ("DIRICHLET", -30.0)},
+                       boundaries=[D.BoundarySpec("el", "DIRICHLET", -20.0, [(y, x) for x, y in pts][::-1], True,
+                                                  "SS")], dt=2e-8),
+    }
+    rng = np.random.default_rng(4)
+    n = 400
+    rz = np.stack([rng.uniform(0.001, 0.015, n), rng.uniform(0.001, 0.027, n)], 1)
+    vel = rng.normal(size=(n, 3)) * 3000.0
+    out = {}
+    for dom, deck in decks.items():
+        geo, st = geometry(deck)
+        o = oracle_for(deck, geo, st)
+        o.initial_solve()
+        if dom == "RZ":
+            load(o, rz, vel)
+        else:
+            load(o, rz[:, ::-1], vel[:, [1, 0, 2]])
+        for _ in range(3):
+            o.move()
+        o.update_species_fields()
+        m = deck.mesh
+        grid = lambda a: np.asarray(a).reshape(m.nj, m.ni)
+        p = o.particles(0)
+        k = np.argsort(p["id"])
+        out[dom] = dict(vol=grid(geo.node_vol), nt=grid(geo.node_type), phi=grid(o.field("phi")),
+                        efi=grid(o.field("efi")), efj=grid(o.field("efj")), nd=grid(o.field("nd", 0)),
+                        id=p["id"][k], pos=p["pos"][k], vel=p["vel"][k])
+    a, b = out["RZ"], out["ZR"]
+    assert np.array_equal(a["nt"], b["nt"].T)
+    assert np.allclose(a["vol"], b["vol"].T, rtol=1e-13, atol=0)
+    assert np.allclose(a["phi"], b["phi"].T, rtol=0, atol=1e-9)
+    emax = np.abs(a["efi"]).max()
+    assert emax > 1e3
+    assert np.abs(a["efi"] - b["efj"].T).max() < 1e-9 * emax and np.abs(a["efj"] - b["efi"].T).max() < 1e-9 * emax
+    assert len(a["id"]) > n // 2 and np.array_equal(a["id"], b["id"])
+    assert np.allclose(a["pos"][:, :2], b["pos"][:, [1, 0]], rtol=0, atol=1e-12)
+    assert np.allclose(a["vel"], b["vel"][:, [1, 0, 2]], rtol=1e-9, atol=1e-9)
+    assert a["nd"].max() > 0
+    assert np.allclose(a["nd"], b["nd"].T, rtol=1e-9, atol=1e-9 * a["nd"].max())
+
+
 def test_symmetry_face_mirrors_and_open_face_removes():
     deck = box_deck("XY", ni=21, nj=11, bc={"BOTTOM": ("SYMMETRY", 0)})
     geo, _ = geometry(deck)
