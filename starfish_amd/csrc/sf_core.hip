@@ -311,7 +311,7 @@ int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species) {
     if (!rc) rc = dalloc(ctx, &sp->offsets, ncells + 1);
     if (!rc) rc = dalloc(ctx, &sp->cursor, ncells);
     if (!rc) rc = dalloc(ctx, &sp->blocksum, ncells / 4096 + 2);
-    if (!rc) rc = dalloc(ctx, &sp->stats, 8);
+    if (!rc) rc = dalloc(ctx, &sp->stats, (size_t)STAT_SLOTS * STAT_STRIDE);
     if (rc) {
         free_species(sp);
         return rc;

@@ -33,6 +33,11 @@ constexpr int NCH = 7;                          // deposit channels: n, nu, nv, 
 constexpr int PUSH_THREADS = 256;
 constexpr int CHUNK = 256;                      // particles staged in LDS per round
 constexpr int CHUNK_STRIDE = 37;                // odd stride of the lane->particle permutation
+// Mover sums are added into STAT_SLOTS copies (by workgroup index) of STAT_STRIDE int64 each, one
+// 128-byte line apart, so that the atomics of thousands of waves do not queue on five addresses.
+// Slot 0, entry 7 is the tile counter of the persistent push kernels.
+constexpr int STAT_SLOTS = 64;
+constexpr int STAT_STRIDE = 16;
 
 // Particle state, structure of arrays, one buffer.
 struct PState {
@@ -100,7 +105,7 @@ struct Species {
     int32_t* offsets = nullptr;    // [ncells+1] exclusive scan (cell ranges of the sorted array)
     int32_t* cursor = nullptr;     // [ncells]
     int32_t* blocksum = nullptr;   // scan scratch
-    long long* stats = nullptr;    // [8] int64 fixed-point mover sums
+    long long* stats = nullptr;    // [STAT_SLOTS][STAT_STRIDE] int64 fixed-point mover sums
     int id_counter = 0;
     int num_samples = 0;
     double kscale[NCH];

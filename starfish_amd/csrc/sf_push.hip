@@ -92,6 +92,8 @@ __device__ __forceinline__ long long wave_sum_ll(long long v) {
     return v;
 }
 
+// Wave sums of the mover sums, added into the workgroup's slot of the stats array (the host adds
+// the slots).  A single copy would serialise the atomics of every wave of the launch at its end.
 __device__ void stat_flush(StatAcc s, long long* stats) {
     s.n = wave_sum_ll(s.n);
     s.p0 = wave_sum_ll(s.p0);
@@ -99,11 +101,12 @@ __device__ void stat_flush(StatAcc s, long long* stats) {
     s.p2 = wave_sum_ll(s.p2);
     s.e = wave_sum_ll(s.e);
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd((unsigned long long*)&stats[0], (unsigned long long)s.n);
-        atomicAdd((unsigned long long*)&stats[1], (unsigned long long)s.p0);
-        atomicAdd((unsigned long long*)&stats[2], (unsigned long long)s.p1);
-        atomicAdd((unsigned long long*)&stats[3], (unsigned long long)s.p2);
-        atomicAdd((unsigned long long*)&stats[4], (unsigned long long)s.e);
+        unsigned long long* slot = (unsigned long long*)stats + (size_t)(blockIdx.x % STAT_SLOTS) * STAT_STRIDE;
+        atomicAdd(&slot[0], (unsigned long long)s.n);
+        atomicAdd(&slot[1], (unsigned long long)s.p0);
+        atomicAdd(&slot[2], (unsigned long long)s.p1);
+        atomicAdd(&slot[3], (unsigned long long)s.p2);
+        atomicAdd(&slot[4], (unsigned long long)s.e);
     }
 }
 
@@ -158,9 +161,15 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
     StatAcc st{0, 0, 0, 0, 0};
     // Persistent workgroups take tiles from a counter until none is left, so tiles that
     // take longer (bounces near objects, particles that left the LDS window) do not leave the
-    // rest of the device idle at the end of the launch.
+    // rest of the device idle at the end of the launch.  A workgroup's first tile is its own
+    // index; each later one is requested from the counter one tile ahead, so that the atomic's
+    // latency overlaps the work of the current tile.
+    int next_tile = blockIdx.x;   // used by thread 0
     for (;;) {
-    if (t == 0) s_tile = (int)atomicAdd(tile_counter, 1u);
+    if (t == 0) {
+        s_tile = next_tile;
+        next_tile = (int)gridDim.x + (int)atomicAdd(tile_counter, 1u);
+    }
     __syncthreads();
     const int tile = s_tile;
     if (tile >= ntiles) break;
@@ -973,7 +982,7 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     const bool sort_step = (sp->since_sort + 1 >= sp->sort_interval) || sp->nsorted == 0 ||
                            (sp->np - sp->nsorted) > sp->np / 8;
     const int64_t np_before = sp->np, nsorted_before = sp->nsorted;
-    SF_HIP(ctx, hipMemsetAsync(sp->stats, 0, 8 * sizeof(long long), ctx->stream));
+    SF_HIP(ctx, hipMemsetAsync(sp->stats, 0, (size_t)STAT_SLOTS * STAT_STRIDE * sizeof(long long), ctx->stream));
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     const bool hb = md.bf != nullptr;
     const dim3 tg(tile_grid(ctx, !sort_step, hb)), tb(PUSH_THREADS);
@@ -1052,12 +1061,14 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     sp->num_samples++;
     // mover sums (KineticMaterial.java:247-259); the count is the number of survivors
-    long long st[8];
-    SF_HIP(ctx, hipMemcpyAsync(st, sp->stats, 8 * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+    long long slots[STAT_SLOTS * STAT_STRIDE], st[5] = {0, 0, 0, 0, 0};
+    SF_HIP(ctx, hipMemcpyAsync(slots, sp->stats, sizeof(slots), hipMemcpyDeviceToHost, ctx->stream));
     int64_t total = 0;
     if (sort_step) SF_HIP(ctx, hipMemcpyAsync(&total, sp->offsets + ncells, sizeof(int32_t), hipMemcpyDeviceToHost,
                                               ctx->stream));
     SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < STAT_SLOTS; k++)
+        for (int q = 0; q < 5; q++) st[q] += slots[k * STAT_STRIDE + q];
     double inv_kv = 1.0 / sp->kscale[1];
     sp->stats_host[0] = (double)st[0] * sp->desc.spwt * sp->mass;
     sp->stats_host[1] = (double)st[1] * inv_kv * sp->desc.spwt * sp->mass;
