@@ -21,7 +21,7 @@ constexpr double PI = 3.14159265358979323846;
 
 // Tile of cells owned by one workgroup of the push/deposit pass.
 constexpr int TILE_W = 32;                      // cells along i
-constexpr int TILE_H = 8;                       // cells along j
+constexpr int TILE_H = 4;                       // cells along j
 constexpr int TILE_HALO = 2;                    // cells of halo on every side (drift between sorts)
 constexpr int TILE_NW = TILE_W + 1 + 2 * TILE_HALO;   // nodes along i
 constexpr int TILE_NH = TILE_H + 1 + 2 * TILE_HALO;
@@ -35,7 +35,6 @@ constexpr int CHUNK = 256;                      // particles staged in LDS per r
 constexpr int CHUNK_STRIDE = 37;                // odd stride of the lane->particle permutation
 // Mover sums are added into STAT_SLOTS copies (by workgroup index) of STAT_STRIDE int64 each, one
 // 128-byte line apart, so that the atomics of thousands of waves do not queue on five addresses.
-// Slot 0, entry 7 is the tile counter of the persistent push kernels.
 constexpr int STAT_SLOTS = 64;
 constexpr int STAT_STRIDE = 16;
 
@@ -140,9 +139,7 @@ void poisson_free(PoissonState* p);
 
 struct sf_ctx {
     int device = 0;
-    int ncu = 0;                    // compute units of the device (persistent-grid sizing)
-    int grid_inplace = 0, grid_sort = 0;   // resident workgroups of the persistent tile kernels
-    int grid_inplace_b = 0, grid_sort_b = 0;   // the same with a magnetic field
+    int ncu = 0;                    // compute units of the device
     hipStream_t stream = nullptr;
     std::string err;
     double dt = 1e-7;

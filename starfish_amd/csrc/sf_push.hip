@@ -154,25 +154,13 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
                                                        unsigned long long* __restrict__ acc,
                                                        int32_t* __restrict__ counts, long long* stats) {
     __shared__ TileLds L;
-    __shared__ int s_tile;
     const int t = threadIdx.x;
-    const int ntiles = m.ntx * m.nty;
-    unsigned int* tile_counter = (unsigned int*)&stats[7];
     StatAcc st{0, 0, 0, 0, 0};
-    // Persistent workgroups take tiles from a counter until none is left, so tiles that
-    // take longer (bounces near objects, particles that left the LDS window) do not leave the
-    // rest of the device idle at the end of the launch.  A workgroup's first tile is its own
-    // index; each later one is requested from the counter one tile ahead, so that the atomic's
-    // latency overlaps the work of the current tile.
-    int next_tile = blockIdx.x;   // used by thread 0
-    for (;;) {
-    if (t == 0) {
-        s_tile = next_tile;
-        next_tile = (int)gridDim.x + (int)atomicAdd(tile_counter, 1u);
-    }
-    __syncthreads();
-    const int tile = s_tile;
-    if (tile >= ntiles) break;
+    // One workgroup per tile.  The dispatcher starts a waiting tile as soon as a workgroup slot
+    // frees up, so tiles that take longer (bounces near objects, particles that left the LDS
+    // window) do not leave the rest of the device idle at the end of the launch; this balances
+    // better than persistent workgroups taking tiles from a counter.
+    const int tile = blockIdx.x;
     const int tx = tile % m.ntx, ty = tile / m.ntx;
     const int ti0 = tx * TILE_W, tj0 = ty * TILE_H;            // first cell of the tile
     const int n_i0 = ti0 - TILE_HALO, n_j0 = tj0 - TILE_HALO;  // first node of the LDS tile
@@ -361,12 +349,10 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
         int ci = c_i0 + k % TILE_CW, cj = c_j0 + k / TILE_CW;
         atomicAdd(&counts[cj * m.nci + ci], h);
     }
-    __syncthreads();   // LDS and s_tile are reused by the next tile
-    }
     stat_flush(st, stats);
 }
 
-// In-place step: four workgroups per CU (39 KB of LDS each, at most 128 VGPRs).
+// In-place step: four workgroups per CU (29 KB of LDS each, at most 128 VGPRs).
 __global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile_inplace(MeshP m, MeshD md, PushC pc, PState s,
                                                                            const int32_t* __restrict__ offsets,
                                                                            unsigned long long* __restrict__ acc,
@@ -729,6 +715,7 @@ __global__ void __launch_bounds__(256) inject_maxwellian_kernel(MeshP m, MeshD m
                                                                 unsigned long long* __restrict__ accepted) {
     const double* seg = geom;
     const double* cum = geom + 4 * nseg;
+    unsigned long long n_ok = 0;   // added to accepted once per wave
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         ParticleRng rng{k0, k1, (uint32_t)k, step, tag};
         // Spline.randomT (XY): uniform in length, binary search over the cumulative lengths
@@ -793,8 +780,10 @@ __global__ void __launch_bounds__(256) inject_maxwellian_kernel(MeshP m, MeshD m
         d.vy[p] = (float)vel[1];
         d.vz[p] = (float)vel[2];
         if (with_ids) d.id[p] = id0 + k;
-        atomicAdd(accepted, 1ull);
+        n_ok++;
     }
+    for (int o = 32; o > 0; o >>= 1) n_ok += __shfl_xor(n_ok, o);
+    if ((threadIdx.x & 63) == 0 && n_ok) atomicAdd(accepted, n_ok);
 }
 
 int inject_maxwellian_device(sf_ctx* ctx, HostSource& s, int num, uint64_t seed, uint32_t step, uint32_t tag,
@@ -954,20 +943,8 @@ int sort_species(sf_ctx* ctx, Species* sp) {
     return SF_OK;
 }
 
-// persistent grid of a tile kernel: the workgroups that are resident at once (occupancy x CUs),
-// at most one per tile
-static int tile_grid(sf_ctx* ctx, bool inplace, bool has_b) {
-    int& g = has_b ? (inplace ? ctx->grid_inplace_b : ctx->grid_sort_b) : (inplace ? ctx->grid_inplace : ctx->grid_sort);
-    if (g == 0) {
-        int per_cu = 0;
-        const void* k = has_b ? (inplace ? (const void*)push_deposit_tile_inplace_b : (const void*)push_deposit_tile_sort_b)
-                              : (inplace ? (const void*)push_deposit_tile_inplace : (const void*)push_deposit_tile_sort);
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PUSH_THREADS, 0);
-        if (e != hipSuccess || per_cu <= 0) per_cu = 2;
-        g = per_cu * std::max(1, ctx->ncu);
-    }
-    return std::max(1, std::min(ctx->mp.ntx * ctx->mp.nty, g));
-}
+// one workgroup per tile
+static int tile_grid(const sf_ctx* ctx) { return std::max(1, ctx->mp.ntx * ctx->mp.nty); }
 
 int push_deposit(sf_ctx* ctx, Species* sp) {
     if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
@@ -985,7 +962,7 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     SF_HIP(ctx, hipMemsetAsync(sp->stats, 0, (size_t)STAT_SLOTS * STAT_STRIDE * sizeof(long long), ctx->stream));
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     const bool hb = md.bf != nullptr;
-    const dim3 tg(tile_grid(ctx, !sort_step, hb)), tb(PUSH_THREADS);
+    const dim3 tg(tile_grid(ctx)), tb(PUSH_THREADS);
     const dim3 g_tail(grid_for(sp->np - sp->nsorted, 256, 4096)), b256(256);
     if (sort_step) {
         if (sp->nsorted > 0) {
