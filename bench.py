@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--particles", type=float, default=2e7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--sort-interval", type=int, default=3)
+    ap.add_argument("--sort-interval", type=int, default=4)
     ap.add_argument("--learned-configs", type=int, default=8,
                     help="C2-style runs (random cylinder radius / bias) that provide training snapshots")
     ap.add_argument("--learned-train-steps", type=int, default=600)

@@ -184,7 +184,7 @@ int sf_push_deposit(sf_ctx* ctx, int species);
 /* Steps between per-cell counting sorts (KineticMaterial.sortParticlesToCells,
    KineticMaterial.java:1150-1179, orders the particles only for merging; here the order serves
    the tiled deposit).  Steps in between push in place and leave dead particles as holes that the
-   next sort compacts.  Results do not depend on the interval.  Default 3, minimum 1. */
+   next sort compacts.  Results do not depend on the interval.  Default 4, minimum 1. */
 int sf_species_set_sort_interval(sf_ctx* ctx, int species, int interval);
 /* which: 0 nd, 1 u, 2 v, 3 w (instantaneous, KineticMaterial.updateFields);
    4 count-sum, 5 u-sum, 6 v-sum, 7 w-sum, 8 uu-sum, 9 vv-sum, 10 ww-sum, 11 mpc-sum

@@ -22,7 +22,7 @@ constexpr double PI = 3.14159265358979323846;
 // Tile of cells owned by one workgroup of the push/deposit pass.
 constexpr int TILE_W = 32;                      // cells along i
 constexpr int TILE_H = 4;                       // cells along j
-constexpr int TILE_HALO = 2;                    // cells of halo on every side (drift between sorts)
+constexpr int TILE_HALO = 3;                    // cells of halo on every side (drift between sorts)
 constexpr int TILE_NW = TILE_W + 1 + 2 * TILE_HALO;   // nodes along i
 constexpr int TILE_NH = TILE_H + 1 + 2 * TILE_HALO;
 constexpr int TILE_NODES = TILE_NW * TILE_NH;
@@ -91,7 +91,7 @@ struct Species {
     int64_t capacity = 0;
     int64_t np = 0, nsorted = 0;   // slots in use (incl. dead holes between sorts), sorted prefix
     int64_t nalive = 0;            // particles alive after the last push or add
-    int sort_interval = 3;         // steps between counting sorts; other steps push in place
+    int sort_interval = 4;         // steps between counting sorts; other steps push in place
     int since_sort = 0;
     int cur = 0;
     PState buf[2]{};

@@ -352,7 +352,7 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
     stat_flush(st, stats);
 }
 
-// In-place step: four workgroups per CU (29 KB of LDS each, at most 128 VGPRs).
+// In-place step: four workgroups per CU (35 KB of LDS each, at most 128 VGPRs).
 __global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile_inplace(MeshP m, MeshD md, PushC pc, PState s,
                                                                            const int32_t* __restrict__ offsets,
                                                                            unsigned long long* __restrict__ acc,
