@@ -535,21 +535,29 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_write(int32_t* __restrict__
 // ----------------------------------------------------------------------------------------
 // pass B: recompute the push, write each particle to its slot in the new cell order
 // ----------------------------------------------------------------------------------------
+// Slot of each valid lane in its cell: the lanes of one key are grouped by ballots, the
+// lowest lane of each group reserves the group's slots with one atomic on the cell cursor, and
+// the other lanes take consecutive slots after it.  The atomics of all groups of the wave are
+// issued together, so the wave waits for one round trip however many cells it holds.
 __device__ __forceinline__ int64_t wave_rank(bool valid, int key, int32_t* cursor) {
     const int lane = threadIdx.x & 63;
     unsigned long long active = __ballot(valid);
-    int64_t dst = -1;
+    int my_leader = lane;
+    unsigned long long my_group = 0ull;
     while (active) {
-        int leader = __ffsll((long long)active) - 1;
-        int lk = __shfl(key, leader);
-        unsigned long long mm = __ballot(valid && key == lk);
-        int b = 0;
-        if (lane == leader) b = atomicAdd(&cursor[lk], __popcll(mm));
-        b = __shfl(b, leader);
-        if (valid && key == lk) dst = b + __popcll(mm & ((1ull << lane) - 1ull));
+        const int leader = __ffsll((long long)active) - 1;
+        const int lk = __shfl(key, leader);
+        const unsigned long long mm = __ballot(valid && key == lk);
+        if (valid && key == lk) {
+            my_leader = leader;
+            my_group = mm;
+        }
         active &= ~mm;
     }
-    return dst;
+    int b = 0;
+    if (valid && lane == my_leader) b = atomicAdd(&cursor[key], __popcll(my_group));
+    b = __shfl(b, my_leader);
+    return valid ? b + __popcll(my_group & ((1ull << lane) - 1ull)) : -1;
 }
 
 template <bool IDS, bool HAS_B>
