@@ -47,6 +47,7 @@ struct MccParams {
 __global__ void __launch_bounds__(256) mcc_kernel(MeshP m, PState s, int64_t np, MccTarget tg, MccParams pp,
                                                   unsigned long long* __restrict__ count, double* __restrict__ S,
                                                   unsigned long long* __restrict__ ncol) {
+    unsigned long long ncol_local = 0;   // added to ncol once per wave
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = s.cell[p];
         if (c < 0) continue;   // hole left by a dead particle
@@ -107,8 +108,10 @@ __global__ void __launch_bounds__(256) mcc_kernel(MeshP m, PState s, int64_t np,
             atomicAdd(S + (size_t)(sj + 1) * m.ni + si, w[3] * dS);
         }
         atomicAdd(count + (size_t)sj * m.ni + si, 1ull);   // col-count-sum at node ((int)lc0, (int)lc1)
-        atomicAdd(ncol, 1ull);
+        ncol_local++;
     }
+    for (int o = 32; o > 0; o >>= 1) ncol_local += __shfl_xor(ncol_local, o);
+    if ((threadIdx.x & 63) == 0 && ncol_local) atomicAdd(ncol, ncol_local);
 }
 
 __global__ void mcc_fields_kernel(int64_t nn, int which, const unsigned long long* __restrict__ count,
