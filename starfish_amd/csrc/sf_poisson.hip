@@ -192,7 +192,8 @@ __global__ void newton_residual(MeshP m, const uint8_t* __restrict__ cls, const 
                                 const double* __restrict__ bfix, const double* __restrict__ rho,
                                 const double* __restrict__ x, double* __restrict__ r, float* __restrict__ D,
                                 double* __restrict__ P, double inv_eps, double Cn0, double phi0, double inv_kTe,
-                                int linear, double* __restrict__ partial) {
+                                int linear, double* __restrict__ partial, const int* __restrict__ go) {
+    if (go != nullptr && go[0] != 1) return;   // queued behind a linear solve that has not converged
     const int64_t nn = (int64_t)m.ni * m.nj;
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
@@ -239,7 +240,8 @@ __global__ void ref_norm_kernel(MeshP m, const uint8_t* __restrict__ cls, const 
 __global__ void refine_residual(MeshP m, const uint8_t* __restrict__ cls, const double* __restrict__ r,
                                 const double* __restrict__ y, const double* __restrict__ P,
                                 double* __restrict__ rr, float* __restrict__ d, float* __restrict__ res,
-                                double* __restrict__ partial) {
+                                double* __restrict__ partial, const int* __restrict__ go) {
+    if (go != nullptr && go[0] != 1) return;
     const int64_t nn = (int64_t)m.ni * m.nj;
     double a_rr = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
@@ -484,7 +486,9 @@ __global__ void __launch_bounds__(256) pcg_update(MeshP m, const uint8_t* __rest
 }
 
 // y += d (FP64 accumulation of the FP32 correction)
-__global__ void accumulate_y(int64_t nn, const float* __restrict__ d, double* __restrict__ y) {
+__global__ void accumulate_y(int64_t nn, const float* __restrict__ d, double* __restrict__ y,
+                             const int* __restrict__ go) {
+    if (go != nullptr && go[0] != 1) return;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
         y[u] += (double)d[u];
 }
@@ -501,7 +505,8 @@ __global__ void absmax_kernel(int64_t nn, const double* __restrict__ y, double* 
 // kte > 0 limits each node's update: a rise above kTe is damped to kTe (1 + ln(y / kTe)), a fall
 // is clipped at -10 kTe; kte <= 0 takes the full step.
 __global__ void newton_update(int64_t nn, const uint8_t* __restrict__ cls, double* __restrict__ x,
-                              double* __restrict__ y, double kte) {
+                              double* __restrict__ y, double kte, const int* __restrict__ go) {
+    if (go != nullptr && go[0] != 1) return;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
         double v = y[u];
         if (kte > 0) {
@@ -781,17 +786,31 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     // y is zero at the start of every Newton iteration (newton_update clears it), so the first
     // refinement residual r - J y is the Newton residual itself: one read serves both tests.
     SF_HIP(ctx, hipMemsetAsync(P->y, 0, nn * sizeof(double), st));
-    for (int it = 0; it <= nl_max; it++) {
+    // Newton step limit, node by node.  The linearised exponential underestimates how fast the
+    // electron density grows with phi, so a large rise overshoots (and overflows exp when the
+    // solve starts far below a dense plasma): rises beyond kTe are damped logarithmically,
+    // falls are clipped at 10 kTe.  Nodes already close stay on the quadratic path, and the
+    // limit changes the path only, not the discrete solution.  (Against a plain 10 kTe clip
+    // this cuts the steady-state Newton iterations of the cylinder deck from 8.4 to 5.4.)
+    const double lim = D.linear ? 0.0 : D.Te0;
+    auto launch_residual = [&](const int* go) {
         hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                            (const double*)P->W, (const double*)P->bfix, (const double*)ctx->d_rho,
                            (const double*)P->x, P->r, P->D, P->P, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
-                           P->partial);
+                           P->partial, go);
         hipLaunchKernelGGL(refine_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                            (const double*)P->r, (const double*)P->y, (const double*)P->P, P->rr, P->d, P->res,
-                           P->partial);
+                           P->partial, go);
         finalize<1>(P, st, P->partial, ge, S_NORM2);
-        rc = read_scalars(ctx, P);
-        if (rc) return rc;
+    };
+    bool have_residual = false;   // the residual of the current x is already in host_scal
+    for (int it = 0; it <= nl_max; it++) {
+        if (!have_residual) {
+            launch_residual(nullptr);
+            rc = read_scalars(ctx, P);
+            if (rc) return rc;
+        }
+        have_residual = false;
         rnorm = std::sqrt(P->host_scal[S_NORM2]);
         rel = rnorm / ref;
         if (!std::isfinite(rel)) return sf_fail(ctx, SF_ERR_STATE, "Newton residual is not finite (diverged)");
@@ -801,12 +820,13 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         // inexact Newton: solve J y = r to max(eta*||r||, 0.5*tol*ref) with FP32 PCG + FP64 refinement
         double lin_target = std::max(D.linear ? 0.5 * rel_tol * ref : eta * rnorm, 0.5 * rel_tol * ref);
         int refine = 0;
+        bool stepped = false;   // the update and the next residual ran behind the linear solve
         while (true) {
             double rr = rnorm;   // the first refinement residual was computed with the Newton residual
             if (refine > 0) {
                 hipLaunchKernelGGL(refine_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                                    (const double*)P->r, (const double*)P->y, (const double*)P->P, P->rr, P->d,
-                                   P->res, P->partial);
+                                   P->res, P->partial, nullptr);
                 finalize<1>(P, st, P->partial, ge, S_NORM2);
                 rc = read_scalars(ctx, P);
                 if (rc) return rc;
@@ -821,9 +841,15 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             SF_HIP(ctx, hipMemcpyAsync(P->istate, zero, 8, hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL(pcg_precond, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const float*)P->D,
                                (const float*)P->res, P->z, pz0);
+            // When this linear solve is the Newton step's last (it aims at the step's own target),
+            // the update and the next Newton residual are queued behind the first batch of PCG
+            // iterations.  They run only if that batch converged (the kernels read istate), and
+            // then one host synchronisation serves both the PCG check and the next residual.
+            const bool ahead = !debug && inner_tol == lin_target;
             float* pa = P->p0;
             float* pb = P->p1;
             int k = 0, its = 0, batch = 8;   // iterations queued between host checks; grows while unconverged
+            bool first_batch = true, done_in_first = false;
             while (true) {
                 const int nb = std::min(batch, max_it - k);
                 for (int b = 0; b < nb; b++, k++) {
@@ -839,35 +865,49 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
                                        (const int*)P->istate);
                     std::swap(pa, pb);
                 }
+                const bool queued = ahead && first_batch;
+                if (queued) {
+                    const int* go = (const int*)P->istate;
+                    hipLaunchKernelGGL(accumulate_y, dim3(ge), dim3(256), 0, st, nn, (const float*)P->d, P->y, go);
+                    hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x,
+                                       P->y, lim, go);
+                    launch_residual(go);
+                }
                 SF_HIP(ctx, hipGetLastError());
                 SF_HIP(ctx, hipMemcpyAsync(P->host_istate, P->istate, 8, hipMemcpyDeviceToHost, st));
+                if (queued) SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
                 SF_HIP(ctx, hipStreamSynchronize(st));
-                if (P->host_istate[0]) { its = P->host_istate[1]; break; }
+                if (P->host_istate[0]) {
+                    its = P->host_istate[1];
+                    done_in_first = first_batch;
+                    break;
+                }
                 if (k >= max_it) { its = k; break; }
                 batch = std::min(2 * batch, 64);
+                first_batch = false;
             }
             pcg_total += its;
-            hipLaunchKernelGGL(accumulate_y, dim3(ge), dim3(256), 0, st, nn, (const float*)P->d, P->y);
+            if (ahead && done_in_first && P->host_istate[0] == 1) {
+                stepped = true;
+                break;
+            }
+            hipLaunchKernelGGL(accumulate_y, dim3(ge), dim3(256), 0, st, nn, (const float*)P->d, P->y, nullptr);
             refine++;
             // PCG aimed at the Newton step's own target (far above FP32's drift) and reached it:
             // no FP64 check is needed; the next Newton residual verifies the step anyway.
             if (P->host_istate[0] == 1 && inner_tol == lin_target) break;
         }
-        // Newton step limit, node by node.  The linearised exponential underestimates how fast the
-        // electron density grows with phi, so a large rise overshoots (and overflows exp when the
-        // solve starts far below a dense plasma): rises beyond kTe are damped logarithmically,
-        // falls are clipped at 10 kTe.  Nodes already close stay on the quadratic path, and the
-        // limit changes the path only, not the discrete solution.  (Against a plain 10 kTe clip
-        // this cuts the steady-state Newton iterations of the cylinder deck from 8.4 to 5.4.)
-        double lim = 0.0;
-        if (!D.linear) {
-            lim = D.Te0;
-            if (debug) {
-                hipLaunchKernelGGL(absmax_kernel, dim3(ge), dim3(256), 0, st, nn, (const double*)P->y, P->partial);
-                finalize<1, true>(P, st, P->partial, ge, S_YMAX);
-            }
+        if (stepped) {
+            have_residual = true;
+            SF_HIP(ctx, hipGetLastError());
+            continue;
         }
-        hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y, lim);
+        if (debug && !D.linear) {
+            hipLaunchKernelGGL(absmax_kernel, dim3(ge), dim3(256), 0, st, nn, (const double*)P->y, P->partial);
+            finalize<1, true>(P, st, P->partial, ge, S_YMAX);
+        }
+        hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y, lim,
+                           nullptr);
         if (debug) {
             rc = read_scalars(ctx, P);
             if (rc) return rc;
@@ -1122,7 +1162,7 @@ int poisson_rel_residual(sf_ctx* ctx, double* rel) {
     finalize<1>(P, st, P->partial, ge, S_R0);
     hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
                        (const double*)P->bfix, (const double*)ctx->d_rho, (const double*)P->x, P->r, P->D, P->P,
-                       inv_eps, Cn0, D.phi0, inv_kTe, D.linear, P->partial);
+                       inv_eps, Cn0, D.phi0, inv_kTe, D.linear, P->partial, nullptr);
     finalize<1>(P, st, P->partial, ge, S_NORM2);
     int rc = read_scalars(ctx, P);
     if (rc) return rc;
