@@ -166,6 +166,7 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
     const int n_i0 = ti0 - TILE_HALO, n_j0 = tj0 - TILE_HALO;  // first node of the LDS tile
     const int c_i0 = ti0 - TILE_HALO, c_j0 = tj0 - TILE_HALO;  // first cell of the LDS histogram
     const size_t nn = (size_t)m.ni * m.nj;
+    const int pt = (t * CHUNK_STRIDE) & (PUSH_THREADS - 1);   // this thread's slot of a window
 
     for (int k = t; k < NCH * TILE_NODES; k += PUSH_THREADS) (&L.acc[0][0])[k] = 0ull;
     for (int k = t; k < TILE_CELLS; k += PUSH_THREADS) L.hist[k] = 0;
@@ -176,22 +177,48 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
         L.ef[k] = md.ef[(size_t)j * m.ni + i];
         L.hs[k] = md.node_hasseg[(size_t)j * m.ni + i];
     }
-    if (t == 0) {
+    // The row segments (uniform, from the cell offsets of the last sort).  Every thread knows them,
+    // so the first window's particle loads are issued before the barrier and overlap the staging
+    // of the tile; thread 0 also leaves them in LDS for the window loop.
+    int first_row = TILE_H, first_start = 0, first_len = 0;
+    {
         int pre = 0;
+#pragma unroll
         for (int r = 0; r < TILE_H; r++) {
-            int row = tj0 + r;
+            const int row = tj0 + r;
             int st = 0, len = 0;
             if (row < m.ncj) {
-                int k0 = row * m.nci + ti0;
-                int k1 = row * m.nci + min(ti0 + TILE_W, m.nci);
+                const int k0 = row * m.nci + ti0;
+                const int k1 = row * m.nci + min(ti0 + TILE_W, m.nci);
                 st = offsets[k0];
                 len = offsets[k1] - st;
             }
-            L.seg_start[r] = st;
-            L.seg_pre[r] = pre;
+            if (first_row == TILE_H && len > 0) {
+                first_row = r;
+                first_start = st;
+                first_len = len;
+            }
+            if (t == 0) {
+                L.seg_start[r] = st;
+                L.seg_pre[r] = pre;
+            }
             pre += len;
         }
-        L.seg_pre[TILE_H] = pre;
+        if (t == 0) L.seg_pre[TILE_H] = pre;
+    }
+    PReg first;
+    if (pt < first_len) {
+        first.g = first_start + pt;
+        first.c = s.cell[first.g];
+        first.fx = s.fx[first.g];
+        first.fy = s.fy[first.g];
+        first.vx = s.vx[first.g];
+        first.vy = s.vy[first.g];
+        first.vz = s.vz[first.g];
+    } else {
+        first.g = 0;
+        first.c = -1;
+        first.fx = first.fy = first.vx = first.vy = first.vz = 0.0f;
     }
     __syncthreads();
 
@@ -206,7 +233,6 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
         if (li >= 0 && li < TILE_NW && lj >= 0 && lj < TILE_NH) return L.hs[lj * TILE_NW + li];
         return md.node_hasseg[(size_t)j * m.ni + i];
     };
-    const int pt = (t * CHUNK_STRIDE) & (PUSH_THREADS - 1);
     // The tile's particles: TILE_H contiguous row segments, each visited in windows of
     // PUSH_THREADS slots (the last window of a row may be partial).  The barriers between
     // windows wait for LDS only, so the previous window's stores stay in flight across them.
@@ -228,14 +254,12 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
             q.fx = q.fy = q.vx = q.vy = q.vz = 0.0f;
         }
     };
-    int cr = 0, cb = 0;
-    while (cr < TILE_H && row_len(cr) == 0) cr++;
+    int cr = first_row, cb = 0;
+    PReg cur = first;   // the loads of each window are issued at the end of the window before it
     while (cr < TILE_H) {
         const int rlen = row_len(cr);
         const int64_t rst = L.seg_start[cr];
         const bool ok = cb + pt < rlen;
-        PReg cur;
-        fetch(cr, cb, cur);
         int nr = cr, nb = cb + PUSH_THREADS;
         if (nb >= rlen) {
             nr = cr + 1;
@@ -327,6 +351,7 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
                 keys[g] = kc;
             }
         }
+        if (nr < TILE_H) fetch(nr, nb, cur);
         lds_barrier();
         cr = nr;
         cb = nb;
