@@ -236,7 +236,8 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
     // The tile's particles: TILE_H contiguous row segments, each visited in windows of
     // PUSH_THREADS slots (the last window of a row may be partial).  The barriers between
     // windows wait for LDS only, so the previous window's stores stay in flight across them.
-    // (Loading the next window into registers ahead of time measured slower: the extra
+    // A window's loads are issued after the stores of the window before, ahead of the barrier
+    // between them.  (Loading it before the previous window's push measured slower: the extra
     // registers cost more than the latency it hides at four workgroups per CU.)
     auto row_len = [&](int r) { return L.seg_pre[r + 1] - L.seg_pre[r]; };
     auto fetch = [&](int r, int b, PReg& q) {
