@@ -32,7 +32,7 @@ constexpr int TILE_CELLS = TILE_CW * TILE_CH;
 constexpr int NCH = 7;                          // deposit channels: n, nu, nv, nw, nuu, nvv, nww
 constexpr int PUSH_THREADS = 256;
 constexpr int CHUNK = 256;                      // particles staged in LDS per round
-constexpr int CHUNK_STRIDE = 37;                // odd stride of the lane->particle permutation
+constexpr int CHUNK_STRIDE = 151;               // odd stride of the lane->particle permutation
 // Mover sums are added into STAT_SLOTS copies (by workgroup index) of STAT_STRIDE int64 each, one
 // 128-byte line apart, so that the atomics of thousands of waves do not queue on five addresses.
 constexpr int STAT_SLOTS = 64;
