@@ -22,9 +22,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-BYTES_PUSH_INPLACE = 48.0   # in-place step: read + write the 24 B particle state
-BYTES_PUSH_DEPOSIT = 28.0   # sort step, pass A: read 24 B of state, write the 4 B new-cell key
-BYTES_PUSH_SCATTER = 52.0   # sort step, pass B: read 24 B state + 4 B key, write 24 B state
+BYTES_PUSH_INPLACE = 48.0   # every step: read + write the 24 B particle state in its slot
+BYTES_SCATTER = 48.0        # sort step: read the 24 B pushed state (its cell is the key), write it sorted
 BYTES_STATE = 48.0          # read + write of the 24 B particle state (minimal push + deposit)
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2_bench.txt")
 TRAFFIC_PROFILE_PARTICLES = 19.8e6   # average particles per push dispatch in the profiled runs
@@ -180,8 +179,8 @@ def main():
     sim.ctx.call("sf_ctx_synchronize")
     kern = np.zeros(4)
     pushed = 0
-    # per-kernel totals: (ms, particles) for the in-place pass, the sort-step pass A and pass B
-    k_inplace, k_sortA, k_scatter = [0.0, 0, 0], [0.0, 0, 0], [0.0, 0, 0]
+    # per-kernel totals: (ms, particles, launches) for the tiled push and the sort-step scatter
+    k_push, k_scatter = [0.0, 0, 0], [0.0, 0, 0]
     solve_ms, pcg_its, newton_its = [], [], []
     worst_res = 0.0
     tail_ms = 0.0
@@ -201,8 +200,8 @@ def main():
         # the tiled kernels are timed alone, over the cell-sorted particles they push; the kernel
         # for particles appended since the last sort is in the particle phase but not here
         tail_ms += split[1]
-        for k, ms, n in (((k_sortA, split[0], nsorted), (k_scatter, tm[2], npush)) if tm[2] > 0
-                         else ((k_inplace, split[0], nsorted),)):
+        for k, ms, n in (((k_push, split[0], nsorted), (k_scatter, tm[2], npush)) if tm[2] > 0
+                         else ((k_push, split[0], nsorted),)):
             k[0] += ms
             k[1] += n
             k[2] += 1
@@ -221,9 +220,8 @@ def main():
     def rate(k, nbytes):
         return nbytes * k[1] / (k[0] * 1e-3) / 1e9 if k[0] > 0 else 0.0
 
-    kernels = [("push_deposit_tile_inplace", k_inplace, BYTES_PUSH_INPLACE),
-               ("push_deposit_tile_sort", k_sortA, BYTES_PUSH_DEPOSIT),
-               ("push_scatter", k_scatter, BYTES_PUSH_SCATTER)]
+    kernels = [("push_deposit_tile_inplace", k_push, BYTES_PUSH_INPLACE),
+               ("copy_scatter", k_scatter, BYTES_SCATTER)]
     dname, dk, dbytes = max(kernels, key=lambda e: e[1][0])
     achieved = rate(dk, dbytes)
     per_launch = dk[1] / max(1, dk[2])
@@ -271,7 +269,7 @@ def main():
         "full_step_particles_per_s": pushed / wall,
         "particle_phase_ms": {"push_deposit": kern[0] / args.steps,
                               "push_deposit_appended": tail_ms / args.steps, "scan": kern[1] / args.steps,
-                              "push_scatter": kern[2] / args.steps, "normalize": kern[3] / args.steps},
+                              "scatter": kern[2] / args.steps, "normalize": kern[3] / args.steps},
         "pcg": {"ms_per_solve": float(np.mean(solve_ms)), "pcg_iterations_per_solve": float(np.mean(pcg_its)),
                 "newton_per_solve": float(np.mean(newton_its)), "worst_rel_residual": worst_res,
                 "rel_tol": deck.solver.rel_tol, "cold_start": cold_info, "learned_guess": learned},

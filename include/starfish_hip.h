@@ -269,9 +269,9 @@ int sf_sources_sample(sf_ctx* ctx, int64_t* injected);
 int sf_sources_sample_device(sf_ctx* ctx, uint64_t seed, int64_t* injected);
 
 /* ------------------------------------------------------------------------------------------
- * Timing of the last sf_push_deposit (device ms, with sf_set_timing(ctx, 1)): out4[0] pass A
- * (push + deposit), out4[1] cell scan (sort step) or macro-particle count (in-place step),
- * out4[2] pass B (push + scatter; exactly 0 on an in-place step), out4[3] normalisation.
+ * Timing of the last sf_push_deposit (device ms, with sf_set_timing(ctx, 1)): out4[0] the push
+ * (push + deposit, in place), out4[1] cell scan (sort step) or macro-particle count (other steps),
+ * out4[2] the scatter into cell order (exactly 0 on a step without a sort), out4[3] normalisation.
  * ------------------------------------------------------------------------------------------ */
 int sf_last_timings(sf_ctx* ctx, double* out4);
 /* Split of out4[0]: out3[0] the tiled kernel over the cell-sorted particles, out3[1] the kernel

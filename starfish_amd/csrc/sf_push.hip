@@ -2,12 +2,14 @@
 //
 // One step of KineticMaterial.updateFields() (core/materials/KineticMaterial.java:117-197,
 // 1553-1595) runs as:
-//   pass A  (push_deposit_tile / push_deposit_tail): gather E, push, boundary processing and
-//           deposit of the 7 moment channels into LDS int64 accumulators of a 32x8-cell tile,
-//           LDS histogram of the new cells; writes the packed new cell per particle.
-//   scan    (scan_*): exclusive scan of the cell histogram -> cell ranges, cursors, mpc-sum.
-//   pass B  (push_scatter): the identical push recomputed from the same inputs, particle
-//           written to its slot of the new cell (counting sort + compaction of the dead).
+//   push    (push_deposit_tile / push_deposit_tail): gather E, push, boundary processing and
+//           deposit of the 7 moment channels into LDS int64 accumulators of a 32x4-cell tile,
+//           LDS histogram of the new cells; the pushed state is written back to the
+//           particle's own slot (a dead particle becomes a hole, cell = -1).
+//   scan    (scan_*, sort steps): exclusive scan of the cell histogram -> cell ranges, cursors,
+//           mpc-sum.  On other steps the histogram only feeds mpc-sum (mpc_accumulate).
+//   scatter (copy_scatter, sort steps): each pushed particle to its slot of its new cell
+//           (counting sort + compaction of the holes).
 //   normalise (normalize_nodes): int64 channels -> nd, u, v, w; running sums += channels.
 // Particles are read coalesced in cell-sorted order.  The deposit and the sums are 64-bit
 // fixed point, so every density and moment is independent of the order of the additions.
@@ -75,6 +77,24 @@ __device__ __forceinline__ void new_cell(const MeshP& m, double lc0, double lc1,
     if (cj < 0) cj = 0;
 }
 
+// Cell and FP32 offsets of an alive particle.  An offset that rounds up to 1 in FP32 stands for
+// the same position as offset 0 in the next cell and is stored that way, so the cell of every
+// stored particle is floor(lc) (an offset of exactly 1 remains only in the last cell row/column).
+__device__ __forceinline__ void cell_offsets(const MeshP& m, double lc0, double lc1, int& ci, int& cj, float& fx,
+                                             float& fy) {
+    new_cell(m, lc0, lc1, ci, cj);
+    fx = (float)(lc0 - ci);
+    fy = (float)(lc1 - cj);
+    if (fx >= 1.0f && ci < m.nci - 1) {
+        ci++;
+        fx = 0.0f;
+    }
+    if (fy >= 1.0f && cj < m.ncj - 1) {
+        cj++;
+        fy = 0.0f;
+    }
+}
+
 struct StatAcc {
     long long n, p0, p1, p2, e;
 };
@@ -118,7 +138,7 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // ----------------------------------------------------------------------------------------
-// pass A, sorted particles: one workgroup per tile of TILE_W x TILE_H cells
+// push of the cell-sorted particles: one workgroup per tile of TILE_W x TILE_H cells
 // ----------------------------------------------------------------------------------------
 struct TileLds {
     unsigned long long acc[NCH][TILE_NODES];
@@ -138,19 +158,17 @@ struct PReg {
     float fx, fy, vx, vy, vz;
 };
 
-// INPLACE = false: sort step, the new cell of every particle goes to keys[] (pass B moves it).
-// INPLACE = true: the pushed state is written back to the particle's own slot; dead particles
-// become holes (cell = -1) until the next sort.
+// The pushed state is written back to the particle's own slot; dead particles become holes
+// (cell = -1) until the next sort.
 // The tile's particles (its cell rows, contiguous after the last sort) are visited in windows of
 // PUSH_THREADS; thread t takes the window's particle (t * CHUNK_STRIDE) mod PUSH_THREADS, so the
 // lanes of a wave fall about one cell apart and their LDS atomics hit different nodes.  Each
 // thread loads its particle straight into registers.  Results go back through
 // LDS by slot and are stored coalesced: stores in the permuted order write partial cache lines
 // and run at about half the bandwidth of coalesced ones on this part.
-template <bool INPLACE, bool HAS_B>
+template <bool HAS_B>
 __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const MeshD& md, const PushC& pc,
                                                        const PState& s, const int32_t* __restrict__ offsets,
-                                                       int32_t* __restrict__ keys,
                                                        unsigned long long* __restrict__ acc,
                                                        int32_t* __restrict__ counts, long long* stats) {
     __shared__ TileLds L;
@@ -276,10 +294,8 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
             bool alive = push_particle<HAS_B>(m, md, pc, load_e, hasseg, lc0, lc1, v0, v1, v2);
             if (alive) {
                 int ci, cj;
-                new_cell(m, lc0, lc1, ci, cj);
+                cell_offsets(m, lc0, lc1, ci, cj, fxn, fyn);
                 key = pack_cell(ci, cj);
-                fxn = (float)(lc0 - ci);
-                fyn = (float)(lc1 - cj);
                 vxn = (float)v0;
                 vyn = (float)v1;
                 vzn = (float)v2;
@@ -326,30 +342,24 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
         }
         if (ok) {
             L.o_cell[pt] = key;   // -1: dead, a hole until the next sort
-            if (INPLACE) {
-                L.o_fx[pt] = fxn;
-                L.o_fy[pt] = fyn;
-                L.o_vx[pt] = vxn;
-                L.o_vy[pt] = vyn;
-                L.o_vz[pt] = vzn;
-            }
+            L.o_fx[pt] = fxn;
+            L.o_fy[pt] = fyn;
+            L.o_vx[pt] = vxn;
+            L.o_vy[pt] = vyn;
+            L.o_vz[pt] = vzn;
         }
         lds_barrier();
         const int wlen = min(PUSH_THREADS, rlen - cb);
         if (t < wlen) {
             const int64_t g = rst + cb + t;
             const int32_t kc = L.o_cell[t];
-            if (INPLACE) {
-                s.cell[g] = kc;
-                if (kc >= 0) {
-                    s.fx[g] = L.o_fx[t];
-                    s.fy[g] = L.o_fy[t];
-                    s.vx[g] = L.o_vx[t];
-                    s.vy[g] = L.o_vy[t];
-                    s.vz[g] = L.o_vz[t];
-                }
-            } else {
-                keys[g] = kc;
+            s.cell[g] = kc;
+            if (kc >= 0) {
+                s.fx[g] = L.o_fx[t];
+                s.fy[g] = L.o_fy[t];
+                s.vx[g] = L.o_vx[t];
+                s.vy[g] = L.o_vy[t];
+                s.vz[g] = L.o_vz[t];
             }
         }
         if (nr < TILE_H) fetch(nr, nb, cur);
@@ -378,45 +388,29 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
     stat_flush(st, stats);
 }
 
-// In-place step: four workgroups per CU (35 KB of LDS each, at most 128 VGPRs).
+// Four workgroups per CU (35 KB of LDS each, at most 128 VGPRs).
 __global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile_inplace(MeshP m, MeshD md, PushC pc, PState s,
                                                                            const int32_t* __restrict__ offsets,
                                                                            unsigned long long* __restrict__ acc,
                                                                            int32_t* __restrict__ counts,
                                                                            long long* stats) {
-    push_deposit_tile_body<true, false>(m, md, pc, s, offsets, nullptr, acc, counts, stats);
+    push_deposit_tile_body<false>(m, md, pc, s, offsets, acc, counts, stats);
 }
-// Sort step (pass A): the key write needs a few more registers; no register cap, so no spills.
-__global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile_sort(MeshP m, MeshD md, PushC pc, PState s,
-                                                                      const int32_t* __restrict__ offsets,
-                                                                      int32_t* __restrict__ keys,
-                                                                      unsigned long long* __restrict__ acc,
-                                                                      int32_t* __restrict__ counts, long long* stats) {
-    push_deposit_tile_body<false, false>(m, md, pc, s, offsets, keys, acc, counts, stats);
-}
-// The same passes with a magnetic field loaded (Boris where the gathered B is nonzero).
+// The same pass with a magnetic field loaded (Boris where the gathered B is nonzero).
 __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile_inplace_b(MeshP m, MeshD md, PushC pc, PState s,
                                                                           const int32_t* __restrict__ offsets,
                                                                           unsigned long long* __restrict__ acc,
                                                                           int32_t* __restrict__ counts,
                                                                           long long* stats) {
-    push_deposit_tile_body<true, true>(m, md, pc, s, offsets, nullptr, acc, counts, stats);
-}
-__global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile_sort_b(MeshP m, MeshD md, PushC pc, PState s,
-                                                                        const int32_t* __restrict__ offsets,
-                                                                        int32_t* __restrict__ keys,
-                                                                        unsigned long long* __restrict__ acc,
-                                                                        int32_t* __restrict__ counts, long long* stats) {
-    push_deposit_tile_body<false, true>(m, md, pc, s, offsets, keys, acc, counts, stats);
+    push_deposit_tile_body<true>(m, md, pc, s, offsets, acc, counts, stats);
 }
 
 // ----------------------------------------------------------------------------------------
-// pass A, unsorted tail (particles added since the last sort): global atomics
+// push of the unsorted tail (particles added since the last sort): global atomics
 // ----------------------------------------------------------------------------------------
-template <bool INPLACE, bool HAS_B>
+template <bool HAS_B>
 __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, PushC pc, PState s, int64_t p0,
-                                                         int64_t p1, int32_t* __restrict__ keys,
-                                                         unsigned long long* __restrict__ acc,
+                                                         int64_t p1, unsigned long long* __restrict__ acc,
                                                          int32_t* __restrict__ counts, long long* stats) {
     const size_t nn = (size_t)m.ni * m.nj;
     auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
@@ -424,18 +418,15 @@ __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, Push
     StatAcc st{0, 0, 0, 0, 0};
     for (int64_t p = p0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < p1;
          p += (int64_t)gridDim.x * blockDim.x) {
-        if (s.cell[p] < 0) {           // hole left by a dead particle
-            if (!INPLACE) keys[p] = -1;
-            continue;
-        }
+        if (s.cell[p] < 0) continue;   // hole left by a dead particle
         PIn q = load_particle(s, p);
         bool alive = push_particle<HAS_B>(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
         int32_t key = -1;
         if (alive) {
             int ci, cj;
-            new_cell(m, q.lc0, q.lc1, ci, cj);
+            float fxn, fyn;
+            cell_offsets(m, q.lc0, q.lc1, ci, cj, fxn, fyn);
             key = pack_cell(ci, cj);
-            float fxn = (float)(q.lc0 - ci), fyn = (float)(q.lc1 - cj);
             float vxn = (float)q.v0, vyn = (float)q.v1, vzn = (float)q.v2;
             stat_add(st, pc, vxn, vyn, vzn);
             atomicAdd(&counts[cj * m.nci + ci], 1);
@@ -445,16 +436,13 @@ __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, Push
                 channel_values(pc, vxn, vyn, vzn, val);
                 deposit_global(m, acc, nn, ci, cj, di, dj, val);
             }
-            if (INPLACE) {
-                s.fx[p] = fxn;
-                s.fy[p] = fyn;
-                s.vx[p] = vxn;
-                s.vy[p] = vyn;
-                s.vz[p] = vzn;
-            }
+            s.fx[p] = fxn;
+            s.fy[p] = fyn;
+            s.vx[p] = vxn;
+            s.vy[p] = vyn;
+            s.vz[p] = vzn;
         }
-        if (INPLACE) s.cell[p] = key;
-        else keys[p] = key;
+        s.cell[p] = key;
     }
     stat_flush(st, stats);
 }
@@ -559,7 +547,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_write(int32_t* __restrict__
 }
 
 // ----------------------------------------------------------------------------------------
-// pass B: recompute the push, write each particle to its slot in the new cell order
+// sort steps: each pushed particle to its slot in the new cell order
 // ----------------------------------------------------------------------------------------
 // Slot of each valid lane in its cell: the lanes of one key are grouped by ballots, the
 // lowest lane of each group reserves the group's slots with one atomic on the cell cursor, and
@@ -586,51 +574,6 @@ __device__ __forceinline__ int64_t wave_rank(bool valid, int key, int32_t* curso
     return valid ? b + __popcll(my_group & ((1ull << lane) - 1ull)) : -1;
 }
 
-template <bool IDS, bool HAS_B>
-__global__ void __launch_bounds__(256) push_scatter(MeshP m, MeshD md, PushC pc, PState s, PState d, int64_t np,
-                                                    const int32_t* __restrict__ keys, int32_t* __restrict__ cursor) {
-    auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
-    auto hasseg = [&](int i, int j) -> int { return md.node_hasseg[(size_t)j * m.ni + i]; };
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t nloop = (np + stride - 1) / stride * stride;   // uniform trip count for the wave ops
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nloop; p += stride) {
-        bool valid = p < np;
-        int32_t key = valid ? keys[p] : -1;
-        valid = valid && key >= 0;
-        int ci = 0, cj = 0, lkey = 0;
-        float fxn = 0, fyn = 0, vxn = 0, vyn = 0, vzn = 0;
-        int32_t idv = 0;
-        if (valid) {
-            PIn q = load_particle(s, p);
-            push_particle<HAS_B>(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
-            ci = cell_i(key);
-            cj = cell_j(key);
-            lkey = cj * m.nci + ci;
-            // the slot belongs to pass A's cell; should the recomputed push land a rounding step
-            // across its edge, the offset is clamped into that cell (an offset of exactly 1 is
-            // legitimate only in the last cell row / column: a particle on the last node line)
-            fxn = fmaxf((float)(q.lc0 - ci), 0.0f);
-            fyn = fmaxf((float)(q.lc1 - cj), 0.0f);
-            if (ci < m.nci - 1) fxn = fminf(fxn, 0.99999994f);
-            if (cj < m.ncj - 1) fyn = fminf(fyn, 0.99999994f);
-            vxn = (float)q.v0;
-            vyn = (float)q.v1;
-            vzn = (float)q.v2;
-            if (IDS) idv = s.id[p];
-        }
-        int64_t dst = wave_rank(valid, lkey, cursor);
-        if (valid) {
-            d.cell[dst] = key;
-            d.fx[dst] = fxn;
-            d.fy[dst] = fyn;
-            d.vx[dst] = vxn;
-            d.vy[dst] = vyn;
-            d.vz[dst] = vzn;
-            if (IDS) d.id[dst] = idv;
-        }
-    }
-}
-
 // sort without a push (after particles were loaded): key = current cell
 __global__ void __launch_bounds__(256) count_current(MeshP m, PState s, int64_t np, int32_t* __restrict__ keys,
                                                      int32_t* __restrict__ counts) {
@@ -651,15 +594,26 @@ __global__ void __launch_bounds__(256) copy_scatter(MeshP m, PState s, PState d,
         int32_t key = valid ? keys[p] : -1;
         valid = valid && key >= 0;
         int lkey = valid ? cell_j(key) * m.nci + cell_i(key) : 0;
+        // the state is loaded before the slot is known, so the loads overlap the cursor atomics
+        float fx = 0, fy = 0, vx = 0, vy = 0, vz = 0;
+        int32_t idv = 0;
+        if (valid) {
+            fx = s.fx[p];
+            fy = s.fy[p];
+            vx = s.vx[p];
+            vy = s.vy[p];
+            vz = s.vz[p];
+            if (IDS) idv = s.id[p];
+        }
         int64_t dst = wave_rank(valid, lkey, cursor);
         if (valid) {
             d.cell[dst] = key;
-            d.fx[dst] = s.fx[p];
-            d.fy[dst] = s.fy[p];
-            d.vx[dst] = s.vx[p];
-            d.vy[dst] = s.vy[p];
-            d.vz[dst] = s.vz[p];
-            if (IDS) d.id[dst] = s.id[p];
+            d.fx[dst] = fx;
+            d.fy[dst] = fy;
+            d.vx[dst] = vx;
+            d.vy[dst] = vy;
+            d.vz[dst] = vz;
+            if (IDS) d.id[dst] = idv;
         }
     }
 }
@@ -998,42 +952,24 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     const bool hb = md.bf != nullptr;
     const dim3 tg(tile_grid(ctx)), tb(PUSH_THREADS);
     const dim3 g_tail(grid_for(sp->np - sp->nsorted, 256, 4096)), b256(256);
-    if (sort_step) {
-        if (sp->nsorted > 0) {
-            if (hb)
-                hipLaunchKernelGGL(push_deposit_tile_sort_b, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets, sp->key,
-                                   sp->acc, sp->counts, sp->stats);
-            else
-                hipLaunchKernelGGL(push_deposit_tile_sort, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets, sp->key,
-                                   sp->acc, sp->counts, sp->stats);
-        }
-        if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-        if (sp->np > sp->nsorted) {
-            if (hb)
-                hipLaunchKernelGGL((push_deposit_tail<false, true>), g_tail, b256, 0, ctx->stream, m, md, pc, s,
-                                   sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
-            else
-                hipLaunchKernelGGL((push_deposit_tail<false, false>), g_tail, b256, 0, ctx->stream, m, md, pc, s,
-                                   sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
-        }
-    } else {
-        if (sp->nsorted > 0) {
-            if (hb)
-                hipLaunchKernelGGL(push_deposit_tile_inplace_b, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets,
-                                   sp->acc, sp->counts, sp->stats);
-            else
-                hipLaunchKernelGGL(push_deposit_tile_inplace, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets,
-                                   sp->acc, sp->counts, sp->stats);
-        }
-        if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-        if (sp->np > sp->nsorted) {
-            if (hb)
-                hipLaunchKernelGGL((push_deposit_tail<true, true>), g_tail, b256, 0, ctx->stream, m, md, pc, s,
-                                   sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
-            else
-                hipLaunchKernelGGL((push_deposit_tail<true, false>), g_tail, b256, 0, ctx->stream, m, md, pc, s,
-                                   sp->nsorted, sp->np, sp->key, sp->acc, sp->counts, sp->stats);
-        }
+    // Every step pushes in place (a dead particle becomes a hole, cell = -1).  A sort step then
+    // moves the pushed particles into cell order by their new cell, compacting the holes.
+    if (sp->nsorted > 0) {
+        if (hb)
+            hipLaunchKernelGGL(push_deposit_tile_inplace_b, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets,
+                               sp->acc, sp->counts, sp->stats);
+        else
+            hipLaunchKernelGGL(push_deposit_tile_inplace, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets,
+                               sp->acc, sp->counts, sp->stats);
+    }
+    if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    if (sp->np > sp->nsorted) {
+        if (hb)
+            hipLaunchKernelGGL(push_deposit_tail<true>, g_tail, b256, 0, ctx->stream, m, md, pc, s, sp->nsorted,
+                               sp->np, sp->acc, sp->counts, sp->stats);
+        else
+            hipLaunchKernelGGL(push_deposit_tail<false>, g_tail, b256, 0, ctx->stream, m, md, pc, s, sp->nsorted,
+                               sp->np, sp->acc, sp->counts, sp->stats);
     }
     SF_HIP(ctx, hipGetLastError());
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
@@ -1047,21 +983,12 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     if (sort_step && sp->np > 0) {
         const dim3 gs(grid_for(sp->np, 256, 1 << 20));
-        if (sp->desc.track_ids) {
-            if (hb)
-                hipLaunchKernelGGL((push_scatter<true, true>), gs, b256, 0, ctx->stream, m, md, pc, s, d, sp->np, sp->key,
-                                   sp->cursor);
-            else
-                hipLaunchKernelGGL((push_scatter<true, false>), gs, b256, 0, ctx->stream, m, md, pc, s, d, sp->np,
-                                   sp->key, sp->cursor);
-        } else {
-            if (hb)
-                hipLaunchKernelGGL((push_scatter<false, true>), gs, b256, 0, ctx->stream, m, md, pc, s, d, sp->np,
-                                   sp->key, sp->cursor);
-            else
-                hipLaunchKernelGGL((push_scatter<false, false>), gs, b256, 0, ctx->stream, m, md, pc, s, d, sp->np,
-                                   sp->key, sp->cursor);
-        }
+        if (sp->desc.track_ids)
+            hipLaunchKernelGGL(copy_scatter<true>, gs, b256, 0, ctx->stream, m, s, d, sp->np, (const int32_t*)s.cell,
+                               sp->cursor);
+        else
+            hipLaunchKernelGGL(copy_scatter<false>, gs, b256, 0, ctx->stream, m, s, d, sp->np, (const int32_t*)s.cell,
+                               sp->cursor);
         SF_HIP(ctx, hipGetLastError());
     }
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
@@ -1098,7 +1025,7 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
         hipEventElapsedTime(&e, ctx->ev[3], ctx->ev[4]);
         ctx->last_ms[0] = a;
         ctx->last_ms[1] = b;
-        ctx->last_ms[2] = sort_step ? c : 0.0;   // no pass B on an in-place step
+        ctx->last_ms[2] = sort_step ? c : 0.0;   // no scatter on a step without a sort
         ctx->last_ms[3] = e;
     }
     sp->nalive = st[0];
