@@ -267,6 +267,11 @@ int sf_sources_sample(sf_ctx* ctx, int64_t* injected);
    (particle of the batch, batch, source).  The shared java.util.Random stream is not consumed, so
    the draws match the reference's in distribution, not value by value. */
 int sf_sources_sample_device(sf_ctx* ctx, uint64_t seed, int64_t* injected);
+/* Replaces Source.getMassGeneratedInst / clearMassGeneratedInst bookkeeping (core/source/Source.java:34-47,
+   mass_generated_inst += mpw * mass per added particle at :193), which StatsModule.saveStats writes:
+   the number of particles source `source` (0-based, in creation order) has added to the domain since
+   the context was created, by either sampling path.  The mass is count * spwt * species mass. */
+int sf_source_added(sf_ctx* ctx, int source, int64_t* count);
 
 /* ------------------------------------------------------------------------------------------
  * Timing of the last sf_push_deposit (device ms, with sf_set_timing(ctx, 1)): out4[0] the push

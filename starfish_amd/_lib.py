@@ -122,6 +122,7 @@ SIGNATURES = {
                                        IP]),
     "sf_sources_sample": (C.c_int, [P, C.POINTER(I64)]),
     "sf_sources_sample_device": (C.c_int, [P, C.c_uint64, C.POINTER(C.c_int64)]),
+    "sf_source_added": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64)]),
     "sf_last_timings": (C.c_int, [P, DP]),
     "sf_last_push_timings": (C.c_int, [P, DP]),
     "sf_set_timing": (C.c_int, [P, C.c_int]),

@@ -503,6 +503,7 @@ int sf_sources_sample_device(sf_ctx* ctx, uint64_t seed, int64_t* injected) {
         int64_t added = 0;
         int rc = inject_maxwellian_device(ctx, s, num, seed, step, (uint32_t)q, &added);
         if (rc) return rc;
+        s.added += added;
         total += added;
     }
     if (injected) *injected = total;
@@ -578,10 +579,21 @@ int sf_sources_sample(sf_ctx* ctx, int64_t* injected) {
             int rc = add_particles_device(ctx, sp, (int64_t)x.size(), x.data(), y.data(), u.data(), v.data(), w.data(),
                                       nullptr, 1);
             if (rc) return rc;
+            s.added += sp->np - before;
             total += sp->np - before;
         }
     }
     if (injected) *injected = total;
+    return SF_OK;
+}
+
+// Source.mass_generated bookkeeping (Source.java:34-47, 193): the count of particles the source
+// has added to the domain; mass = count * spwt * species mass.
+int sf_source_added(sf_ctx* ctx, int source, int64_t* count) {
+    SF_CHECK_CTX(ctx);
+    if (source < 0 || source >= (int)ctx->sources.size() || !count)
+        return sf_fail(ctx, SF_ERR_ARG, "sf_source_added: bad source index %d", source);
+    *count = ctx->sources[source].added;
     return SF_OK;
 }
 

@@ -129,6 +129,7 @@ struct HostSource {
     std::vector<double> seg;       // x1,y1,x2,y2 per linear segment
     std::vector<double> cum;
     double area = 0, mdot = 0, v_drift = 0, v_th = 0, rem = 0;
+    int64_t added = 0;             // particles this source has placed in the domain so far
     double* d_geom = nullptr;      // device copy of seg then cum, for sampling on the device
 };
 
