@@ -782,7 +782,13 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     double rel = 0, rnorm = 0;
     bool converged = false;
     const bool debug = getenv("SF_DEBUG_SOLVE") != nullptr;
-    const double eta = getenv("SF_NEWTON_ETA") ? atof(getenv("SF_NEWTON_ETA")) : 1e-3;
+    // Forcing term of the inexact Newton steps: 1e-3 for a solve that starts far from its solution
+    // (first residual at least half the reference residual, as from phi = 0), 1e-2 for a solve
+    // warm-started from the previous step.  Measured on the cylinder deck: the warm solve takes
+    // 1.00 instead of 1.06 ms with 1e-2, while a cold solve with 1e-2 takes 117 instead of 70 ms.
+    const char* eta_env = getenv("SF_NEWTON_ETA");
+    double eta = eta_env ? atof(eta_env) : 1e-3;
+    bool eta_chosen = eta_env != nullptr;
     // y is zero at the start of every Newton iteration (newton_update clears it), so the first
     // refinement residual r - J y is the Newton residual itself: one read serves both tests.
     SF_HIP(ctx, hipMemsetAsync(P->y, 0, nn * sizeof(double), st));
@@ -816,6 +822,10 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         if (!std::isfinite(rel)) return sf_fail(ctx, SF_ERR_STATE, "Newton residual is not finite (diverged)");
         if (rel <= rel_tol) { converged = true; break; }
         if (it == nl_max) break;
+        if (!eta_chosen) {
+            eta = rel < 0.5 ? 1e-2 : 1e-3;
+            eta_chosen = true;
+        }
         newton++;
         // inexact Newton: solve J y = r to max(eta*||r||, 0.5*tol*ref) with FP32 PCG + FP64 refinement
         double lin_target = std::max(D.linear ? 0.5 * rel_tol * ref : eta * rnorm, 0.5 * rel_tol * ref);
