@@ -81,7 +81,9 @@ struct PushC {
     double qm_dt;          // q/m * dt
     double qm;             // q/m
     double dt;
-    double kscale[NCH];    // fixed-point scales 2^k per channel
+    // fixed-point scales 2^k: density channel, the three velocity channels, the three second
+    // moments (one value per group, so the push holds three scalars instead of seven)
+    double kn, kv, kvv;
     double spwt;
 };
 

@@ -23,13 +23,13 @@ namespace sf {
 // ----------------------------------------------------------------------------------------
 
 __device__ __forceinline__ void channel_values(const PushC& pc, double v0, double v1, double v2, double* val) {
-    val[0] = pc.kscale[0];
-    val[1] = v0 * pc.kscale[1];
-    val[2] = v1 * pc.kscale[2];
-    val[3] = v2 * pc.kscale[3];
-    val[4] = v0 * v0 * pc.kscale[4];
-    val[5] = v1 * v1 * pc.kscale[5];
-    val[6] = v2 * v2 * pc.kscale[6];
+    val[0] = pc.kn;
+    val[1] = v0 * pc.kv;
+    val[2] = v1 * pc.kv;
+    val[3] = v2 * pc.kv;
+    val[4] = v0 * v0 * pc.kvv;
+    val[5] = v1 * v1 * pc.kvv;
+    val[6] = v2 * v2 * pc.kvv;
 }
 
 // Global-memory deposit (used for the unsorted tail and for particles that left the tile).
@@ -101,10 +101,10 @@ struct StatAcc {
 
 __device__ __forceinline__ void stat_add(StatAcc& s, const PushC& pc, double v0, double v1, double v2) {
     s.n += 1;
-    s.p0 += quant(1.0, v0 * pc.kscale[1]);
-    s.p1 += quant(1.0, v1 * pc.kscale[1]);
-    s.p2 += quant(1.0, v2 * pc.kscale[1]);
-    s.e += quant(1.0, sqrt(v0 * v0 + v1 * v1 + v2 * v2) * pc.kscale[1]);
+    s.p0 += quant(1.0, v0 * pc.kv);
+    s.p1 += quant(1.0, v1 * pc.kv);
+    s.p2 += quant(1.0, v2 * pc.kv);
+    s.e += quant(1.0, sqrt(v0 * v0 + v1 * v1 + v2 * v2) * pc.kv);
 }
 
 __device__ __forceinline__ long long wave_sum_ll(long long v) {
@@ -880,7 +880,9 @@ static PushC make_pushc(sf_ctx* ctx, Species* sp) {
     pc.qm_dt = sp->qm * ctx->dt;
     pc.qm = sp->qm;
     pc.dt = ctx->dt;
-    for (int c = 0; c < NCH; c++) pc.kscale[c] = sp->kscale[c];
+    pc.kn = sp->kscale[0];
+    pc.kv = sp->kscale[1];    // = kscale[2] = kscale[3]
+    pc.kvv = sp->kscale[4];   // = kscale[5] = kscale[6]
     pc.spwt = sp->desc.spwt;
     return pc;
 }
