@@ -8,9 +8,10 @@ A step is one Starfish main-loop iteration: source injection, push + deposit + c
 rho, Newton-PCG potential solve to a 1e-8 relative residual, E = -grad(phi).
 
 Output: one JSON line.  `value` is particles pushed and deposited per second of device time of
-the particle phase (push/deposit pass, cell scan, push/scatter pass, normalisation), summed over
-the timed steps with HIP events on the library's stream.  The full-step rate (all phases, host
-clock) and the potential-solve cost are reported beside it.
+the particle phase (push kernels, cell scan, scatter into cell order, normalisation), summed
+over the timed steps with HIP events on the library's stream.  The full-step rate (all phases,
+host clock), the potential-solve cost, the CPU baselines of the reference algorithm and the
+errors against it on a sample of the same workload are reported beside it.
 """
 import argparse
 import json
@@ -22,23 +23,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-BYTES_PUSH_INPLACE = 48.0   # every step: read + write the 24 B particle state in its slot
-BYTES_SCATTER = 48.0        # sort step: read the 24 B pushed state (its cell is the key), write it sorted
-BYTES_STATE = 48.0          # read + write of the 24 B particle state (minimal push + deposit)
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2_bench.txt")
-TRAFFIC_PROFILE_PARTICLES = 19.8e6   # average particles per push dispatch in the profiled runs
+BYTES_PUSH = 48.0          # read + write of the 24 B particle state per particle pushed
+BYTES_SCATTER = 48.0       # sort step: read the 24 B pushed state (its cell is the key), write it sorted
+BYTES_STATE = 48.0         # read + write of the 24 B particle state (minimal push + deposit)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_push_tile_fast_xy_c2.txt")
 
 
 def measured_bytes_per_particle(kernel):
-    """HBM bytes per particle of one push kernel, from the committed FETCH_SIZE + WRITE_SIZE summary
-    (already scaled by the copy-kernel calibration); None when the summary or the kernel is absent."""
+    """HBM bytes per particle of one push kernel from the committed FETCH_SIZE + WRITE_SIZE summary
+    (its lines: kernel, particles per dispatch, bytes per dispatch, already scaled by the
+    copy-kernel calibration); None when the summary or the kernel is absent."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 parts = line.split()
-                if parts and parts[0] == kernel:
-                    return float(parts[-1]) / TRAFFIC_PROFILE_PARTICLES
-    except OSError:
+                if len(parts) >= 3 and parts[0] == kernel:
+                    return float(parts[2]) / float(parts[1])
+    except (OSError, ValueError):
         pass
     return None
 
@@ -50,7 +51,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--particles", type=float, default=2e7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all host cores)")
     ap.add_argument("--sort-interval", type=int, default=4)
     ap.add_argument("--learned-configs", type=int, default=8,
                     help="C2-style runs (random cylinder radius / bias) that provide training snapshots")
@@ -62,13 +63,16 @@ def parse():
     return ap.parse_args()
 
 
-def learned_guess(sim, deck, args):
+def learned_guess(sim, deck, args, rho0):
     """Trains the learned initial guess on snapshots of C2-style runs (cylinder radius
-    U(0.02, 0.1) m, bias U(-200, 0) V; none is the benchmark's cylinder) and compares, on the
-    benchmark's own states, the solve from phi = 0 (plain), from the previous step's phi (warm)
-    and from the network's guess (learned).  Every solve runs to the same residual tolerance."""
+    U(0.02, 0.1) m, bias U(-200, 0) V; none is the benchmark's cylinder).  Reported: (1) the cold
+    initial solve of the benchmark (preloaded plasma, phi = 0) against the same solve started from
+    the network's guess, and (2) on the benchmark's own time-loop states, the solve from phi = 0
+    (plain), from the previous step's phi (warm) and from the guess (learned).  Every solve runs to
+    the same residual tolerance."""
     import ctypes as C
     import numpy as np
+    from starfish_amd import _lib
     from starfish_amd.learned import LearnedGuess, generate_c2_snapshots, solve_from
     t0 = time.perf_counter()
     m = deck.mesh
@@ -83,6 +87,27 @@ def learned_guess(sim, deck, args):
             loss = net.train(args.learned_train_steps // 4, batch=4, lr=1e-3 if r < 3 else 3e-4)
         sim.ctx.call("sf_ctx_synchronize")
         t_train = time.perf_counter() - t1
+        # (1) the cold initial solve, from phi = 0 and from the guess, on the initial rho
+        prev_phi, prev_rho = sim.phi(), sim.rho()
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        cold = {}
+        for start in ("zero", "learned"):
+            sim.ctx.call("sf_rho_set", dp(rho0))
+            sim.ctx.call("sf_phi_set", dp(np.zeros(sim.nn)))
+            info = _lib.SolveInfo()
+            if start == "zero":
+                rc = sim.ctx.call("sf_poisson_solve", None, C.byref(info))
+            else:
+                used = C.c_int()
+                rc = _lib.check(sim.ctx.h, _lib.load().sf_poisson_solve_learned(sim.ctx.h, net.ptr, 0, C.byref(info),
+                                                                               C.byref(used)))
+            cold[start] = {"pcg_iterations": info.pcg_iterations, "newton": info.newton_iterations, "ms": info.ms,
+                           "rel_residual": info.rel_residual, "converged": rc == 0}
+        sim.ctx.call("sf_rho_set", dp(prev_rho))
+        sim.ctx.call("sf_phi_set", dp(prev_phi))
+        cold["iteration_ratio"] = cold["zero"]["pcg_iterations"] / max(1, cold["learned"]["pcg_iterations"])
+        cold["time_ratio"] = cold["zero"]["ms"] / max(1e-9, cold["learned"]["ms"])
+        # (2) the time loop's states
         rows = []
         mat = sim.materials[0]
         for _ in range(4):
@@ -100,12 +125,13 @@ def learned_guess(sim, deck, args):
             return float(np.mean([r[key][k] for r in rows]))
         out = {"samples": net.num_samples, "configs": args.learned_configs, "train_steps": args.learned_train_steps,
                "final_loss": loss, "generation_s": round(t_gen, 2), "training_s": round(t_train, 2),
-               "states": len(rows),
+               "cold_initial_solve": cold, "states": len(rows),
                "plain": {"pcg_iterations": agg("zero", 0), "newton": agg("zero", 1), "ms": agg("zero", 2)},
                "warm": {"pcg_iterations": agg("current", 0), "newton": agg("current", 1), "ms": agg("current", 2)},
                "learned": {"pcg_iterations": agg("learned", 0), "newton": agg("learned", 1), "ms": agg("learned", 2)},
                "worst_rel_residual": max(max(r[k][3] for k in r) for r in rows),
-               "all_converged": all(r[k][4] == 0 for r in rows for k in r)}
+               "all_converged": all(r[k][4] == 0 for r in rows for k in r) and cold["zero"]["converged"]
+               and cold["learned"]["converged"]}
         out["iteration_ratio_plain_over_learned"] = out["plain"]["pcg_iterations"] / max(1.0, out["learned"]["pcg_iterations"])
         return out, net
     except Exception:
@@ -113,40 +139,170 @@ def learned_guess(sim, deck, args):
         raise
 
 
-def cpu_baseline(sim, deck, threads, target_s=15.0):
-    """Times the oracle's restatement of the reference CPU path (threaded mover, serial deposit and
-    moment sampling) on a sample of the same particle population, mesh and E field."""
+def _rel_errs(g, r):
+    """max and RMS error of g against r, relative to the largest / RMS magnitude of r"""
     import numpy as np
+    d = np.asarray(g, np.float64) - np.asarray(r, np.float64)
+    return {"max": float(np.abs(d).max() / max(np.abs(r).max(), 1e-300)),
+            "rms": float(np.sqrt(np.mean(d * d)) / max(np.sqrt(np.mean(np.asarray(r, np.float64) ** 2)), 1e-300))}
+
+
+def cpu_baseline(sim, deck, args):
+    """The oracle (the C++ FP64 restatement of the reference CPU path: threaded mover, serial
+    deposit and moment sampling, Newton with the reference's Gauss-Seidel) timed on this host, and
+    used as the checker of the device on a sample of the same workload:
+
+    * push+deposit particles/s on a 2e6-particle sample of the benchmark's population, mesh and E
+      field, with every host core, and with one thread on a 2e5-particle sample;
+    * one warm Newton+GS potential solve of the benchmark mesh from the device's last rho and phi
+      (the reference's own tolerances), as the time loop would call it;
+    * C1 (81x41, the reference's CPU deck) steps 101-200 with every host core, by phase;
+    * errors of the device against the oracle at the benchmark's size: one push+deposit step of a
+      2e5-particle sample from identical states (n, u, v, w, rho = q n, and T after computeFields),
+      and phi from one FP64 Newton step with a direct solve of the reference's assembled system
+      from the device's solution (Newton's quadratic convergence makes that step the error)."""
+    import numpy as np
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from helpers import oracle_for
+    from helpers import geometry, oracle_for
+    from starfish_amd import deck as D
+    from starfish_amd.plugin import Simulation
+    nproc = args.cpu_threads or os.cpu_count() or 1
     geo = sim.geo
-    o = oracle_for(deck, geo, None, nthreads=threads)
     ei, ej = sim.efield()
-    o.set_field("efi", ei)
-    o.set_field("efj", ej)
     mat = sim.materials[0]
     p = mat.particles()
     n_all = len(p["x"])
-    n = min(n_all, 2_000_000)
-    sel = np.linspace(0, n_all - 1, n).astype(np.int64)
-    pos = np.stack([p["x"][sel], p["y"][sel], np.zeros(n)], 1)
-    vel = np.stack([p["u"][sel], p["v"][sel], p["w"][sel]], 1)
-    o.load_particles(0, pos, vel)
-    pushed, t_total, steps = 0, 0.0, 0
-    while t_total < target_s and steps < 50:
-        npre = o.num_particles(0)
-        t0 = time.perf_counter()
+    out = {}
+
+    def push_rate(threads, n, target_s, max_steps):
+        o = oracle_for(deck, geo, None, nthreads=threads)
+        o.set_field("efi", ei)
+        o.set_field("efj", ej)
+        sel = np.linspace(0, n_all - 1, n).astype(np.int64)
+        pos = np.stack([p["x"][sel], p["y"][sel], np.zeros(n)], 1)
+        vel = np.stack([p["u"][sel], p["v"][sel], p["w"][sel]], 1)
+        o.load_particles(0, pos, vel)
+        pushed, t_total, steps = 0, 0.0, 0
+        while t_total < target_s and steps < max_steps:
+            npre = o.num_particles(0)
+            t0 = time.perf_counter()
+            o.move()
+            o.update_species_fields()
+            t_total += time.perf_counter() - t0
+            pushed += npre
+            steps += 1
+        o.close()
+        return pushed / t_total, steps, t_total
+
+    n_big = min(n_all, 2_000_000)
+    rate_all, st_all, t_all = push_rate(nproc, n_big, 10.0, 40)
+    n_one = min(n_all, 200_000)
+    rate_one, st_one, t_one = push_rate(1, n_one, 4.0, 20)
+    out["push_deposit"] = {
+        "all_cores": {"particles_per_s": rate_all, "threads": nproc, "sample": n_big, "steps": st_all,
+                      "seconds": round(t_all, 2)},
+        "one_thread": {"particles_per_s": rate_one, "threads": 1, "sample": n_one, "steps": st_one,
+                       "seconds": round(t_one, 2)}}
+
+    # one warm Newton+GS solve of the benchmark mesh (Solver.solveNonLinearNR + LinearSolverGS)
+    o = oracle_for(deck, geo, None, nthreads=nproc)
+    o.set_field("rho", sim.rho())
+    o.set_field("phi", sim.phi())
+    t0 = time.perf_counter()
+    o.solve_potential()
+    t_solve = time.perf_counter() - t0
+    o.close()
+    out["newton_gs_warm_solve_ms"] = t_solve * 1e3
+
+    # C1: the reference's CPU deck, steps 101-200, by phase (sources, move, deposit+samples, solve)
+    c1 = D.config_c1()
+    geo1, st1 = geometry(c1)
+    o = oracle_for(c1, geo1, st1, nthreads=nproc)
+    o.add_source(0, c1.boundary("inlet").points, False, c1.sources[0].mdot, c1.sources[0].v_drift,
+                 c1.sources[0].temperature)
+    o.initial_solve()
+    o.step(100)
+    t_a = o.timers()
+    w0 = time.perf_counter()
+    o.step(100)
+    wall = time.perf_counter() - w0
+    t_b = o.timers()
+    np_c1 = o.num_particles(0)
+    o.close()
+    ph = (t_b - t_a) / 100 * 1e3
+    out["c1_steps_101_200"] = {"threads": nproc, "ms_per_step": wall / 100 * 1e3, "particles_at_200": int(np_c1),
+                               "ms_sources": ph[0], "ms_move": ph[1], "ms_deposit_samples": ph[2],
+                               "ms_solve": ph[3]}
+
+    # errors at the benchmark's size: one step of a 2e5-particle sample from identical states
+    ns = min(n_all, 200_000)
+    rng = np.random.default_rng(7)
+    sel = np.sort(rng.choice(n_all, ns, replace=False))
+    ids = np.arange(ns, dtype=np.int32)
+    geo2, st2 = geometry(deck)
+    o = oracle_for(deck, geo2, st2, nthreads=nproc)
+    o.set_field("efi", ei)
+    o.set_field("efj", ej)
+    o.load_particles(0, np.stack([p["x"][sel], p["y"][sel], np.zeros(ns)], 1),
+                     np.stack([p["u"][sel], p["v"][sel], p["w"][sel]], 1), ids=ids)
+    small = Simulation(deck, capacity=2 * ns, track_ids=True)
+    try:
+        small.set_efield(ei, ej)
+        sm = small.materials[0]
+        sm.add_particles(p["x"][sel], p["y"][sel], p["u"][sel], p["v"][sel], p["w"][sel], ids=ids, rewind=False)
+        # the first step pushes the appended sample and sorts it; the second runs the tiled kernels
+        # from states that agree to FP32 (the oracle rounds its state as the device stores it)
         o.move()
         o.update_species_fields()
-        t_total += time.perf_counter() - t0
-        pushed += npre
-        steps += 1
+        o.round_state_fp32(0)
+        sm.update()
+        o.move()
+        o.update_species_fields()
+        sm.update()
+        err = {"particles": {"device": sm.getNp(), "oracle": int(o.num_particles(0))}}
+        for name in ("nd", "u", "v", "w"):
+            err[name] = _rel_errs(sm.field(name), o.field(name, 0))
+        err["rho"] = err["nd"]             # one species: rho = q n on both sides
+        err["t"] = _rel_errs(sm.field("t"), o.field("t", 0))
+    finally:
+        small.close()
+    # phi: one FP64 Newton step with a direct solve of the reference's assembled system
+    A = o.operator(0).tocsr()
+    fixed = o.fixed_nodes().astype(bool)
     o.close()
-    return {"value": pushed / t_total, "unit": "particles/s", "cores": threads, "kind": "port",
+    phi, rho = sim.phi(), sim.rho()
+    m = deck.mesh
+    ni, nj = m.ni, m.nj
+    nn = ni * nj
+    I, J = np.arange(nn) % ni, np.arange(nn) // ni
+    ft = geo.face_type
+    right, top, left, bottom = ft[:nj], ft[nj:nj + ni], ft[nj + ni:2 * nj + ni], ft[2 * nj + ni:]
+    face = ((I == 0) & (left[J] != 4) & (left[J] != 0)) | ((I == ni - 1) & (right[J] != 4) & (right[J] != 0)) | \
+           ((J == 0) & (bottom[I] != 4) & (bottom[I] != 0)) | ((J == nj - 1) & (top[I] != 4) & (top[I] != 0))
+    s = deck.solver
+    eps = 8.85418782e-12 * s.eps_r
+    b = -rho / eps
+    b[fixed] = geo.node_bc[fixed]
+    bx = np.where(fixed, 0, 1.60217646e-19 / eps * s.n0 * np.exp((phi - s.phi0) / s.Te0))
+    P = bx / s.Te0
+    b = b + bx
+    b[face & ~fixed] = 0
+    P[face & ~fixed] = 0
+    F = A @ phi - b
+    y = spla.spsolve((A - sps.diags(P)).tocsc(), F)
+    err["phi"] = {"max": float(np.abs(y).max() / np.abs(phi).max()),
+                  "rms": float(np.sqrt(np.mean(y * y)) / np.sqrt(np.mean(phi * phi))),
+                  "max_volts": float(np.abs(y).max())}
+    out["errors_vs_oracle"] = err
+    base = {"value": rate_all, "unit": "particles/s", "cores": nproc, "kind": "port",
             "sample": "%d particles of the same population on the same 1024x512 mesh and E field, %d steps of "
-                      "KineticMaterial.moveParticles (%d mover threads) + serial updateFields/updateSamples, "
-                      "%.1f s" % (n, steps, threads, t_total)}
+                      "KineticMaterial.moveParticles (%d mover threads) + serial updateFields/updateSamples, %.1f s"
+                      % (n_big, st_all, nproc, t_all)}
+    base.update(out)
+    return base
 
 
 def main():
@@ -169,6 +325,7 @@ def main():
     mat.preload(nparts, drift, temp, seed=1)
     t0 = time.perf_counter()
     rc = sim.initial_solve()
+    rho0 = sim.rho()
     cold = sim.solve_infos[-1]
     cold_info = {"ms": cold.ms, "newton": cold.newton_iterations, "pcg_iterations": cold.pcg_iterations,
                  "rel_residual": cold.rel_residual, "converged": rc == 0, "wall_s": time.perf_counter() - t0}
@@ -179,8 +336,9 @@ def main():
     sim.ctx.call("sf_ctx_synchronize")
     kern = np.zeros(4)
     pushed = 0
-    # per-kernel totals: (ms, particles, launches) for the tiled push and the sort-step scatter
-    k_push, k_scatter = [0.0, 0, 0], [0.0, 0, 0]
+    # per-kernel totals: (ms, particles, launches): the tiled push (the particles it pushed itself),
+    # the queue of particles it left to the full reference loop, and the sort-step scatter
+    k_push, k_queue, k_scatter = [0.0, 0, 0], [0.0, 0, 0], [0.0, 0, 0]
     solve_ms, pcg_its, newton_its = [], [], []
     worst_res = 0.0
     tail_ms = 0.0
@@ -193,18 +351,24 @@ def main():
         sim.ctx.call("sf_last_timings", tm.ctypes.data_as(C.POINTER(C.c_double)))
         split = np.zeros(3)
         sim.ctx.call("sf_last_push_timings", split.ctypes.data_as(C.POINTER(C.c_double)))
+        qd = np.zeros(2)
+        sim.ctx.call("sf_last_push_queue", qd.ctypes.data_as(C.POINTER(C.c_double)))
         kern += tm
         npush = npre + inj.value
-        nsorted = npush - int(split[2])     # particles the tiled kernel pushed
+        nsorted = npush - int(split[2])     # particles in the cell-sorted part of the array
+        nqueued = int(qd[1])
         pushed += npush
-        # the tiled kernels are timed alone, over the cell-sorted particles they push; the kernel
-        # for particles appended since the last sort is in the particle phase but not here
         tail_ms += split[1]
-        for k, ms, n in (((k_push, split[0], nsorted), (k_scatter, tm[2], npush)) if tm[2] > 0
-                         else ((k_push, split[0], nsorted),)):
-            k[0] += ms
-            k[1] += n
-            k[2] += 1
+        k_push[0] += split[0]
+        k_push[1] += nsorted - nqueued
+        k_push[2] += 1
+        k_queue[0] += qd[0]
+        k_queue[1] += nqueued
+        k_queue[2] += 1
+        if tm[2] > 0:
+            k_scatter[0] += tm[2]
+            k_scatter[1] += npush
+            k_scatter[2] += 1
         sim.update_fields()
         info = sim.solve_infos[-1]
         solve_ms.append(info.ms)
@@ -220,29 +384,29 @@ def main():
     def rate(k, nbytes):
         return nbytes * k[1] / (k[0] * 1e-3) / 1e9 if k[0] > 0 else 0.0
 
-    kernels = [("push_deposit_tile_inplace", k_push, BYTES_PUSH_INPLACE),
-               ("copy_scatter", k_scatter, BYTES_SCATTER)]
-    dname, dk, dbytes = max(kernels, key=lambda e: e[1][0])
-    achieved = rate(dk, dbytes)
-    per_launch = dk[1] / max(1, dk[2])
-    bpp = measured_bytes_per_particle(dname)
+    per_launch = k_push[1] / max(1, k_push[2])
+    bpp = measured_bytes_per_particle("push_tile_fast_xy")
+    achieved = rate(k_push, BYTES_PUSH)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4),
                 "traffic": round(bpp * per_launch) if bpp else None,
                 "traffic_bytes_per_particle": round(bpp, 2) if bpp else None,
-                "algorithmic_bytes_per_launch": round(dbytes * per_launch),
+                "kernel": "push_tile_fast_xy", "bytes_per_particle": BYTES_PUSH,
                 "particles_per_launch": round(per_launch),
-                "kernel": dname, "bytes_per_particle": dbytes,
-                "ms_per_launch": round(dk[0] / max(1, dk[2]), 4), "launches": dk[2],
-                "per_kernel_GBs": {name: round(rate(k, b), 1) for name, k, b in kernels},
+                "algorithmic_bytes_per_launch": round(BYTES_PUSH * per_launch),
+                "ms_per_launch": round(k_push[0] / max(1, k_push[2]), 4), "launches": k_push[2],
+                "per_kernel_GBs": {"push_tile_fast_xy": round(achieved, 1),
+                                   "push_deposit_queue": round(rate(k_queue, BYTES_PUSH), 1),
+                                   "copy_scatter": round(rate(k_scatter, BYTES_SCATTER), 1)},
+                "queued_per_step": round(k_queue[1] / max(1, k_queue[2])),
+                "queue_ms_per_step": round(k_queue[0] / max(1, k_queue[2]), 4),
                 "particle_phase_state_GBs": round(BYTES_STATE * pushed / particle_s / 1e9, 1)}
     cpu = None
     if not args.no_cpu_baseline:
-        thr = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(sim, deck, thr)
+        cpu = cpu_baseline(sim, deck, args)
     learned = None
     if not args.no_learned:
-        learned, net = learned_guess(sim, deck, args)
+        learned, net = learned_guess(sim, deck, args, rho0)
         net.close()
     out = {
         "metric": "particle push+deposit/s (2D ES-PIC, 1024x512 XY, 2e7 particles)",
@@ -255,11 +419,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+i64",
         "data": "synthetic (preloaded drifting Maxwellian, seed 1, plus boundary injection)",
         "config": {"workload": "ES-PIC O+ past a -100 V cylinder, XY 1024x512 nodes dx=1e-3 m, "
                                "%d particles preloaded, Boltzmann electrons n0=1e10 kTe=2 eV, dt=1e-7 s" % npart,
                    "particles_final": mat.getNp(), "state": "int32 cell + f32 offsets + f32 velocity (24 B)",
+                   "precision": "push in FP32 relative to the cell away from segments and faces, FP64 reference "
+                                "loop elsewhere; deposit rounded once in FP64 into int64 fixed point; PCG FP32 "
+                                "with FP64 residual refinement and FP64 Newton",
                    "injection": ("host, java.util.Random replay" if args.host_sources else
                                  "sampled on the device (Philox4x32-10)"),
                    "deposit": "int64 fixed point, 7 moment channels", "solver": "Newton + FP32 PCG (block red-black SGS preconditioner) "
@@ -267,8 +434,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "full_step_particles_per_s": pushed / wall,
-        "particle_phase_ms": {"push_deposit": kern[0] / args.steps,
-                              "push_deposit_appended": tail_ms / args.steps, "scan": kern[1] / args.steps,
+        "particle_phase_ms": {"push_tiles": k_push[0] / args.steps, "push_queue": k_queue[0] / args.steps,
+                              "push_appended": tail_ms / args.steps, "scan": kern[1] / args.steps,
                               "scatter": kern[2] / args.steps, "normalize": kern[3] / args.steps},
         "pcg": {"ms_per_solve": float(np.mean(solve_ms)), "pcg_iterations_per_solve": float(np.mean(pcg_its)),
                 "newton_per_solve": float(np.mean(newton_its)), "worst_rel_residual": worst_res,

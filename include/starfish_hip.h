@@ -27,6 +27,9 @@ extern "C" {
 
 #define SF_OK 0
 #define SF_WARN_NOT_CONVERGED 1
+/* sf_push_deposit: a particle's moment contribution exceeded the int64 fixed-point range (a speed
+   far beyond the species' vmax); the moment channels of that step are not valid */
+#define SF_WARN_MOMENT_RANGE 2
 #define SF_ERR_ARG (-1)
 #define SF_ERR_HIP (-2)
 #define SF_ERR_STATE (-3)
@@ -93,6 +96,12 @@ int sf_ctx_set_rng(sf_ctx* ctx, int64_t seed, int raw_state);
 int sf_ctx_get_rng_state(const sf_ctx* ctx, int64_t* state);
 /* Blocks until all queued device work of the context is done. */
 int sf_ctx_synchronize(sf_ctx* ctx);
+/* Current time step of TimeModule (Starfish.getIt(), core/common/TimeModule.java:131-142; the
+   first step is 1).  The push refreshes the computeFields outputs when the step is a multiple of
+   10 and on a species' first update (KineticMaterial.updateGasProperties,
+   core/materials/KineticMaterial.java:1553-1563).  While the host never sets it, each species
+   counts its own sf_push_deposit calls as the step. */
+int sf_ctx_set_it(sf_ctx* ctx, int it);
 
 /* ------------------------------------------------------------------------------------------
  * Host-side mesh geometry (what Starfish's Mesh.setBoundaries + Mesh.initNodes produce,
@@ -126,9 +135,11 @@ int sf_mesh_setup(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_type, c
 
 /* ------------------------------------------------------------------------------------------
  * Kinetic species (KineticMaterial, core/materials/KineticMaterial.java; Material keys
- * charge/molwt/spwt/init nd_back, Material.java:62-80).
+ * charge/molwt/spwt/init nd_back/init T, Material.java:62-80, 580-608).
  * vmax: expected maximum particle speed (m/s), sets the fixed-point scale of the velocity
- * moment channels (0 selects 1e5 m/s).  capacity: maximum particles held on the device.
+ * moment channels (0 selects 1e5 m/s).  Faster particles are still deposited exactly (a slower
+ * quantisation); beyond about 2000 vmax a contribution no longer fits 64 bits and the push
+ * returns SF_WARN_MOMENT_RANGE.  capacity: maximum particles held on the device.
  * ------------------------------------------------------------------------------------------ */
 typedef struct {
     double charge;       /* in units of e (Material "charge") */
@@ -138,6 +149,7 @@ typedef struct {
     double vmax;         /* fixed-point velocity scale hint, m/s */
     int64_t capacity;    /* particle capacity */
     int track_ids;       /* 1: carry an int32 particle id through sort/compaction */
+    double init_T;       /* init T (Material init_vals.T, default 0): temperature of empty nodes */
 } sf_species_desc;
 int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species);
 
@@ -188,8 +200,11 @@ int sf_push_deposit(sf_ctx* ctx, int species);
 int sf_species_set_sort_interval(sf_ctx* ctx, int species, int interval);
 /* which: 0 nd, 1 u, 2 v, 3 w (instantaneous, KineticMaterial.updateFields);
    4 count-sum, 5 u-sum, 6 v-sum, 7 w-sum, 8 uu-sum, 9 vv-sum, 10 ww-sum, 11 mpc-sum
-   (KineticMaterial.updateSamples); 12 t, 13 t1, 14 t2, 15 t3, 16 nd-ave, 17 u-ave, 18 v-ave,
-   19 w-ave, 20 p, 21 mpc (KineticMaterial.computeFields, evaluated on request). */
+   (KineticMaterial.updateSamples, running FP64 sums on the device); 12 t, 13 t1, 14 t2, 15 t3,
+   16 nd-ave, 17 u-ave, 18 v-ave, 19 w-ave, 20 p, 21 mpc (KineticMaterial.computeFields,
+   KineticMaterial.java:1606-1671, as last refreshed by the push: on the species' first update
+   and on steps that are multiples of 10; empty nodes hold init_T).  mpc is cell data, written
+   at node (i, j) of cell (i, j). */
 int sf_species_field(sf_ctx* ctx, int species, int which, double* out);
 /* KineticMaterial.clearSamples (KineticMaterial.java:1509-1529) */
 int sf_species_clear_samples(sf_ctx* ctx, int species);
@@ -212,7 +227,8 @@ typedef struct {
     int max_it;          /* max PCG iterations per linear solve (SolverModule "max_it") */
     double rel_tol;      /* stop when ||F||_2 / ||F_ref||_2 <= rel_tol (FP64 residual) */
     int nl_max_it;       /* max Newton iterations (SolverModule "nl_max_it") */
-    double nl_tol;       /* Newton also stops when ||y||_2/N < nl_tol (reference criterion) */
+    double nl_tol;       /* > 0: Newton also stops once a step y has ||y||_2 / N < nl_tol, N the node
+                            count (the reference's criterion, Solver.java:371-381); <= 0: only rel_tol */
 } sf_poisson_desc;
 typedef struct {
     int newton_iterations;
@@ -258,7 +274,13 @@ int sf_operator_apply(sf_ctx* ctx, const double* x, double* y);
  * ------------------------------------------------------------------------------------------ */
 int sf_source_maxwellian(sf_ctx* ctx, int species, int npts, const double* pts, int reverse, double mdot,
                          double v_drift, double temp, int* source);
-/* SourceModule.sampleSources (core/source/SourceModule.java:231-283): every source injects */
+/* MaxwellianSource <v_az> (sources/MaxwellianSource.java:62, 117: added to the third velocity
+   component) and Source <start_it> / <stop_it> (or <end_it>) (core/source/Source.java:68-71): the
+   source samples only on steps start_it <= it <= stop_it (stop_it < 0: no end), and outside that
+   window it does not regenerate either (SourceModule.java:240-241).  The step is the one set with
+   sf_ctx_set_it, else the count of sampling calls. */
+int sf_source_options(sf_ctx* ctx, int source, double v_az, int start_it, int stop_it);
+/* SourceModule.sampleSources (core/source/SourceModule.java:231-283): every active source injects */
 int sf_sources_sample(sf_ctx* ctx, int64_t* injected);
 /* The same injection with the particles sampled on the device (no host sampling or upload):
    identical particle counts per step, positions along the boundary and drifting-Maxwellian
@@ -283,6 +305,11 @@ int sf_last_timings(sf_ctx* ctx, double* out4);
  * over the particles appended since the last sort (injected ones), out3[2] how many slots that
  * second kernel covered. */
 int sf_last_push_timings(sf_ctx* ctx, double* out3);
+/* The XY tile kernel pushes in FP32 every particle whose cell is away from segments and mesh faces
+ * and queues the others for the full reference loop: out2[0] the queue kernel's ms (with
+ * sf_set_timing), out2[1] how many particles the last push queued.  The environment variable
+ * SF_NO_FAST_PUSH (at context creation) makes every step run the full loop in the tile kernel. */
+int sf_last_push_queue(sf_ctx* ctx, double* out2);
 int sf_set_timing(sf_ctx* ctx, int enabled);
 
 /* ------------------------------------------------------------------------------------------
@@ -297,6 +324,9 @@ int sf_set_timing(sf_ctx* ctx, int enabled);
  * ------------------------------------------------------------------------------------------ */
 #define SF_MCC_CEX 0
 #define SF_MCC_MEX 1
+/* tag value that lets the library choose the smallest tag not used by another live object of
+   the same kind and seed; explicit tags must be below 2^28 */
+#define SF_TAG_AUTO 0xFFFFFFFFu
 typedef struct {
     int source_species;      /* <source> (kinetic) */
     int model;               /* SF_MCC_CEX or SF_MCC_MEX (<model>) */
@@ -306,7 +336,9 @@ typedef struct {
     int stop_it;             /* <stop_it>: no passes after this step (<= 0: none) */
     double max_target_temp;  /* <max_target_temp> (<= 0: no limit) */
     uint64_t seed;           /* Philox key */
-    uint32_t tag;            /* distinguishes the streams of several interactions */
+    uint32_t tag;            /* distinguishes the streams of several interactions (< 2^28, or
+                                SF_TAG_AUTO); MCC, DSMC and device injection streams never overlap,
+                                and two live MCC objects with the same seed and tag are refused */
 } sf_mcc_desc;
 typedef struct sf_mcc sf_mcc;
 int sf_mcc_create(sf_ctx* ctx, const sf_mcc_desc* desc, sf_mcc** out);
@@ -347,7 +379,7 @@ typedef struct {
     double sig_cr_max0;              /* <sig_cr_max> initial value (reference default 1e-16) */
     int frequency;                   /* <frequency> */
     uint64_t seed;
-    uint32_t tag;
+    uint32_t tag;                    /* < 2^28 or SF_TAG_AUTO, unique per seed among live DSMC objects */
 } sf_dsmc_desc;
 typedef struct sf_dsmc sf_dsmc;
 int sf_dsmc_create(sf_ctx* ctx, const sf_dsmc_desc* desc, sf_dsmc** out);

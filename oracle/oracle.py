@@ -60,6 +60,11 @@ def load():
         "orc_update_efield": (None, [P]),
         "orc_initial_solve": (None, [P]),
         "orc_step": (None, [P, C.c_int]),
+        "orc_source_options": (None, [P, C.c_int, C.c_double, C.c_int, C.c_int]),
+        "orc_species_init_T": (None, [P, C.c_int, C.c_double]),
+        "orc_set_it": (None, [P, C.c_int]),
+        "orc_sample_sources": (None, [P]),
+        "orc_get_it": (C.c_int, [P]),
         "orc_get_stats": (None, [P, C.c_int, DP]),
         "orc_get_timers": (None, [P, DP]),
         "orc_last_lin_iterations": (C.c_int, [P]),
@@ -143,6 +148,20 @@ class Oracle:
         pts = np.array([c for p in points for c in p], dtype=np.float64)
         return self.lib.orc_add_source(self.h, sp, len(points), _dp(pts), 1 if reverse else 0, mdot, v_drift, temp)
 
+    def source_options(self, src, v_az=0.0, start_it=0, stop_it=-1):
+        self.lib.orc_source_options(self.h, src, v_az, start_it, stop_it)
+
+    def set_init_T(self, sp, T):
+        self.lib.orc_species_init_T(self.h, sp, T)
+
+    @property
+    def it(self):
+        return self.lib.orc_get_it(self.h)
+
+    @it.setter
+    def it(self, value):
+        self.lib.orc_set_it(self.h, int(value))
+
     def load_particles(self, sp, pos3, vel3, ids=None, rewind=False, mpw=None):
         pos3 = np.ascontiguousarray(pos3, np.float64)
         vel3 = np.ascontiguousarray(vel3, np.float64)
@@ -207,6 +226,9 @@ class Oracle:
 
     def initial_solve(self):
         self.lib.orc_initial_solve(self.h)
+
+    def sample_sources(self):
+        self.lib.orc_sample_sources(self.h)
 
     def step(self, n=1):
         self.lib.orc_step(self.h, n)

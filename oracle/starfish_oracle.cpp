@@ -386,6 +386,7 @@ struct Source {
     double mdot, v_drift, v_th, v_az = 0;
     double mp_rem = 0;
     int num_mp = 0;
+    int start_it = 0, stop_it = -1;       /* Source.java:68-71 */
 };
 
 struct Gradient {                         /* Solver.java:62-74 */
@@ -1076,11 +1077,18 @@ struct orc_sim {
         }
     }
 
+    /* SourceModule.sampleSources (SourceModule.java:231-283): sources outside their window skip */
+    void sample_sources() {
+        for (auto& s : sources) {
+            if (it < s.start_it || (s.stop_it >= 0 && it > s.stop_it)) continue;
+            sample_source(s);
+        }
+    }
     /* Starfish.MainLoop body (core/common/Starfish.java:77-121) */
     void step() {
         using clk = std::chrono::steady_clock;
         auto t0 = clk::now();
-        for (auto& s : sources) sample_source(s);
+        sample_sources();
         auto t1 = clk::now();
         for (auto& sp : species) move_species(*sp);
         auto t2 = clk::now();
@@ -1233,6 +1241,18 @@ int orc_add_source(orc_sim* s, int sp, int npts, const double* pts, int reverse,
     src.v_th = std::sqrt(2 * K_B * temp / s->species[sp]->mass);   /* Utils.computeVth */
     s->sources.push_back(src);
     return (int)s->sources.size() - 1;
+}
+
+void orc_species_init_T(orc_sim* s, int sp, double T) { s->species[sp]->init_T = T; }
+void orc_set_it(orc_sim* s, int it) { s->it = it; }
+void orc_sample_sources(orc_sim* s) { s->sample_sources(); }
+int orc_get_it(const orc_sim* s) { return s->it; }
+
+void orc_source_options(orc_sim* s, int src, double v_az, int start_it, int stop_it) {
+    Source& so = s->sources[src];
+    so.v_az = v_az;                                    /* MaxwellianSource.java:62, 117 */
+    so.start_it = start_it;
+    so.stop_it = stop_it;
 }
 
 void orc_load_particles(orc_sim* s, int spi, int64_t n, const double* pos3, const double* vel3, const double* mpw,
@@ -1411,7 +1431,7 @@ extern "C" int64_t orc_mcc_perform(orc_sim* s, int spi, const double* den, const
     rng.key[0] = (uint32_t)seed;
     rng.key[1] = (uint32_t)(seed >> 32);
     rng.step = (uint32_t)it;
-    rng.tag = tag;
+    rng.tag = (1u << 28) | tag;          /* MCC streams carry module 1 in the tag's top bits */
     int64_t ncol = 0;
     for (Particle& part : sp.parts) {
         if (part.mpw <= 0) continue;
@@ -1542,7 +1562,7 @@ extern "C" int64_t orc_dsmc_perform(orc_sim* s, int spa, int spb, int sigma_type
     rng.key[0] = (uint32_t)seed;
     rng.key[1] = (uint32_t)(seed >> 32);
     rng.step = (uint32_t)it;
-    rng.tag = tag;
+    rng.tag = (2u << 28) | tag;          /* DSMC streams carry module 2 in the tag's top bits */
     int64_t ntot = 0;
     for (int j = 0; j < ncj; j++)
         for (int i = 0; i < nci; i++) {

@@ -73,6 +73,14 @@ int orc_solve_potential(orc_sim* s);       /* PoissonSolver.update; returns NR i
 void orc_update_efield(orc_sim* s);        /* PotentialSolver.updateGradientField */
 void orc_initial_solve(orc_sim* s);        /* SolverModule.updateFields before the main loop */
 void orc_step(orc_sim* s, int nsteps);     /* Starfish.MainLoop body */
+/* MaxwellianSource v_az and Source start_it / stop_it (end_it) */
+void orc_source_options(orc_sim* s, int src, double v_az, int start_it, int stop_it);
+/* Material init_vals.T (the temperature computeFields writes on empty nodes) */
+void orc_species_init_T(orc_sim* s, int sp, double T);
+/* TimeModule step counter (Starfish.getIt(); step() advances it) */
+void orc_set_it(orc_sim* s, int it);
+void orc_sample_sources(orc_sim* s);       /* SourceModule.sampleSources at the current step */
+int orc_get_it(const orc_sim* s);
 void orc_get_stats(const orc_sim* s, int sp, double* out5); /* mass, momentum[3], energy sums */
 void orc_get_timers(const orc_sim* s, double* out4);         /* seconds: source, move, deposit+samples, solve+E */
 int orc_last_lin_iterations(const orc_sim* s);

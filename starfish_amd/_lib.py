@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "libstarfish_hip.so")
 
 SF_OK = 0
 SF_WARN_NOT_CONVERGED = 1
+SF_WARN_MOMENT_RANGE = 2
 
 DOMAIN = {"XY": 0, "RZ": 1, "ZR": 2}
 FACE = {"OPEN": -1, "DIRICHLET": 0, "NEUMANN": 1, "PERIODIC": 2, "SYMMETRY": 3, "MESH": 4, "SINK": 5,
@@ -31,7 +32,7 @@ class MeshDesc(C.Structure):
 
 class SpeciesDesc(C.Structure):
     _fields_ = [("charge", C.c_double), ("molwt", C.c_double), ("spwt", C.c_double), ("nd_back", C.c_double),
-                ("vmax", C.c_double), ("capacity", C.c_int64), ("track_ids", C.c_int)]
+                ("vmax", C.c_double), ("capacity", C.c_int64), ("track_ids", C.c_int), ("init_T", C.c_double)]
 
 
 class PoissonDesc(C.Structure):
@@ -75,6 +76,7 @@ SIGNATURES = {
     "sf_ctx_set_rng": (C.c_int, [P, I64, C.c_int]),
     "sf_ctx_get_rng_state": (C.c_int, [P, C.POINTER(I64)]),
     "sf_ctx_synchronize": (C.c_int, [P]),
+    "sf_ctx_set_it": (C.c_int, [P, C.c_int]),
     "sf_geometry_build": (C.c_int, [P, C.POINTER(MeshDesc), IP, DP, C.c_int, IP, DP, IP, IP, DP, IP,
                                     C.POINTER(P)]),
     "sf_geometry_sizes": (C.c_int, [P, IP, IP]),
@@ -123,8 +125,10 @@ SIGNATURES = {
     "sf_sources_sample": (C.c_int, [P, C.POINTER(I64)]),
     "sf_sources_sample_device": (C.c_int, [P, C.c_uint64, C.POINTER(C.c_int64)]),
     "sf_source_added": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64)]),
+    "sf_source_options": (C.c_int, [P, C.c_int, C.c_double, C.c_int, C.c_int]),
     "sf_last_timings": (C.c_int, [P, DP]),
     "sf_last_push_timings": (C.c_int, [P, DP]),
+    "sf_last_push_queue": (C.c_int, [P, DP]),
     "sf_set_timing": (C.c_int, [P, C.c_int]),
     "sf_cnn_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(P)]),
     "sf_cnn_destroy": (C.c_int, [P]),

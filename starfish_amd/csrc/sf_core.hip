@@ -45,6 +45,40 @@ Species* get_species(sf_ctx* ctx, int s) {
 }
 }  // namespace
 
+namespace sf {
+
+int claim_stream(sf_ctx* ctx, uint64_t seed, uint32_t module, uint32_t tag, uint32_t* stream_tag) {
+    auto taken = [&](uint32_t t) {
+        for (auto& s : ctx->rng_streams)
+            if (s.first == seed && s.second == (module | t)) return true;
+        return false;
+    };
+    if (tag == SF_TAG_AUTO) {
+        tag = 0;
+        while (taken(tag)) tag++;
+    } else if (tag > RNG_TAG_MASK) {
+        return sf_fail(ctx, SF_ERR_ARG, "stream tag %u is not below 2^28", tag);
+    } else if (taken(tag)) {
+        return sf_fail(ctx, SF_ERR_ARG,
+                       "another live object of the same kind draws from seed %llu with tag %u; give it another tag",
+                       (unsigned long long)seed, tag);
+    }
+    ctx->rng_streams.emplace_back(seed, module | tag);
+    *stream_tag = module | tag;
+    return SF_OK;
+}
+
+void release_stream(sf_ctx* ctx, uint64_t seed, uint32_t stream_tag) {
+    auto& v = ctx->rng_streams;
+    for (size_t k = 0; k < v.size(); k++)
+        if (v[k].first == seed && v[k].second == stream_tag) {
+            v.erase(v.begin() + k);
+            return;
+        }
+}
+
+}  // namespace sf
+
 extern "C" {
 
 const char* sf_version(void) { return "starfish_hip 0.1 (gfx950)"; }
@@ -75,6 +109,7 @@ int sf_ctx_create(int device, sf_ctx** out) {
         return SF_ERR_HIP;
     }
     for (auto& ev : c->ev) hipEventCreate(&ev);
+    c->no_fast_push = getenv("SF_NO_FAST_PUSH") != nullptr;
     *out = c;
     return SF_OK;
 }
@@ -93,6 +128,9 @@ static void free_species(Species* sp) {
     dfree(sp->acc);
     dfree(sp->sums);
     dfree(sp->mpc_sum);
+    dfree(sp->gas);
+    dfree(sp->flags);
+    dfree(sp->qcount);
     dfree(sp->den);
     dfree(sp->counts);
     dfree(sp->offsets);
@@ -109,6 +147,7 @@ static void free_mesh(sf_ctx* c) {
     dfree(c->d_nseg_idx);
     dfree(c->d_seg);
     dfree(c->d_hasseg);
+    dfree(c->d_cellslow);
     dfree(c->d_node_vol);
     dfree(c->d_phi);
     dfree(c->d_rho);
@@ -161,6 +200,13 @@ int sf_ctx_synchronize(sf_ctx* ctx) {
     SF_CHECK_CTX(ctx);
     SF_HIP(ctx, hipSetDevice(ctx->device));
     SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SF_OK;
+}
+
+int sf_ctx_set_it(sf_ctx* ctx, int it) {
+    SF_CHECK_CTX(ctx);
+    if (it < 1) return sf_fail(ctx, SF_ERR_ARG, "sf_ctx_set_it: the step counter starts at 1 (got %d)", it);
+    ctx->it = it;
     return SF_OK;
 }
 
@@ -244,6 +290,23 @@ int sf_mesh_setup(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_type, c
     TRY(dalloc(ctx, &ctx->d_nseg_idx, idx.size()));
     TRY(dalloc(ctx, &ctx->d_seg, (size_t)nseg));
     TRY(dalloc(ctx, &ctx->d_hasseg, nn));
+    // Cells whose particles can take the push's fast path: a move of less than one cell from
+    // cell (i, j) has a ProcessBoundary bounding box inside nodes [i-2, i+2] x [j-2, j+2] (one
+    // node of margin for rounding) and stays inside the mesh when 2 <= i <= nci-3 and
+    // 2 <= j <= ncj-3.  Any other cell, or one whose neighbourhood lists a DIRICHLET/SINK
+    // segment, is "slow": its particles run the full reference loop.
+    const int nci = ni - 1, ncj = nj - 1;
+    std::vector<uint8_t> cslow((size_t)nci * ncj, 1);
+    for (int j = 2; j <= ncj - 3; j++)
+        for (int i = 2; i <= nci - 3; i++) {
+            uint8_t any = 0;
+            for (int b = j - 2; b <= j + 2 && !any; b++)
+                for (int a = i - 2; a <= i + 2; a++)
+                    if (has[(size_t)b * ni + a]) { any = 1; break; }
+            cslow[(size_t)j * nci + i] = any;
+        }
+    TRY(dalloc(ctx, &ctx->d_cellslow, cslow.size()));
+    SF_HIP(ctx, hipMemcpy(ctx->d_cellslow, cslow.data(), cslow.size(), hipMemcpyHostToDevice));
     TRY(dalloc(ctx, &ctx->d_node_vol, nn));
     TRY(dalloc(ctx, &ctx->d_phi, nn));
     TRY(dalloc(ctx, &ctx->d_rho, nn));
@@ -275,6 +338,9 @@ int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species) {
     if (!(desc->spwt > 0)) return sf_fail(ctx, SF_ERR_ARG, "sf_species_create: spwt must be > 0");
     if (desc->capacity < 1 || desc->capacity > (int64_t)2000000000)
         return sf_fail(ctx, SF_ERR_ARG, "sf_species_create: capacity must be in [1, 2e9]");
+    if (!(desc->vmax >= 0) || !std::isfinite(desc->vmax))
+        return sf_fail(ctx, SF_ERR_ARG, "sf_species_create: vmax must be finite and >= 0");
+    if (!std::isfinite(desc->init_T)) return sf_fail(ctx, SF_ERR_ARG, "sf_species_create: init_T must be finite");
     SF_HIP(ctx, hipSetDevice(ctx->device));
     Species* sp = new Species();
     sp->desc = *desc;
@@ -306,6 +372,9 @@ int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species) {
     if (!rc) rc = dalloc(ctx, &sp->acc, NCH * nn);
     if (!rc) rc = dalloc(ctx, &sp->sums, NCH * nn);
     if (!rc) rc = dalloc(ctx, &sp->mpc_sum, ncells);
+    if (!rc) rc = dalloc(ctx, &sp->gas, NGAS * nn);
+    if (!rc) rc = dalloc(ctx, &sp->flags, 1);
+    if (!rc) rc = dalloc(ctx, &sp->qcount, 1);
     if (!rc) rc = dalloc(ctx, &sp->den, 4 * nn);
     if (!rc) rc = dalloc(ctx, &sp->counts, ncells);
     if (!rc) rc = dalloc(ctx, &sp->offsets, ncells + 1);
@@ -317,8 +386,16 @@ int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species) {
         return rc;
     }
     hipMemset(sp->acc, 0, NCH * nn * sizeof(unsigned long long));
-    hipMemset(sp->sums, 0, NCH * nn * sizeof(long long));
+    hipMemset(sp->sums, 0, NCH * nn * sizeof(double));
     hipMemset(sp->mpc_sum, 0, ncells * sizeof(long long));
+    hipMemset(sp->flags, 0, sizeof(int));
+    // Material.init: the "t" field starts at init_vals.T, t1-t3 and the averages at zero
+    // (Material.java:718-736, KineticMaterial.java:103-110)
+    {
+        std::vector<double> g(NGAS * nn, 0.0);
+        for (size_t u = 0; u < nn; u++) g[u] = desc->init_T;
+        hipMemcpy(sp->gas, g.data(), NGAS * nn * sizeof(double), hipMemcpyHostToDevice);
+    }
     hipMemset(sp->counts, 0, ncells * sizeof(int32_t));
     hipMemset(sp->offsets, 0, (ncells + 1) * sizeof(int32_t));
     // Material.init: nd starts at nd_back, u = v = w = 0
@@ -537,7 +614,7 @@ int sf_species_clear_samples(sf_ctx* ctx, int species) {
     if (!sp) return sf_fail(ctx, SF_ERR_ARG, "unknown species %d", species);
     const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj, ncells = (size_t)ctx->mp.nci * ctx->mp.ncj;
     SF_HIP(ctx, hipSetDevice(ctx->device));
-    SF_HIP(ctx, hipMemsetAsync(sp->sums, 0, NCH * nn * sizeof(long long), ctx->stream));
+    SF_HIP(ctx, hipMemsetAsync(sp->sums, 0, NCH * nn * sizeof(double), ctx->stream));
     SF_HIP(ctx, hipMemsetAsync(sp->mpc_sum, 0, ncells * sizeof(long long), ctx->stream));
     SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     sp->num_samples = 0;
@@ -575,6 +652,14 @@ int sf_last_push_timings(sf_ctx* ctx, double* out3) {
     return SF_OK;
 }
 
+int sf_last_push_queue(sf_ctx* ctx, double* out2) {
+    SF_CHECK_CTX(ctx);
+    if (!out2) return SF_ERR_ARG;
+    out2[0] = ctx->last_queue[0];
+    out2[1] = ctx->last_queue[1];
+    return SF_OK;
+}
+
 int sf_set_timing(sf_ctx* ctx, int enabled) {
     SF_CHECK_CTX(ctx);
     ctx->timing = enabled != 0;
@@ -585,8 +670,8 @@ int sf_set_timing(sf_ctx* ctx, int enabled) {
 
 namespace sf {
 
-// Downloads instantaneous fields, moment sums, and the KineticMaterial.computeFields
-// derived fields (KineticMaterial.java:1606-1671), evaluated in FP64 on the host.
+// Downloads the instantaneous fields (nd, u, v, w), the running moment sums (FP64 on the
+// device), mpc-sum, and the computeFields outputs the push last refreshed.
 int species_field(sf_ctx* ctx, Species* sp, int which, double* out) {
     const MeshP& m = ctx->mp;
     const size_t nn = (size_t)m.ni * m.nj;
@@ -595,61 +680,23 @@ int species_field(sf_ctx* ctx, Species* sp, int which, double* out) {
         SF_HIP(ctx, hipMemcpy(out, sp->den + which * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
         return SF_OK;
     }
-    std::vector<long long> sums(NCH * nn);
-    SF_HIP(ctx, hipMemcpy(sums.data(), sp->sums, NCH * nn * sizeof(long long), hipMemcpyDeviceToHost));
-    const double spwt = sp->desc.spwt;
-    auto sumf = [&](int ch, size_t u) { return (double)sums[ch * nn + u] / sp->kscale[ch] * spwt; };
     if (which >= 4 && which <= 10) {
         // count-sum, u-sum, v-sum, w-sum, uu-sum, vv-sum, ww-sum
-        static const int chmap[7] = {0, 1, 2, 3, 4, 5, 6};
-        int ch = chmap[which - 4];
-        for (size_t u = 0; u < nn; u++) out[u] = sumf(ch, u);
+        SF_HIP(ctx, hipMemcpy(out, sp->sums + (which - 4) * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
         return SF_OK;
     }
-    const size_t ncells = (size_t)m.nci * m.ncj;
-    std::vector<long long> mpc(ncells);
-    SF_HIP(ctx, hipMemcpy(mpc.data(), sp->mpc_sum, ncells * sizeof(long long), hipMemcpyDeviceToHost));
-    if (which == 11 || which == 21) {
-        // mpc-sum / mpc are cell data stored on node (i,j) of the cell
-        double f = which == 21 ? (sp->num_samples > 0 ? 1.0 / sp->num_samples : 0.0) : 1.0;
+    if (which == 11) {
+        // mpc-sum is cell data, stored on node (i,j) of the cell
+        const size_t ncells = (size_t)m.nci * m.ncj;
+        std::vector<long long> mpc(ncells);
+        SF_HIP(ctx, hipMemcpy(mpc.data(), sp->mpc_sum, ncells * sizeof(long long), hipMemcpyDeviceToHost));
         for (size_t u = 0; u < nn; u++) out[u] = 0;
         for (int j = 0; j < m.ncj; j++)
-            for (int i = 0; i < m.nci; i++) out[(size_t)j * m.ni + i] = (double)mpc[(size_t)j * m.nci + i] * f;
+            for (int i = 0; i < m.nci; i++) out[(size_t)j * m.ni + i] = (double)mpc[(size_t)j * m.nci + i];
         return SF_OK;
     }
-    const double inv_ns = sp->num_samples > 0 ? 1.0 / sp->num_samples : 0.0;
-    for (size_t u = 0; u < nn; u++) {
-        double count = sumf(0, u);
-        double T = 0, T1 = 0, T2 = 0, T3 = 0;
-        double uu_m = 0, vv_m = 0, ww_m = 0;
-        if (count > 0) {
-            double uu = sumf(1, u) / count, vv = sumf(2, u) / count, ww = sumf(3, u) / count;
-            double uu2 = sumf(4, u) / count - uu * uu;
-            double vv2 = sumf(5, u) / count - vv * vv;
-            double ww2 = sumf(6, u) / count - ww * ww;
-            double f = sp->mass / K_B;
-            T = (uu2 + vv2 + ww2) * f / 3.0;
-            T1 = uu2 * f;
-            T2 = vv2 * f;
-            T3 = ww2 * f;
-            uu_m = uu;
-            vv_m = vv;
-            ww_m = ww;
-        }
-        double nd_ave = count * inv_ns / ctx->h_node_vol[u];
-        switch (which) {
-            case 12: out[u] = T; break;
-            case 13: out[u] = T1; break;
-            case 14: out[u] = T2; break;
-            case 15: out[u] = T3; break;
-            case 16: out[u] = nd_ave; break;
-            case 17: out[u] = uu_m; break;
-            case 18: out[u] = vv_m; break;
-            case 19: out[u] = ww_m; break;
-            case 20: out[u] = nd_ave * K_B * T; break;
-            default: out[u] = 0;
-        }
-    }
+    // 12..21: t, t1, t2, t3, nd-ave, u-ave, v-ave, w-ave, p, mpc
+    SF_HIP(ctx, hipMemcpy(out, sp->gas + (size_t)(which - 12) * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
     return SF_OK;
 }
 

@@ -299,7 +299,10 @@ int sf_dsmc_create(sf_ctx* ctx, const sf_dsmc_desc* desc, sf_dsmc** out) {
     pp.spwt_b = B->desc.spwt;
     pp.k0 = (uint32_t)desc->seed;
     pp.k1 = (uint32_t)(desc->seed >> 32);
-    pp.tag = desc->tag;
+    {
+        int rc = claim_stream(ctx, desc->seed, RNG_MODULE_DSMC, desc->tag, &pp.tag);
+        if (rc) return rc;
+    }
     SF_HIP(ctx, hipSetDevice(ctx->device));
     const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj, nc = (size_t)ctx->mp.nci * ctx->mp.ncj;
     sf_dsmc* d = new sf_dsmc();
@@ -326,6 +329,7 @@ int sf_dsmc_create(sf_ctx* ctx, const sf_dsmc_desc* desc, sf_dsmc** out) {
         hipFree(d->d_ncol);
         hipFree(d->d_overflow);
         delete d;
+        release_stream(ctx, desc->seed, pp.tag);
         return sf_fail(ctx, SF_ERR_HIP, "sf_dsmc_create: %s", hipGetErrorString(e));
     }
     *out = d;
@@ -342,6 +346,7 @@ int sf_dsmc_destroy(sf_dsmc* d) {
     hipFree(d->d_real);
     hipFree(d->d_ncol);
     hipFree(d->d_overflow);
+    release_stream(d->ctx, d->desc.seed, d->pp.tag);
     delete d;
     return SF_OK;
 }

@@ -485,6 +485,18 @@ int sf_source_maxwellian(sf_ctx* ctx, int species, int npts, const double* pts, 
     return SF_OK;
 }
 
+int sf_source_options(sf_ctx* ctx, int source, double v_az, int start_it, int stop_it) {
+    SF_CHECK_CTX(ctx);
+    if (source < 0 || source >= (int)ctx->sources.size())
+        return sf_fail(ctx, SF_ERR_ARG, "sf_source_options: bad source index %d", source);
+    if (!std::isfinite(v_az)) return sf_fail(ctx, SF_ERR_ARG, "sf_source_options: v_az must be finite");
+    HostSource& s = ctx->sources[source];
+    s.v_az = v_az;
+    s.start_it = start_it;
+    s.stop_it = stop_it < 0 ? -1 : stop_it;
+    return SF_OK;
+}
+
 // SourceModule.sampleSources with the particles sampled on the device: the same particle counts
 // (mdot dt / (m spwt) plus the carried remainder), positions and velocities drawn from
 // per-particle Philox streams keyed by seed instead of the shared generator
@@ -494,14 +506,16 @@ int sf_sources_sample_device(sf_ctx* ctx, uint64_t seed, int64_t* injected) {
     SF_HIP(ctx, hipSetDevice(ctx->device));
     int64_t total = 0;
     const uint32_t step = ctx->inject_step++;
+    const int it = source_step(ctx);
     for (size_t q = 0; q < ctx->sources.size(); q++) {
         HostSource& s = ctx->sources[q];
+        if (!source_active(s, it)) continue;
         Species* sp = ctx->species[s.species];
         double mp = (s.mdot * ctx->dt) / (sp->mass * sp->desc.spwt) + s.rem;
         int num = (int)mp;
         s.rem = mp - num;
         int64_t added = 0;
-        int rc = inject_maxwellian_device(ctx, s, num, seed, step, (uint32_t)q, &added);
+        int rc = inject_maxwellian_device(ctx, s, num, seed, step, RNG_MODULE_INJECT | (uint32_t)q, &added);
         if (rc) return rc;
         s.added += added;
         total += added;
@@ -516,7 +530,9 @@ int sf_sources_sample(sf_ctx* ctx, int64_t* injected) {
     if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
     int64_t total = 0;
     const double dt = ctx->dt;
+    const int it = source_step(ctx);
     for (HostSource& s : ctx->sources) {
+        if (!source_active(s, it)) continue;
         Species* sp = ctx->species[s.species];
         double mp = (s.mdot * dt) / (sp->mass * sp->desc.spwt) + s.rem;
         int num = (int)mp;
@@ -562,7 +578,7 @@ int sf_sources_sample(sf_ctx* ctx, int64_t* injected) {
                 }
                 vel[0] = vm[0] + n0 * s.v_drift;
                 vel[1] = vm[1] + n1 * s.v_drift;
-                vel[2] = vm[2];
+                vel[2] = vm[2] + s.v_az;
                 (void)ctx->rng.next_double();    // part.dt = rnd*dt, overwritten by addParticle
             } while (n0 * vel[0] + n1 * vel[1] <= 0);
             num--;

@@ -19,6 +19,14 @@ constexpr double AMU = 1.66053886e-27;
 constexpr double FLT_EPS = 1e-7;
 constexpr double PI = 3.14159265358979323846;
 
+// The top four bits of the Philox counter's tag word name the module that draws, so that MCC, DSMC
+// and the device injection never share a stream, whatever tags their callers choose; the library
+// also refuses two live MCC (or DSMC) objects with the same seed and tag.
+constexpr uint32_t RNG_TAG_MASK = 0x0FFFFFFFu;
+constexpr uint32_t RNG_MODULE_MCC = 1u << 28;
+constexpr uint32_t RNG_MODULE_DSMC = 2u << 28;
+constexpr uint32_t RNG_MODULE_INJECT = 3u << 28;
+
 // Tile of cells owned by one workgroup of the push/deposit pass.
 constexpr int TILE_W = 32;                      // cells along i
 constexpr int TILE_H = 4;                       // cells along j
@@ -30,6 +38,10 @@ constexpr int TILE_CW = TILE_W + 2 * TILE_HALO; // cells incl. halo
 constexpr int TILE_CH = TILE_H + 2 * TILE_HALO;
 constexpr int TILE_CELLS = TILE_CW * TILE_CH;
 constexpr int NCH = 7;                          // deposit channels: n, nu, nv, nw, nuu, nvv, nww
+constexpr int NGAS = 10;                        // computeFields outputs kept on the device
+// bits of Species::flags, set by the push kernels
+constexpr int PUSH_FLAG_FAST = 1;               // a velocity beyond the fixed-point fast range: exact slow quantisation
+constexpr int PUSH_FLAG_OVERFLOW = 2;           // a moment contribution beyond the int64 range: moments invalid
 constexpr int PUSH_THREADS = 256;
 constexpr int CHUNK = 256;                      // particles staged in LDS per round
 constexpr int CHUNK_STRIDE = 151;               // odd stride of the lane->particle permutation
@@ -85,6 +97,9 @@ struct PushC {
     // moments (one value per group, so the push holds three scalars instead of seven)
     double kn, kv, kvv;
     double spwt;
+    // speeds beyond vfast need the exact (slow) quantisation, beyond vover a contribution does
+    // not fit an int64 (the step's moments are then flagged as invalid)
+    double vfast, vover;
 };
 
 struct Species {
@@ -97,10 +112,13 @@ struct Species {
     int since_sort = 0;
     int cur = 0;
     PState buf[2]{};
-    int32_t* key = nullptr;        // new cell key per particle (-1 dead)
+    int32_t* key = nullptr;        // new cell key per particle (-1 dead); the push's slow-path queue
+    int* qcount = nullptr;         // length of the slow-path queue of the last push
     unsigned long long* acc = nullptr;   // [NCH][nn] int64 deposit accumulators
-    long long* sums = nullptr;     // [NCH][nn] running moment sums (int64)
+    double* sums = nullptr;        // [NCH][nn] running moment sums, physical units (FP64)
     long long* mpc_sum = nullptr;  // [ncells]
+    double* gas = nullptr;         // [NGAS][nn] computeFields results (t, t1-t3, nd-ave, u/v/w-ave, p, mpc)
+    int* flags = nullptr;          // device status word of the push (SF_PUSH_FLAG_*)
     double* den = nullptr;         // [4][nn] nd, u, v, w
     int32_t* counts = nullptr;     // [ncells] histogram of new cells
     int32_t* offsets = nullptr;    // [ncells+1] exclusive scan (cell ranges of the sorted array)
@@ -109,6 +127,8 @@ struct Species {
     long long* stats = nullptr;    // [STAT_SLOTS][STAT_STRIDE] int64 fixed-point mover sums
     int id_counter = 0;
     int num_samples = 0;
+    bool first_time = true;        // KineticMaterial.first_time: computeFields on the first update
+    int64_t pushes = 0;            // sf_push_deposit calls (the step counter when none is set)
     double kscale[NCH];
     double stats_host[5] = {0, 0, 0, 0, 0};
 };
@@ -131,6 +151,8 @@ struct HostSource {
     std::vector<double> seg;       // x1,y1,x2,y2 per linear segment
     std::vector<double> cum;
     double area = 0, mdot = 0, v_drift = 0, v_th = 0, rem = 0;
+    double v_az = 0;               // MaxwellianSource v_az, added to the third velocity component
+    int start_it = 0, stop_it = -1;   // Source start_it / stop_it (end_it); -1: no end
     int64_t added = 0;             // particles this source has placed in the domain so far
     double* d_geom = nullptr;      // device copy of seg then cum, for sampling on the device
 };
@@ -161,6 +183,7 @@ struct sf_ctx {
     int32_t* d_nseg_idx = nullptr;
     sf::DevSeg* d_seg = nullptr;
     uint8_t* d_hasseg = nullptr;
+    uint8_t* d_cellslow = nullptr; // [ncells] 1: a particle of this cell may meet a segment or a mesh face
     double* d_node_vol = nullptr;
     // fields
     double* d_phi = nullptr;
@@ -172,11 +195,17 @@ struct sf_ctx {
     std::vector<sf::Species*> species;
     std::vector<sf::HostSource> sources;
     uint32_t inject_step = 0;      // batches sampled on the device (counter of their Philox streams)
+    int it = 0;                    // TimeModule step (Starfish.getIt()); 0 = not set by the host
+    int source_calls = 0;          // sf_sources_sample* calls: the sources' step while `it` is not set
+    // (seed, module-tagged stream tag) of every live MCC and DSMC object: no two share a stream
+    std::vector<std::pair<uint64_t, uint32_t>> rng_streams;
     sf::PoissonState* poisson = nullptr;
     bool timing = false;
-    hipEvent_t ev[6]{};
+    hipEvent_t ev[8]{};
     double last_ms[4] = {0, 0, 0, 0};
     double last_push_ms[3] = {0, 0, 0};   // tile kernel ms, tail kernel ms, tail slots
+    double last_queue[2] = {0, 0};        // queue kernel ms, particles the fast tile kernel queued
+    bool no_fast_push = false;            // SF_NO_FAST_PUSH set: every step runs the full tile kernel
     sf::MeshD meshd() const {
         sf::MeshD m;
         m.face = d_face;
@@ -205,6 +234,10 @@ int sf_fail(sf_ctx* ctx, int code, const char* fmt, ...);
     } while (0)
 
 namespace sf {
+// Claims the counter-based stream (seed, module | tag) for a new MCC or DSMC object; tag
+// SF_TAG_AUTO picks the smallest free tag.  Fails if another live object holds the stream.
+int claim_stream(sf_ctx* ctx, uint64_t seed, uint32_t module, uint32_t tag, uint32_t* stream_tag);
+void release_stream(sf_ctx* ctx, uint64_t seed, uint32_t stream_tag);
 int species_alloc_buffers(sf_ctx* ctx, Species* sp);
 int push_deposit(sf_ctx* ctx, Species* sp);
 int sort_species(sf_ctx* ctx, Species* sp);
@@ -220,4 +253,15 @@ int operator_apply(sf_ctx* ctx, const double* x, double* y);
 int preload(sf_ctx* ctx, Species* sp, int64_t n, const double* drift, double temp, uint64_t seed);
 int inject_maxwellian_device(sf_ctx* ctx, HostSource& s, int num, uint64_t seed, uint32_t step, uint32_t tag,
                              int64_t* added);
+// SourceModule.sampleSources' window (SourceModule.java:240-241): the step of this sampling call
+// (the host's step counter, else the number of sampling calls so far)
+inline int source_step(sf_ctx* ctx);
+inline bool source_active(const HostSource& s, int step) {
+    return !(step < s.start_it || (s.stop_it >= 0 && step > s.stop_it));
+}
 }  // namespace sf
+
+inline int sf::source_step(sf_ctx* ctx) {
+    ctx->source_calls++;
+    return ctx->it > 0 ? ctx->it : ctx->source_calls;
+}

@@ -146,6 +146,7 @@ struct sf_mcc {
     bool have_target = false;
     int num_samples = 0;
     double time_start = 0, time_last = 0;
+    uint32_t stream_tag = 0;             // module-tagged counter word of the Philox streams
 };
 
 static int grid_n(int64_t n, int max_blocks) {
@@ -169,11 +170,17 @@ int sf_mcc_create(sf_ctx* ctx, const sf_mcc_desc* desc, sf_mcc** out) {
         return sf_fail(ctx, SF_ERR_UNSUPPORTED, "sf_mcc_create: model %d is not supported", desc->model);
     if (!(desc->sigma >= 0) || !(desc->target_molwt > 0))
         return sf_fail(ctx, SF_ERR_ARG, "sf_mcc_create: sigma must be >= 0 and target_molwt > 0");
+    uint32_t stream_tag = 0;
+    {
+        int rc = claim_stream(ctx, desc->seed, RNG_MODULE_MCC, desc->tag, &stream_tag);
+        if (rc) return rc;
+    }
     SF_HIP(ctx, hipSetDevice(ctx->device));
     const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj;
     sf_mcc* c = new sf_mcc();
     c->ctx = ctx;
     c->desc = *desc;
+    c->stream_tag = stream_tag;
     if (c->desc.frequency < 1) c->desc.frequency = 1;
     c->target_mass = desc->target_molwt * AMU;
     hipError_t e = hipMalloc((void**)&c->d_target, 5 * nn * sizeof(double));
@@ -189,6 +196,7 @@ int sf_mcc_create(sf_ctx* ctx, const sf_mcc_desc* desc, sf_mcc** out) {
         hipFree(c->d_S);
         hipFree(c->d_ncol);
         delete c;
+        release_stream(ctx, desc->seed, stream_tag);
         return sf_fail(ctx, SF_ERR_HIP, "sf_mcc_create: %s", hipGetErrorString(e));
     }
     *out = c;
@@ -203,6 +211,7 @@ int sf_mcc_destroy(sf_mcc* c) {
     hipFree(c->d_count);
     hipFree(c->d_S);
     hipFree(c->d_ncol);
+    release_stream(c->ctx, c->desc.seed, c->stream_tag);
     delete c;
     return SF_OK;
 }
@@ -249,7 +258,7 @@ int sf_mcc_perform(sf_mcc* c, int it, double time, int64_t* collisions) {
     pp.k0 = (uint32_t)c->desc.seed;
     pp.k1 = (uint32_t)(c->desc.seed >> 32);
     pp.step = (uint32_t)it;
-    pp.tag = c->desc.tag;
+    pp.tag = c->stream_tag;
     SF_HIP(ctx, hipMemsetAsync(c->d_ncol, 0, sizeof(unsigned long long), ctx->stream));
     if (sp->np > 0) {
         hipLaunchKernelGGL(mcc_kernel, dim3(grid_n(sp->np, 1 << 16)), dim3(256), 0, ctx->stream, ctx->mp,

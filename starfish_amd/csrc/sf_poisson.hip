@@ -57,7 +57,7 @@ struct PoissonState {
 };
 
 // device scalars layout (FP64): PCG tolerance, squared norms from the finalize kernel
-enum { S_TOL2 = 0, S_RZ0, S_NORM2, S_R0, S_YMAX, S_NSCAL = 8 };
+enum { S_TOL2 = 0, S_RZ0, S_NORM2, S_R0, S_YMAX, S_YNORM2, S_NSCAL = 8 };
 
 // Reductions without grid-wide synchronisation.  A kernel leaves one partial per block
 // (block_partials); the next kernel on the stream combines them (combine_partials, every
@@ -504,11 +504,17 @@ __global__ void absmax_kernel(int64_t nn, const double* __restrict__ y, double* 
 // x += limit(y) (Newton update, Solver.java:366-368 with the sign of the reduced system); y = 0.
 // kte > 0 limits each node's update: a rise above kTe is damped to kTe (1 + ln(y / kTe)), a fall
 // is clipped at -10 kTe; kte <= 0 takes the full step.
-__global__ void newton_update(int64_t nn, const uint8_t* __restrict__ cls, double* __restrict__ x,
-                              double* __restrict__ y, double kte, const int* __restrict__ go) {
+// x += limited y on the free nodes; y is cleared.  ypart receives per-block partial sums of y^2
+// (the Newton step before the limit), the quantity of the reference's convergence test
+// ||y||_2 / N < nl_tol (Solver.java:371-381, Vec.norm).
+__global__ void __launch_bounds__(256) newton_update(int64_t nn, const uint8_t* __restrict__ cls,
+                                                     double* __restrict__ x, double* __restrict__ y, double kte,
+                                                     const int* __restrict__ go, double* __restrict__ ypart) {
     if (go != nullptr && go[0] != 1) return;
+    double v2[1] = {0.0};
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
         double v = y[u];
+        if (cls[u] != CLS_NOT_FREE) v2[0] += v * v;
         if (kte > 0) {
             if (v > kte) v = kte * (1.0 + log(v / kte));
             v = fmax(v, -10.0 * kte);
@@ -516,6 +522,7 @@ __global__ void newton_update(int64_t nn, const uint8_t* __restrict__ cls, doubl
         if (cls[u] != CLS_NOT_FREE) x[u] += v;
         y[u] = 0;
     }
+    block_partials<1>(v2, ypart);
 }
 
 // full potential from the free-node solution: fixed nodes take their value, face nodes the
@@ -766,6 +773,10 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     const int nl_max = D.linear ? 1 : (D.nl_max_it > 0 ? D.nl_max_it : 50);
     double* pz0 = P->pz;
     double* pz1 = P->pz + 2 * std::max(ge, gt);   // partials of both parities, by block
+    double* ypart = P->partial + 2 * std::max(ge, gt);   // Newton step norm partials (not overwritten by residuals)
+    // the reference's Newton stop ||y||_2 / N < nl_tol (N = all nodes), used when nl_tol > 0
+    const double nl_tol = D.linear ? 0.0 : D.nl_tol;
+    bool nl_stop = false;
     SF_HIP(ctx, hipEventRecord(P->e0, st));
     hipLaunchKernelGGL(gather_free, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, (const double*)ctx->d_phi,
                        P->x);
@@ -821,6 +832,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         rel = rnorm / ref;
         if (!std::isfinite(rel)) return sf_fail(ctx, SF_ERR_STATE, "Newton residual is not finite (diverged)");
         if (rel <= rel_tol) { converged = true; break; }
+        if (nl_stop) { converged = true; break; }    // the reference's criterion met on the last step
         if (it == nl_max) break;
         if (!eta_chosen) {
             eta = rel < 0.5 ? 1e-2 : 1e-3;
@@ -880,7 +892,8 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
                     const int* go = (const int*)P->istate;
                     hipLaunchKernelGGL(accumulate_y, dim3(ge), dim3(256), 0, st, nn, (const float*)P->d, P->y, go);
                     hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x,
-                                       P->y, lim, go);
+                                       P->y, lim, go, ypart);
+                    finalize<1>(P, st, ypart, ge, S_YNORM2);
                     launch_residual(go);
                 }
                 SF_HIP(ctx, hipGetLastError());
@@ -910,6 +923,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         if (stepped) {
             have_residual = true;
             SF_HIP(ctx, hipGetLastError());
+            if (nl_tol > 0 && std::sqrt(P->host_scal[S_YNORM2]) / (double)nn < nl_tol) nl_stop = true;
             continue;
         }
         if (debug && !D.linear) {
@@ -917,7 +931,13 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             finalize<1, true>(P, st, P->partial, ge, S_YMAX);
         }
         hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y, lim,
-                           nullptr);
+                           nullptr, ypart);
+        finalize<1>(P, st, ypart, ge, S_YNORM2);
+        if (nl_tol > 0) {
+            rc = read_scalars(ctx, P);
+            if (rc) return rc;
+            if (std::sqrt(P->host_scal[S_YNORM2]) / (double)nn < nl_tol) nl_stop = true;
+        }
         if (debug) {
             rc = read_scalars(ctx, P);
             if (rc) return rc;
@@ -1114,6 +1134,10 @@ int sf_efield_set(sf_ctx* ctx, const double* efi, const double* efj) {
     if (!ctx->mesh_ready || !efi || !efj) return sf_fail(ctx, SF_ERR_ARG, "sf_efield_set: bad state or argument");
     SF_HIP(ctx, hipSetDevice(ctx->device));
     const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj;
+    // a non-finite E would turn particle states into NaN, which no fixed-point deposit can hold
+    for (size_t u = 0; u < nn; u++)
+        if (!std::isfinite(efi[u]) || !std::isfinite(efj[u]))
+            return sf_fail(ctx, SF_ERR_ARG, "sf_efield_set: non-finite value at node %zu", u);
     SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     SF_HIP(ctx, hipMemcpy(ctx->d_efi, efi, nn * 8, hipMemcpyHostToDevice));
     SF_HIP(ctx, hipMemcpy(ctx->d_efj, efj, nn * 8, hipMemcpyHostToDevice));

@@ -49,6 +49,28 @@ __device__ __forceinline__ void deposit_global(const MeshP& m, unsigned long lon
     }
 }
 
+// Deposit of a particle whose speed is beyond the range of the one-FMA quantisation (a
+// contribution of 2^51 or more): rounded with a full conversion instead, into global memory.
+// Deterministic like the fast path (a particle takes one path or the other by its own speed).
+__device__ __forceinline__ void deposit_global_exact(const MeshP& m, unsigned long long* acc, size_t nn, int i, int j,
+                                                     double di, double dj, const double* val) {
+    double w[4];
+    scatter_weights(m, i, j, di, dj, w);
+    const int ci[4] = {0, 1, 1, 0}, cj[4] = {0, 0, 1, 1};
+    for (int c = 0; c < 4; c++) {
+        size_t u = (size_t)(j + cj[c]) * m.ni + (i + ci[c]);
+        for (int ch = 0; ch < NCH; ch++) {
+            long long q = __double2ll_rn(w[c] * val[ch]);
+            if (q != 0) atomicAdd(acc + ch * nn + u, (unsigned long long)q);
+        }
+    }
+}
+
+// Largest velocity component: the one-FMA quantisation needs it below pc.vfast.
+__device__ __forceinline__ double vmax_component(double v0, double v1, double v2) {
+    return fmax(fabs(v0), fmax(fabs(v1), fabs(v2)));
+}
+
 // Load of the FP32 particle state, conversion to logical coordinates.
 struct PIn {
     int ci, cj;
@@ -170,7 +192,8 @@ template <bool HAS_B>
 __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const MeshD& md, const PushC& pc,
                                                        const PState& s, const int32_t* __restrict__ offsets,
                                                        unsigned long long* __restrict__ acc,
-                                                       int32_t* __restrict__ counts, long long* stats) {
+                                                       int32_t* __restrict__ counts, long long* stats,
+                                                       int* flags) {
     __shared__ TileLds L;
     const int t = threadIdx.x;
     StatAcc st{0, 0, 0, 0, 0};
@@ -310,7 +333,11 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
                     double val[NCH];
                     channel_values(pc, vxn, vyn, vzn, val);
                     int li = ci - n_i0, lj = cj - n_j0;
-                    if (li >= 0 && li + 1 < TILE_NW && lj >= 0 && lj + 1 < TILE_NH) {
+                    const double vm = vmax_component(vxn, vyn, vzn);
+                    if (vm >= pc.vfast) {
+                        deposit_global_exact(m, acc, nn, ci, cj, di, dj, val);
+                        atomicOr(flags, vm >= pc.vover ? (PUSH_FLAG_FAST | PUSH_FLAG_OVERFLOW) : PUSH_FLAG_FAST);
+                    } else if (li >= 0 && li + 1 < TILE_NW && lj >= 0 && lj + 1 < TILE_NH) {
                         double w4[4];
                         scatter_weights(m, ci, cj, di, dj, w4);
                         int l0 = lj * TILE_NW + li;
@@ -393,57 +420,378 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_deposit_tile_inplace(Mes
                                                                            const int32_t* __restrict__ offsets,
                                                                            unsigned long long* __restrict__ acc,
                                                                            int32_t* __restrict__ counts,
-                                                                           long long* stats) {
-    push_deposit_tile_body<false>(m, md, pc, s, offsets, acc, counts, stats);
+                                                                           long long* stats, int* flags) {
+    push_deposit_tile_body<false>(m, md, pc, s, offsets, acc, counts, stats, flags);
 }
 // The same pass with a magnetic field loaded (Boris where the gathered B is nonzero).
 __global__ void __launch_bounds__(PUSH_THREADS) push_deposit_tile_inplace_b(MeshP m, MeshD md, PushC pc, PState s,
                                                                           const int32_t* __restrict__ offsets,
                                                                           unsigned long long* __restrict__ acc,
                                                                           int32_t* __restrict__ counts,
-                                                                          long long* stats) {
-    push_deposit_tile_body<true>(m, md, pc, s, offsets, acc, counts, stats);
+                                                                          long long* stats, int* flags) {
+    push_deposit_tile_body<true>(m, md, pc, s, offsets, acc, counts, stats, flags);
 }
 
 // ----------------------------------------------------------------------------------------
-// push of the unsorted tail (particles added since the last sort): global atomics
+// push of one particle with global memory only: the reference loop in FP64 (every boundary
+// case), deposit and histogram with global atomics.  Used for the unsorted tail (particles added
+// since the last sort) and for the particles the fast tile kernel leaves to it.
 // ----------------------------------------------------------------------------------------
 template <bool HAS_B>
-__global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, PushC pc, PState s, int64_t p0,
-                                                         int64_t p1, unsigned long long* __restrict__ acc,
-                                                         int32_t* __restrict__ counts, long long* stats) {
+__device__ __forceinline__ void push_one_global(const MeshP& m, const MeshD& md, const PushC& pc, const PState& s,
+                                                int64_t p, unsigned long long* __restrict__ acc,
+                                                int32_t* __restrict__ counts, int* flags, StatAcc& st) {
     const size_t nn = (size_t)m.ni * m.nj;
     auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
     auto hasseg = [&](int i, int j) -> int { return md.node_hasseg[(size_t)j * m.ni + i]; };
+    PIn q = load_particle(s, p);
+    bool alive = push_particle<HAS_B>(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
+    int32_t key = -1;
+    if (alive) {
+        int ci, cj;
+        float fxn, fyn;
+        cell_offsets(m, q.lc0, q.lc1, ci, cj, fxn, fyn);
+        key = pack_cell(ci, cj);
+        float vxn = (float)q.v0, vyn = (float)q.v1, vzn = (float)q.v2;
+        stat_add(st, pc, vxn, vyn, vzn);
+        atomicAdd(&counts[cj * m.nci + ci], 1);
+        double di = fxn, dj = fyn;
+        if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
+            double val[NCH];
+            channel_values(pc, vxn, vyn, vzn, val);
+            const double vm = vmax_component(vxn, vyn, vzn);
+            if (vm >= pc.vfast) {
+                deposit_global_exact(m, acc, nn, ci, cj, di, dj, val);
+                atomicOr(flags, vm >= pc.vover ? (PUSH_FLAG_FAST | PUSH_FLAG_OVERFLOW) : PUSH_FLAG_FAST);
+            } else {
+                deposit_global(m, acc, nn, ci, cj, di, dj, val);
+            }
+        }
+        s.fx[p] = fxn;
+        s.fy[p] = fyn;
+        s.vx[p] = vxn;
+        s.vy[p] = vyn;
+        s.vz[p] = vzn;
+    }
+    s.cell[p] = key;
+}
+
+template <bool HAS_B>
+__global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, PushC pc, PState s, int64_t p0,
+                                                         int64_t p1, unsigned long long* __restrict__ acc,
+                                                         int32_t* __restrict__ counts, long long* stats,
+                                                         int* flags) {
     StatAcc st{0, 0, 0, 0, 0};
     for (int64_t p = p0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < p1;
          p += (int64_t)gridDim.x * blockDim.x) {
         if (s.cell[p] < 0) continue;   // hole left by a dead particle
-        PIn q = load_particle(s, p);
-        bool alive = push_particle<HAS_B>(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
-        int32_t key = -1;
-        if (alive) {
-            int ci, cj;
-            float fxn, fyn;
-            cell_offsets(m, q.lc0, q.lc1, ci, cj, fxn, fyn);
-            key = pack_cell(ci, cj);
-            float vxn = (float)q.v0, vyn = (float)q.v1, vzn = (float)q.v2;
-            stat_add(st, pc, vxn, vyn, vzn);
-            atomicAdd(&counts[cj * m.nci + ci], 1);
-            double di = fxn, dj = fyn;
-            if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
-                double val[NCH];
-                channel_values(pc, vxn, vyn, vzn, val);
-                deposit_global(m, acc, nn, ci, cj, di, dj, val);
-            }
-            s.fx[p] = fxn;
-            s.fy[p] = fyn;
-            s.vx[p] = vxn;
-            s.vy[p] = vyn;
-            s.vz[p] = vzn;
-        }
-        s.cell[p] = key;
+        push_one_global<HAS_B>(m, md, pc, s, p, acc, counts, flags, st);
     }
+    stat_flush(st, stats);
+}
+
+// The particles the fast tile kernel queued (slots of the cell-sorted array).
+__global__ void __launch_bounds__(256) push_deposit_queue(MeshP m, MeshD md, PushC pc, PState s,
+                                                          const int32_t* __restrict__ queue,
+                                                          const int* __restrict__ qcount,
+                                                          unsigned long long* __restrict__ acc,
+                                                          int32_t* __restrict__ counts, long long* stats,
+                                                          int* flags) {
+    StatAcc st{0, 0, 0, 0, 0};
+    const int nq = *qcount;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nq; k += gridDim.x * blockDim.x)
+        push_one_global<false>(m, md, pc, s, queue[k], acc, counts, flags, st);
+    stat_flush(st, stats);
+}
+
+// ----------------------------------------------------------------------------------------
+// fast tile kernel (XY, no magnetic field): the push of every particle whose cell is away from
+// segments and mesh faces, in FP32 relative to its cell; everything else goes to the queue.
+//
+// A particle of a cell without the slow flag that moves less than one cell in each direction
+// (and is not faster than the fixed-point fast range) is exactly the reference loop's first and
+// only sub-move: ProcessBoundary finds no segment in its bounding box and no face crossing, so
+// the particle lives and its new position is old + v dt.  In cell units that is
+// f' = f + v dt/dh with the new cell i + floor(f'), evaluated in FP32 (the stored precision).
+// The deposit adds the raw bit patterns fma(w, value, 1.5 2^52) of the fixed-point rounding into
+// LDS; every particle adds exactly one such pattern per channel to each corner of its new cell,
+// so the flush subtracts (particles of the four adjacent cells) x bits(1.5 2^52), counted by the
+// tile's cell histogram.  Integer sums modulo 2^64 keep this exact and order independent.
+// ----------------------------------------------------------------------------------------
+struct FastC {
+    float qmdt, cdx, cdy, vfast;   // q/m dt, dt/dx, dt/dy, fast range of the quantisation
+    double kn, kv, kvv;            // fixed-point scales
+    int ni, nj, nci, ncj, ntx;
+};
+
+struct FastLds {
+    unsigned long long acc[NCH][TILE_NODES];
+    int hist[TILE_CELLS];
+    float2 ef[TILE_NODES];
+    uint8_t slow[TILE_CELLS];
+    int seg_start[TILE_H], seg_pre[TILE_H + 1];
+    int32_t o_cell[PUSH_THREADS];
+    float o_fx[PUSH_THREADS], o_fy[PUSH_THREADS], o_vx[PUSH_THREADS], o_vy[PUSH_THREADS], o_vz[PUSH_THREADS];
+};
+
+constexpr int32_t SLOT_KEEP = -2;   // staging marker: the slot is not written by this kernel
+
+__device__ __forceinline__ unsigned long long qbits(double w, double v) {
+    return (unsigned long long)__double_as_longlong(fma(w, v, QMAGIC));
+}
+
+__global__ void __launch_bounds__(PUSH_THREADS, 4) push_tile_fast_xy(FastC fc, const float2* __restrict__ ef,
+                                                                    const uint8_t* __restrict__ cellslow, PState s,
+                                                                    const int32_t* __restrict__ offsets,
+                                                                    unsigned long long* __restrict__ acc,
+                                                                    int32_t* __restrict__ counts,
+                                                                    long long* __restrict__ stats,
+                                                                    int32_t* __restrict__ queue,
+                                                                    int* __restrict__ qcount) {
+    __shared__ FastLds L;
+    const int t = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int tx = tile % fc.ntx, ty = tile / fc.ntx;
+    const int ti0 = tx * TILE_W, tj0 = ty * TILE_H;
+    // the LDS window: nodes and cells from (ti0 - HALO, tj0 - HALO); node (a, b) of the window is
+    // corner (0, 0) of window cell (a, b)
+    const int w_i0 = ti0 - TILE_HALO, w_j0 = tj0 - TILE_HALO;
+    const size_t nn = (size_t)fc.ni * fc.nj;
+    const int pt = (t * CHUNK_STRIDE) & (PUSH_THREADS - 1);
+    const unsigned long long MAGIC_BITS = (unsigned long long)__double_as_longlong(QMAGIC);
+
+    for (int k = t; k < NCH * TILE_NODES; k += PUSH_THREADS) (&L.acc[0][0])[k] = 0ull;
+    for (int k = t; k < TILE_CELLS; k += PUSH_THREADS) {
+        L.hist[k] = 0;
+        const int i = w_i0 + k % TILE_CW, j = w_j0 + k / TILE_CW;
+        L.slow[k] = (i >= 0 && j >= 0 && i < fc.nci && j < fc.ncj) ? cellslow[(size_t)j * fc.nci + i] : 1;
+    }
+    for (int k = t; k < TILE_NODES; k += PUSH_THREADS) {
+        int i = w_i0 + k % TILE_NW, j = w_j0 + k / TILE_NW;
+        i = min(max(i, 0), fc.ni - 1);
+        j = min(max(j, 0), fc.nj - 1);
+        L.ef[k] = ef[(size_t)j * fc.ni + i];
+    }
+    int first_row = TILE_H, first_start = 0, first_len = 0;
+    {
+        int pre = 0;
+#pragma unroll
+        for (int r = 0; r < TILE_H; r++) {
+            const int row = tj0 + r;
+            int st = 0, len = 0;
+            if (row < fc.ncj) {
+                const int k0 = row * fc.nci + ti0;
+                const int k1 = row * fc.nci + min(ti0 + TILE_W, fc.nci);
+                st = offsets[k0];
+                len = offsets[k1] - st;
+            }
+            if (first_row == TILE_H && len > 0) {
+                first_row = r;
+                first_start = st;
+                first_len = len;
+            }
+            if (t == 0) {
+                L.seg_start[r] = st;
+                L.seg_pre[r] = pre;
+            }
+            pre += len;
+        }
+        if (t == 0) L.seg_pre[TILE_H] = pre;
+    }
+    PReg cur;
+    if (pt < first_len) {
+        cur.g = first_start + pt;
+        cur.c = s.cell[cur.g];
+        cur.fx = s.fx[cur.g];
+        cur.fy = s.fy[cur.g];
+        cur.vx = s.vx[cur.g];
+        cur.vy = s.vy[cur.g];
+        cur.vz = s.vz[cur.g];
+    } else {
+        cur.g = 0;
+        cur.c = -1;
+        cur.fx = cur.fy = cur.vx = cur.vy = cur.vz = 0.0f;
+    }
+    __syncthreads();
+
+    auto row_len = [&](int r) { return L.seg_pre[r + 1] - L.seg_pre[r]; };
+    // per-lane mover sums: raw FMA bit patterns, the bias removed with the lane's count at the end
+    long long sn = 0;
+    unsigned long long sp0 = 0, sp1 = 0, sp2 = 0, se = 0;
+    int cr = first_row, cb = 0;
+    while (cr < TILE_H) {
+        const int rlen = row_len(cr);
+        const int64_t rst = L.seg_start[cr];
+        const bool ok = cb + pt < rlen;
+        int nr = cr, nb = cb + PUSH_THREADS;
+        if (nb >= rlen) {
+            nr = cr + 1;
+            nb = 0;
+            while (nr < TILE_H && row_len(nr) == 0) nr++;
+        }
+        int32_t key = SLOT_KEEP;
+        float fxn = 0.f, fyn = 0.f, vxn = 0.f, vyn = 0.f, vzn = 0.f;
+        bool slow = false;
+        if (ok && cur.c >= 0) {
+            const int ci = cur.c & 0xFFFF, cj = (cur.c >> 16) & 0xFFFF;
+            const int hi = ci - w_i0, hj = cj - w_j0;
+            bool fast = (unsigned)hi < (unsigned)TILE_CW && (unsigned)hj < (unsigned)TILE_CH;
+            if (fast) fast = L.slow[hj * TILE_CW + hi] == 0;
+            if (fast) {
+                // Field2D.gather: bilinear in the cell (the slow flag keeps the cell off the last node line)
+                const int l = hj * TILE_NW + hi;
+                const float2 e00 = L.ef[l], e10 = L.ef[l + 1], e01 = L.ef[l + TILE_NW], e11 = L.ef[l + TILE_NW + 1];
+                const float gx = 1.0f - cur.fx, gy = 1.0f - cur.fy;
+                const float ex = gy * (gx * e00.x + cur.fx * e10.x) + cur.fy * (gx * e01.x + cur.fx * e11.x);
+                const float ey = gy * (gx * e00.y + cur.fx * e10.y) + cur.fy * (gx * e01.y + cur.fx * e11.y);
+                vxn = cur.vx + fc.qmdt * ex;
+                vyn = cur.vy + fc.qmdt * ey;
+                vzn = cur.vz;
+                const float dx = vxn * fc.cdx, dy = vyn * fc.cdy;
+                fast = fabsf(dx) < 1.0f && fabsf(dy) < 1.0f &&
+                       fmaxf(fabsf(vxn), fmaxf(fabsf(vyn), fabsf(vzn))) < fc.vfast;
+                if (fast) {
+                    const float xn = cur.fx + dx, yn = cur.fy + dy;
+                    const float flx = floorf(xn), fly = floorf(yn);
+                    fxn = xn - flx;
+                    fyn = yn - fly;
+                    int nci = ci + (int)flx, ncj = cj + (int)fly;
+                    // x' - floor(x') can round up to 1 for x' just below an integer: that position
+                    // is offset 0 of the next cell (the stored cell is floor(lc))
+                    if (fxn >= 1.0f) { fxn = 0.0f; nci++; }
+                    if (fyn >= 1.0f) { fyn = 0.0f; ncj++; }
+                    key = pack_cell(nci, ncj);
+                    // mover sums (KineticMaterial.java:247-259, 406-413)
+                    sn++;
+                    sp0 += qbits(1.0, (double)vxn * fc.kv);
+                    sp1 += qbits(1.0, (double)vyn * fc.kv);
+                    sp2 += qbits(1.0, (double)vzn * fc.kv);
+                    se += qbits(1.0, (double)sqrtf(vxn * vxn + vyn * vyn + vzn * vzn) * fc.kv);
+                    // deposit of the 7 channels from the stored FP32 state (Field2D.scatter)
+                    const double dfx = fxn, dfy = fyn;
+                    const double w00 = (1.0 - dfx) * (1.0 - dfy), w10 = dfx * (1.0 - dfy), w11 = dfx * dfy,
+                                 w01 = (1.0 - dfx) * dfy;
+                    const double v0 = vxn, v1 = vyn, v2 = vzn;
+                    const double val[NCH] = {fc.kn, v0 * fc.kv, v1 * fc.kv, v2 * fc.kv, v0 * v0 * fc.kvv,
+                                             v1 * v1 * fc.kvv, v2 * v2 * fc.kvv};
+                    const int hn = nci - w_i0, hm = ncj - w_j0;
+                    if ((unsigned)hn < (unsigned)TILE_CW && (unsigned)hm < (unsigned)TILE_CH) {
+                        atomicAdd(&L.hist[hm * TILE_CW + hn], 1);
+                        const int l0 = hm * TILE_NW + hn;
+                        // lanes of one cell start at different corners (lane mod 4), so the
+                        // same-address conflicts of an instruction drop about fourfold
+                        const int rot = t & 3;
+                        double wr[4];
+                        int offr[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const int cq = (q + rot) & 3;
+                            wr[q] = cq == 0 ? w00 : (cq == 1 ? w10 : (cq == 2 ? w11 : w01));
+                            offr[q] = cq == 0 ? 0 : (cq == 1 ? 1 : (cq == 2 ? TILE_NW + 1 : TILE_NW));
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; q++)
+#pragma unroll
+                            for (int ch = 0; ch < NCH; ch++) atomicAdd(&L.acc[ch][l0 + offr[q]], qbits(wr[q], val[ch]));
+                    } else {
+                        atomicAdd(&counts[ncj * fc.nci + nci], 1);
+                        const double w4[4] = {w00, w10, w11, w01};
+                        const int ci4[4] = {0, 1, 1, 0}, cj4[4] = {0, 0, 1, 1};
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            const size_t u = (size_t)(ncj + cj4[c]) * fc.ni + (nci + ci4[c]);
+#pragma unroll
+                            for (int ch = 0; ch < NCH; ch++) {
+                                const long long qv = quant(w4[c], val[ch]);
+                                if (qv != 0) atomicAdd(acc + ch * nn + u, (unsigned long long)qv);
+                            }
+                        }
+                    }
+                }
+            }
+            slow = !fast;
+        }
+        // slow particles: the slot goes to the queue kernel, which writes it
+        {
+            const unsigned long long mk = __ballot(slow);
+            if (mk) {
+                const int lane = t & 63;
+                const int leader = __ffsll((long long)mk) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(qcount, __popcll(mk));
+                base = __shfl(base, leader);
+                if (slow) queue[base + __popcll(mk & ((1ull << lane) - 1ull))] = (int32_t)cur.g;
+            }
+        }
+        if (ok) {
+            L.o_cell[pt] = key;
+            L.o_fx[pt] = fxn;
+            L.o_fy[pt] = fyn;
+            L.o_vx[pt] = vxn;
+            L.o_vy[pt] = vyn;
+            L.o_vz[pt] = vzn;
+        }
+        lds_barrier();
+        const int wlen = min(PUSH_THREADS, rlen - cb);
+        if (t < wlen) {
+            const int32_t kc = L.o_cell[t];
+            if (kc != SLOT_KEEP) {
+                const int64_t g = rst + cb + t;
+                s.cell[g] = kc;
+                s.fx[g] = L.o_fx[t];
+                s.fy[g] = L.o_fy[t];
+                s.vx[g] = L.o_vx[t];
+                s.vy[g] = L.o_vy[t];
+                s.vz[g] = L.o_vz[t];
+            }
+        }
+        if (nr < TILE_H) {
+            if (nb + pt < row_len(nr)) {
+                cur.g = L.seg_start[nr] + nb + pt;
+                cur.c = s.cell[cur.g];
+                cur.fx = s.fx[cur.g];
+                cur.fy = s.fy[cur.g];
+                cur.vx = s.vx[cur.g];
+                cur.vy = s.vy[cur.g];
+                cur.vz = s.vz[cur.g];
+            } else {
+                cur.g = 0;
+                cur.c = -1;
+            }
+        }
+        lds_barrier();
+        cr = nr;
+        cb = nb;
+    }
+    __syncthreads();
+    // flush: node (a, b) of the window received one pattern per channel from every particle of
+    // window cells (a, b), (a-1, b), (a, b-1), (a-1, b-1)
+    for (int k = t; k < TILE_NODES; k += PUSH_THREADS) {
+        const int a = k % TILE_NW, b = k / TILE_NW;
+        const int i = w_i0 + a, j = w_j0 + b;
+        if (i < 0 || j < 0 || i >= fc.ni || j >= fc.nj) continue;
+        int np = 0;
+        if (a < TILE_CW && b < TILE_CH) np += L.hist[b * TILE_CW + a];
+        if (a > 0 && b < TILE_CH) np += L.hist[b * TILE_CW + a - 1];
+        if (a < TILE_CW && b > 0) np += L.hist[(b - 1) * TILE_CW + a];
+        if (a > 0 && b > 0) np += L.hist[(b - 1) * TILE_CW + a - 1];
+        if (np == 0) continue;
+        const unsigned long long bias = (unsigned long long)np * MAGIC_BITS;
+        const size_t u = (size_t)j * fc.ni + i;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) {
+            const unsigned long long v = L.acc[ch][k] - bias;
+            if (v != 0ull) atomicAdd(acc + ch * nn + u, v);
+        }
+    }
+    for (int k = t; k < TILE_CELLS; k += PUSH_THREADS) {
+        const int h = L.hist[k];
+        if (h == 0) continue;
+        const int ci = w_i0 + k % TILE_CW, cj = w_j0 + k / TILE_CW;
+        atomicAdd(&counts[cj * fc.nci + ci], h);
+    }
+    const unsigned long long sb = (unsigned long long)sn * MAGIC_BITS;
+    StatAcc st{sn, (long long)(sp0 - sb), (long long)(sp1 - sb), (long long)(sp2 - sb), (long long)(se - sb)};
     stat_flush(st, stats);
 }
 
@@ -622,10 +970,14 @@ __global__ void __launch_bounds__(256) copy_scatter(MeshP m, PState s, PState d,
 // node-wise normalisation (KineticMaterial.updateFields tail, Field2D.divideByField and
 // scaleByVol, Material.update nd_back) and sample accumulation (updateSamples)
 // ----------------------------------------------------------------------------------------
+// The running sums are FP64 in physical units (mpw-weighted): each step adds its exact integer
+// totals, converted once, so they are deterministic and cannot wrap however long sampling runs.
+// fs_n, fs_v, fs_vv are spwt / 2^k of the density, velocity and second-moment channels.
 __global__ void __launch_bounds__(256) normalize_nodes(int64_t nn, unsigned long long* __restrict__ acc,
-                                                       long long* __restrict__ sums, const double* __restrict__ vol,
+                                                       double* __restrict__ sums, const double* __restrict__ vol,
                                                        double* __restrict__ den, double inv_kw, double inv_kv,
-                                                       double spwt, double nd_back, int samples) {
+                                                       double spwt, double nd_back, int samples, double fs_n,
+                                                       double fs_v, double fs_vv) {
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
         long long a[NCH];
 #pragma unroll
@@ -635,7 +987,10 @@ __global__ void __launch_bounds__(256) normalize_nodes(int64_t nn, unsigned long
         }
         if (samples) {
 #pragma unroll
-            for (int ch = 0; ch < NCH; ch++) sums[ch * nn + u] += a[ch];
+            for (int ch = 0; ch < NCH; ch++) {
+                const double f = ch == 0 ? fs_n : (ch < 4 ? fs_v : fs_vv);
+                sums[ch * nn + u] += (double)a[ch] * f;
+            }
         }
         double wsum = (double)a[0] * inv_kw;
         double nd = wsum * spwt / vol[u];
@@ -652,16 +1007,58 @@ __global__ void __launch_bounds__(256) normalize_nodes(int64_t nn, unsigned long
     }
 }
 
+// KineticMaterial.computeFields(MeshData) (core/materials/KineticMaterial.java:1606-1671) over
+// the running sums, in the reference's order of operations: temperatures where count-sum > 0
+// (init_vals.T elsewhere), nd-ave = count-sum * (1/num_samples) / nodeVol, u/v/w-ave = sum /
+// count-sum (0 where the count is 0, Field2D.divideByField), p = nd-ave K T, and mpc =
+// mpc-sum * (1/num_samples) for the cells (row j*ni + i of cell (i, j), 0 on the last node row
+// and column).  gas: [NGAS][nn] t, t1, t2, t3, nd-ave, u-ave, v-ave, w-ave, p, mpc.
+__global__ void __launch_bounds__(256) compute_fields_kernel(MeshP m, const double* __restrict__ sums,
+                                                             const long long* __restrict__ mpc_sum,
+                                                             const double* __restrict__ vol,
+                                                             double* __restrict__ gas, double f, double init_T,
+                                                             double inv_ns) {
+    const int64_t nn = (int64_t)m.ni * m.nj;
+    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
+        const double count = sums[u];
+        double T = init_T, T1 = init_T, T2 = init_T, T3 = init_T;
+        if (count > 0) {
+            const double uu = sums[nn + u] / count, vv = sums[2 * nn + u] / count, ww = sums[3 * nn + u] / count;
+            const double uu2 = sums[4 * nn + u] / count - uu * uu;
+            const double vv2 = sums[5 * nn + u] / count - vv * vv;
+            const double ww2 = sums[6 * nn + u] / count - ww * ww;
+            T = (uu2 + vv2 + ww2) * f / 3.0;
+            T1 = uu2 * f;
+            T2 = vv2 * f;
+            T3 = ww2 * f;
+        }
+        const double nd_ave = count * inv_ns / vol[u];
+        gas[u] = T;
+        gas[nn + u] = T1;
+        gas[2 * nn + u] = T2;
+        gas[3 * nn + u] = T3;
+        gas[4 * nn + u] = nd_ave;
+        gas[5 * nn + u] = count != 0 ? sums[nn + u] / count : 0.0;
+        gas[6 * nn + u] = count != 0 ? sums[2 * nn + u] / count : 0.0;
+        gas[7 * nn + u] = count != 0 ? sums[3 * nn + u] / count : 0.0;
+        gas[8 * nn + u] = nd_ave * K_B * T;
+        const int i = (int)(u % m.ni), j = (int)(u / m.ni);
+        gas[9 * nn + u] = (i < m.nci && j < m.ncj) ? (double)mpc_sum[(size_t)j * m.nci + i] * inv_ns : 0.0;
+    }
+}
+
 // ----------------------------------------------------------------------------------------
 // particle addition (KineticMaterial.addParticle) and device preload
 // ----------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) add_particles_kernel(MeshP m, MeshD md, PState d, int64_t base, int64_t n,
-                                                            const double* __restrict__ in, double qm,
-                                                            double qm_half_dt, double half_dt,
-                                                            int rewind, const int32_t* __restrict__ ids,
-                                                            int id0, int with_ids) {
+// `in` holds the columns x, y, u, v, w, `ld` doubles apart (ld = n for a packed [5][n] block).
+__global__ void __launch_bounds__(256) add_particles_kernel_strided(MeshP m, MeshD md, PState d, int64_t base,
+                                                                    int64_t n, const double* __restrict__ in,
+                                                                    int64_t ld, double qm, double qm_half_dt,
+                                                                    double half_dt, int rewind,
+                                                                    const int32_t* __restrict__ ids, int id0,
+                                                                    int with_ids) {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        double x = in[k], y = in[n + k], u = in[2 * n + k], v = in[3 * n + k], w = in[4 * n + k];
+        double x = in[k], y = in[ld + k], u = in[2 * ld + k], v = in[3 * ld + k], w = in[4 * ld + k];
         double lc0 = (x - m.x0[0]) * m.inv_dh[0];
         double lc1 = (y - m.x0[1]) * m.inv_dh[1];
         if (rewind) {
@@ -696,7 +1093,7 @@ __global__ void __launch_bounds__(256) add_particles_kernel(MeshP m, MeshD md, P
 // attempts (a source whose drift points out of the domain), become holes (cell -1).
 __global__ void __launch_bounds__(256) inject_maxwellian_kernel(MeshP m, MeshD md, PState d, int64_t base, int n,
                                                                 const double* __restrict__ geom, int nseg, double area,
-                                                                double v_drift, double v_th, double qm,
+                                                                double v_drift, double v_th, double v_az, double qm,
                                                                 double qm_half_dt, double half_dt, double dt,
                                                                 uint32_t k0, uint32_t k1, uint32_t step, uint32_t tag,
                                                                 int with_ids, int id0,
@@ -740,7 +1137,7 @@ __global__ void __launch_bounds__(256) inject_maxwellian_kernel(MeshP m, MeshD m
             for (int c = 0; c < 3; c++) vm[c] = (v_th * 1.2) * (sh * 1.0 * ((rng.sum12(q + 12 * c) - 0.5 * 12) * 1.0));
             vel[0] = vm[0] + n0 * v_drift;
             vel[1] = vm[1] + n1 * v_drift;
-            vel[2] = vm[2];
+            vel[2] = vm[2] + v_az;
             inward = n0 * vel[0] + n1 * vel[1] > 0;
         }
         // Source.sampleKinetic nudge, then KineticMaterial.addParticle
@@ -794,7 +1191,7 @@ int inject_maxwellian_device(sf_ctx* ctx, HostSource& s, int num, uint64_t seed,
     SF_HIP(ctx, hipMemsetAsync(d_acc, 0, sizeof(unsigned long long), ctx->stream));
     hipLaunchKernelGGL(inject_maxwellian_kernel, dim3(std::min(4096, (num + 255) / 256)), dim3(256), 0, ctx->stream, ctx->mp,
                        ctx->meshd(), sp->buf[sp->cur], sp->np, num, (const double*)s.d_geom, nseg, s.area, s.v_drift,
-                       s.v_th, sp->qm, -0.5 * ctx->dt * sp->qm, -0.5 * ctx->dt, ctx->dt, (uint32_t)seed,
+                       s.v_th, s.v_az, sp->qm, -0.5 * ctx->dt * sp->qm, -0.5 * ctx->dt, ctx->dt, (uint32_t)seed,
                        (uint32_t)(seed >> 32), step, tag, sp->desc.track_ids, sp->id_counter, d_acc);
     SF_HIP(ctx, hipGetLastError());
     unsigned long long acc = 0;
@@ -884,6 +1281,11 @@ static PushC make_pushc(sf_ctx* ctx, Species* sp) {
     pc.kv = sp->kscale[1];    // = kscale[2] = kscale[3]
     pc.kvv = sp->kscale[4];   // = kscale[5] = kscale[6]
     pc.spwt = sp->desc.spwt;
+    // A corner weight is at most 1, so a contribution is at most |v| kv or v^2 kvv: the one-FMA
+    // rounding holds below 2^51 (margin 2 kept), an int64 below 2^63 (margin 2); the mover sums
+    // round |v| kv <= sqrt(3) max|v_c| kv with the same FMA.
+    pc.vfast = std::min(std::ldexp(1.0, 50) / pc.kv, std::sqrt(std::ldexp(1.0, 50) / pc.kvv));
+    pc.vover = std::min(std::sqrt(std::ldexp(1.0, 62) / pc.kvv), std::ldexp(1.0, 51) / (std::sqrt(3.0) * pc.kv));
     return pc;
 }
 
@@ -950,28 +1352,59 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
                            (sp->np - sp->nsorted) > sp->np / 8;
     const int64_t np_before = sp->np, nsorted_before = sp->nsorted;
     SF_HIP(ctx, hipMemsetAsync(sp->stats, 0, (size_t)STAT_SLOTS * STAT_STRIDE * sizeof(long long), ctx->stream));
+    SF_HIP(ctx, hipMemsetAsync(sp->flags, 0, sizeof(int), ctx->stream));
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     const bool hb = md.bf != nullptr;
+    // XY without a magnetic field: the fast tile kernel, with the particles it leaves queued for
+    // the full reference loop; RZ / ZR / with B: the tile kernel that runs the loop itself
+    const bool fastk = m.domain == SF_DOMAIN_XY && !hb && !ctx->no_fast_push;
     const dim3 tg(tile_grid(ctx)), tb(PUSH_THREADS);
     const dim3 g_tail(grid_for(sp->np - sp->nsorted, 256, 4096)), b256(256);
     // Every step pushes in place (a dead particle becomes a hole, cell = -1).  A sort step then
     // moves the pushed particles into cell order by their new cell, compacting the holes.
+    SF_HIP(ctx, hipMemsetAsync(sp->qcount, 0, sizeof(int), ctx->stream));
     if (sp->nsorted > 0) {
-        if (hb)
+        if (fastk) {
+            FastC fc;
+            fc.qmdt = (float)pc.qm_dt;
+            fc.cdx = (float)(pc.dt * m.inv_dh[0]);
+            fc.cdy = (float)(pc.dt * m.inv_dh[1]);
+            fc.vfast = (float)pc.vfast;
+            fc.kn = pc.kn;
+            fc.kv = pc.kv;
+            fc.kvv = pc.kvv;
+            fc.ni = m.ni;
+            fc.nj = m.nj;
+            fc.nci = m.nci;
+            fc.ncj = m.ncj;
+            fc.ntx = m.ntx;
+            hipLaunchKernelGGL(push_tile_fast_xy, tg, tb, 0, ctx->stream, fc, (const float2*)ctx->d_ef32,
+                               (const uint8_t*)ctx->d_cellslow, s, (const int32_t*)sp->offsets, sp->acc, sp->counts,
+                               sp->stats, sp->key, sp->qcount);
+        } else if (hb) {
             hipLaunchKernelGGL(push_deposit_tile_inplace_b, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets,
-                               sp->acc, sp->counts, sp->stats);
-        else
+                               sp->acc, sp->counts, sp->stats, sp->flags);
+        } else {
             hipLaunchKernelGGL(push_deposit_tile_inplace, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets,
-                               sp->acc, sp->counts, sp->stats);
+                               sp->acc, sp->counts, sp->stats, sp->flags);
+        }
     }
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    if (sp->nsorted > 0 && fastk) {
+        // the queue is at most the sorted particles; the grid strides over the count on the device
+        const int gq = (int)std::min<int64_t>(2048, std::max<int64_t>(1, sp->nsorted / 4096));
+        hipLaunchKernelGGL(push_deposit_queue, dim3(gq), b256, 0, ctx->stream, m, md, pc, s,
+                           (const int32_t*)sp->key, (const int*)sp->qcount, sp->acc, sp->counts, sp->stats,
+                           sp->flags);
+    }
+    if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
     if (sp->np > sp->nsorted) {
         if (hb)
             hipLaunchKernelGGL(push_deposit_tail<true>, g_tail, b256, 0, ctx->stream, m, md, pc, s, sp->nsorted,
-                               sp->np, sp->acc, sp->counts, sp->stats);
+                               sp->np, sp->acc, sp->counts, sp->stats, sp->flags);
         else
             hipLaunchKernelGGL(push_deposit_tail<false>, g_tail, b256, 0, ctx->stream, m, md, pc, s, sp->nsorted,
-                               sp->np, sp->acc, sp->counts, sp->stats);
+                               sp->np, sp->acc, sp->counts, sp->stats, sp->flags);
     }
     SF_HIP(ctx, hipGetLastError());
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
@@ -994,18 +1427,33 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
         SF_HIP(ctx, hipGetLastError());
     }
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    const double spwt = sp->desc.spwt;
     hipLaunchKernelGGL(normalize_nodes, dim3(grid_for(nn, 256, 8192)), dim3(256), 0, ctx->stream, nn, sp->acc,
                        sp->sums, (const double*)ctx->d_node_vol, sp->den, 1.0 / sp->kscale[0], 1.0 / sp->kscale[1],
-                       sp->desc.spwt, sp->desc.nd_back, 1);
+                       spwt, sp->desc.nd_back, 1, spwt / sp->kscale[0], spwt / sp->kscale[1], spwt / sp->kscale[4]);
     SF_HIP(ctx, hipGetLastError());
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    // KineticMaterial.updateGasProperties (KineticMaterial.java:1553-1563): one more sample, and
+    // computeFields on the first update and on steps that are multiples of 10
     sp->num_samples++;
+    sp->pushes++;
+    const int64_t step = ctx->it > 0 ? (int64_t)ctx->it : sp->pushes;
+    if (sp->first_time || step % 10 == 0) {
+        hipLaunchKernelGGL(compute_fields_kernel, dim3(grid_for(nn, 256, 8192)), dim3(256), 0, ctx->stream, m,
+                           (const double*)sp->sums, (const long long*)sp->mpc_sum, (const double*)ctx->d_node_vol,
+                           sp->gas, sp->mass / K_B, sp->desc.init_T, 1.0 / sp->num_samples);
+        SF_HIP(ctx, hipGetLastError());
+    }
+    sp->first_time = false;
     // mover sums (KineticMaterial.java:247-259); the count is the number of survivors
     long long slots[STAT_SLOTS * STAT_STRIDE], st[5] = {0, 0, 0, 0, 0};
     SF_HIP(ctx, hipMemcpyAsync(slots, sp->stats, sizeof(slots), hipMemcpyDeviceToHost, ctx->stream));
     int64_t total = 0;
     if (sort_step) SF_HIP(ctx, hipMemcpyAsync(&total, sp->offsets + ncells, sizeof(int32_t), hipMemcpyDeviceToHost,
                                               ctx->stream));
+    int flags = 0, queued = 0;
+    SF_HIP(ctx, hipMemcpyAsync(&flags, sp->flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    SF_HIP(ctx, hipMemcpyAsync(&queued, sp->qcount, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int k = 0; k < STAT_SLOTS; k++)
         for (int q = 0; q < 5; q++) st[q] += slots[k * STAT_STRIDE + q];
@@ -1015,13 +1463,16 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     sp->stats_host[2] = (double)st[2] * inv_kv * sp->desc.spwt * sp->mass;
     sp->stats_host[3] = (double)st[3] * inv_kv * sp->desc.spwt * sp->mass;
     sp->stats_host[4] = (double)st[4] * inv_kv * sp->desc.spwt * sp->mass;
+    ctx->last_queue[1] = queued;
     if (ctx->timing) {
-        float a, b, c, e, t0 = 0.f;
+        float a, b, c, e, t0 = 0.f, t1 = 0.f;
         hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
         hipEventElapsedTime(&t0, ctx->ev[0], ctx->ev[5]);
+        hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[6]);
         ctx->last_push_ms[0] = t0;
-        ctx->last_push_ms[1] = a - t0;
+        ctx->last_push_ms[1] = a - t1;
         ctx->last_push_ms[2] = (double)(np_before - nsorted_before);
+        ctx->last_queue[0] = t1 - t0;
         hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
         hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
         hipEventElapsedTime(&e, ctx->ev[3], ctx->ev[4]);
@@ -1042,6 +1493,10 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     } else {
         sp->since_sort++;
     }
+    if (flags & PUSH_FLAG_OVERFLOW)
+        return sf_fail(ctx, SF_WARN_MOMENT_RANGE,
+                       "a particle speed beyond %.3g m/s overflowed the 64-bit moment channels of this step "
+                       "(raise the species' vmax)", pc.vover);
     return SF_OK;
 }
 
@@ -1086,8 +1541,8 @@ int add_particles_device(sf_ctx* ctx, Species* sp, int64_t n, const double* x, c
         SF_HIP(ctx, hipMallocAsync((void**)&d_ids, kept * sizeof(int32_t), ctx->stream));
         SF_HIP(ctx, hipMemcpyAsync(d_ids, hid.data(), kept * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     }
-    hipLaunchKernelGGL(add_particles_kernel, dim3(grid_for(kept, 256, 4096)), dim3(256), 0, ctx->stream, ctx->mp,
-                       ctx->meshd(), sp->buf[sp->cur], sp->np, kept, (const double*)d_in, sp->qm,
+    hipLaunchKernelGGL(add_particles_kernel_strided, dim3(grid_for(kept, 256, 4096)), dim3(256), 0, ctx->stream,
+                       ctx->mp, ctx->meshd(), sp->buf[sp->cur], sp->np, kept, (const double*)d_in, kept, sp->qm,
                        -0.5 * ctx->dt * sp->qm, -0.5 * ctx->dt, mode, (const int32_t*)d_ids, sp->id_counter, sp->desc.track_ids);
     SF_HIP(ctx, hipGetLastError());
     SF_HIP(ctx, hipFreeAsync(d_in, ctx->stream));

@@ -63,6 +63,7 @@ class MaterialSpec:
     charge: float = 0.0
     spwt: float = 1.0
     nd_back: float = 0.0
+    init_T: float = 0.0              # init T=... (Material init_vals.T): computeFields' empty-node value
 
 
 @dataclass
@@ -74,6 +75,9 @@ class SourceSpec:
     mdot: float = 0.0
     v_drift: float = 0.0
     temperature: float = 0.0
+    v_az: float = 0.0                # MaxwellianSource v_az (MaxwellianSource.java:62, 117)
+    start_it: int = 0                # Source start_it / stop_it (end_it) (Source.java:68-71)
+    stop_it: int = -1
 
 
 @dataclass
@@ -91,6 +95,9 @@ class SolverSpec:
     nl_tol: float = 1e-4
     rel_tol: float = 1e-8
     learned_guess: bool = False
+    nl_tol_set: bool = False         # nl_tol given in the deck (the GPU solve then also stops on it)
+    skip: int = 1                    # PoissonSolver skip: solve when (it - 1) % skip == 0
+    initial_only: bool = False       # SolverModule initial_only: solve only on the first call
 
 
 @dataclass
@@ -187,6 +194,8 @@ def parse_deck(path):
         tag = el.tag.lower()
         if tag == "material" and parent.get(el) is not None and parent[el].tag.lower() != "materials":
             continue                 # <material> inside <boundary> names the boundary's material
+        if tag == "boundary" and parent.get(el) is not None and parent[el].tag.lower() != "boundaries":
+            continue                 # <boundary> inside a source names the source's boundary
         if tag == "domain":
             dom = el.attrib.get("type", "xy").upper()
             for m in el:
@@ -211,19 +220,30 @@ def parse_deck(path):
                                      parse_path(_get(el, "path")), rev, mat))
         elif tag == "material":
             kind = _get(el, "type").lower()
-            nd_back = 0.0
+            nd_back, init_T = 0.0, 0.0
             init = _get(el, "init", "")
-            for item in init.split(","):
+            for item in init.split(","):              # Material.setInitValues (Material.java:580-608)
                 if "=" in item:
                     k, v = item.split("=")
                     if k.strip().lower() == "nd_back":
                         nd_back = float(v)
+                    elif k.strip().lower() == "t":
+                        init_T = float(v)
             mats.append(MaterialSpec(_get(el, "name"), kind, float(_get(el, "molwt", "1")),
-                                     float(_get(el, "charge", "0")), float(_get(el, "spwt", "1")), nd_back))
+                                     float(_get(el, "charge", "0")), float(_get(el, "spwt", "1")), nd_back,
+                                     init_T))
         elif tag == "boundary_source":
+            if _get(el, "circuit_model", "false").lower() == "true":
+                raise ValueError("source %s: <circuit_model> is not supported by this path" % _get(el, "name"))
+            start_it = int(float(_get(el, "start_it", "0")))
+            stop_it = int(float(_get(el, "stop_it", "-1")))
+            stop_it = int(float(_get(el, "end_it", str(stop_it))))
             srcs.append(SourceSpec(_get(el, "name"), _get(el, "type").lower(), _get(el, "material"),
                                    _get(el, "boundary"), float(_get(el, "mdot", "0")),
-                                   float(_get(el, "v_drift", "0")), float(_get(el, "temperature", "0"))))
+                                   float(_get(el, "v_drift", "0")), float(_get(el, "temperature", "0")),
+                                   float(_get(el, "v_az", "0")), start_it, stop_it))
+            srcs[-1].current = float(_get(el, "current", "0"))
+            srcs[-1].den = float(_get(el, "den", "0"))
         elif tag == "solver":
             s = SolverSpec(type=_get(el, "type").lower())
             s.method = _get(el, "method", "GS").upper()
@@ -235,9 +255,21 @@ def parse_deck(path):
             s.max_it = int(float(_get(el, "max_it", "5000")))
             s.tol = float(_get(el, "tol", "1e-6"))
             s.nl_max_it = int(float(_get(el, "nl_max_it", "50")))
+            s.nl_tol_set = _get(el, "nl_tol", "") != ""
             s.nl_tol = float(_get(el, "nl_tol", "1e-4"))
             s.rel_tol = float(_get(el, "rel_tol", "1e-8"))
             s.learned_guess = _get(el, "learned_guess", "false").lower() == "true"
+            s.skip = int(float(_get(el, "skip", "1")))                  # PoissonSolver.java:101
+            s.initial_only = _get(el, "initial_only", "false").lower() == "true"   # SolverModule.java:72
+            if s.skip < 1:
+                raise ValueError("<skip> must be at least 1")
+            # keys whose reference behaviour this path does not provide are refused, not dropped
+            if _get(el, "qn_switch", "false").lower() == "true":
+                raise ValueError("<qn_switch> (PoissonSolver.java:162-193) is not supported by this path")
+            if _get(el, "n0_pos", "") != "":
+                raise ValueError("<n0_pos> (sampled Boltzmann reference density) is not supported by this path")
+            if int(float(_get(el, "rho_moat", "0"))) != 0:
+                raise ValueError("<rho_moat> is not supported by this path")
             solver = s
         elif tag == "time":
             num_it = int(float(_get(el, "num_it", "0")))
@@ -247,7 +279,43 @@ def parse_deck(path):
                 seed = 0
     if mesh is None:
         raise ValueError("deck has no <domain>/<mesh>")
-    return Deck(mesh, bnds, mats, srcs, solver, num_it, dt, seed)
+    deck = Deck(mesh, bnds, mats, srcs, solver, num_it, dt, seed)
+    for src in srcs:
+        resolve_source_rate(deck, src, getattr(src, "current", 0.0), getattr(src, "den", 0.0))
+    return deck
+
+
+def boundary_area(deck, b):
+    """Spline area of a linear boundary (Spline.computeGeometry): its length in XY (depth 1); the
+    surface of revolution (conical frustra about the axis) in RZ and ZR."""
+    area = 0.0
+    for (x1, y1), (x2, y2) in zip(b.points[:-1], b.points[1:]):
+        length = math.hypot(x2 - x1, y2 - y1)
+        dom = deck.mesh.domain.upper()
+        if dom == "XY":
+            area += length
+        else:
+            r1, r2 = (x1, x2) if dom == "RZ" else (y1, y2)
+            area += math.pi * (r1 + r2) * length
+    return area
+
+
+def resolve_source_rate(deck, src, current=0.0, den=0.0):
+    """MaxwellianSource's rate keys (MaxwellianSource.java:46-75): exactly one of <mdot>, <current>
+    and <den>; current gives mdot = |current / charge * mass|, den gives
+    mdot = den A (v_drift + v_mean/4) mass with v_mean = sqrt(8 K T / (pi mass))."""
+    n = sum(1 for v in (src.mdot, current, den) if v > 0)
+    if n != 1:
+        raise ValueError("source %s: only one of <mdot>, <current>, and <den> can be specified" % src.name)
+    mat = deck.material(src.material)
+    mass = mat.molwt * AMU
+    if current != 0.0:
+        if mat.charge == 0.0:
+            raise ValueError("source %s: cannot use <current> with neutral materials" % src.name)
+        src.mdot = abs(current / (mat.charge * QE) * mass)
+    if den > 0:
+        v_mean = math.sqrt(8 * K_B * src.temperature / (math.pi * mass))
+        src.mdot = den * boundary_area(deck, deck.boundary(src.boundary)) * (src.v_drift + v_mean / 4) * mass
 
 
 # ------------------------------------------------------------------------------------------

@@ -8,6 +8,8 @@ import numpy as np
 
 from . import _lib
 
+TAG_AUTO = 0xFFFFFFFF   # SF_TAG_AUTO
+
 MODELS = {"CEX": 0, "MEX": 1, "ELASTIC": 1}
 
 
@@ -21,14 +23,15 @@ class MccHip:
     sim: plugin.Simulation; source: the KineticHipMaterial (created with track_ids=True);
     model: "CEX" or "MEX"/"ELASTIC"; sigma: constant cross-section (m^2); target_molwt: amu;
     frequency, stop_it, max_target_temp: the reference's keys; seed and tag key the per-particle
-    Philox stream."""
+    Philox stream (tag None: the library picks a tag no other live MCC of this seed uses)."""
 
     FIELDS = {"col-count-sum": 0, "col-real-sum": 1, "col-count": 2, "nu": 3, "S": 4}
 
     def __init__(self, sim, source, model, sigma, target_molwt, frequency=1, stop_it=-1, max_target_temp=-1.0,
-                 seed=0, tag=0):
+                 seed=0, tag=None):
         self.sim = sim
         self.lib = _lib.load()
+        tag = TAG_AUTO if tag is None else tag
         kind = model.upper()
         if kind not in MODELS:
             raise ValueError("unsupported MCC model %s (CEX and MEX/ELASTIC are provided)" % model)
@@ -74,15 +77,16 @@ class DsmcHip:
     sigma: "const" (sigma_c0 in m^2) or "bird463" (VHS from each material's diam, ref_temp and
     visc_temp_index, given as pairs); vss_alpha: pair of the materials' <vss_alpha>;
     sig_cr_max: initial value; frequency: the reference's key; seed and tag key the per-cell
-    Philox streams."""
+    Philox streams (tag None: the library picks a free one)."""
 
     FIELDS = {"col-count-sum": 0, "col-real-sum": 1, "col-count": 2, "nu": 3}
     SIGMA = {"CONST": 0, "BIRD463": 1}
 
     def __init__(self, sim, mat_a, mat_b, sigma="bird463", sigma_c0=0.0, diam=(0.0, 0.0), ref_temp=(275.0, 275.0),
-                 visc_temp_index=(0.85, 0.85), vss_alpha=(1.0, 1.0), sig_cr_max=1e-16, frequency=1, seed=0, tag=0):
+                 visc_temp_index=(0.85, 0.85), vss_alpha=(1.0, 1.0), sig_cr_max=1e-16, frequency=1, seed=0, tag=None):
         self.sim = sim
         self.lib = _lib.load()
+        tag = TAG_AUTO if tag is None else tag
         kind = sigma.upper()
         if kind not in self.SIGMA:
             raise ValueError("unsupported DSMC cross-section %s (const and bird463 are provided)" % sigma)

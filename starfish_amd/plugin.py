@@ -104,17 +104,20 @@ class KineticHipMaterial:
         self.sim = sim
         self.name = spec.name
         self.spec = spec
+        if not vmax:
+            vmax = speed_hint(sim.deck, spec)
         d = _lib.SpeciesDesc(spec.charge, spec.molwt, spec.spwt, spec.nd_back, vmax, int(capacity),
-                             1 if track_ids else 0)
+                             1 if track_ids else 0, getattr(spec, "init_T", 0.0))
         idx = C.c_int()
         sim.ctx.call("sf_species_create", C.byref(d), C.byref(idx))
         self.index = idx.value
         self.mass = spec.molwt * _lib_consts()["AMU"]
         self.charge = spec.charge * _lib_consts()["QE"]
 
-    # KineticMaterial.updateFields() (KineticMaterial.java:117-163) + Material.update nd_back
+    # KineticMaterial.updateFields() (KineticMaterial.java:117-163) + Material.update nd_back;
+    # returns SF_WARN_MOMENT_RANGE (2) when a particle overflowed the fixed-point moments
     def update(self):
-        self.sim.ctx.call("sf_push_deposit", self.index)
+        return self.sim.ctx.call("sf_push_deposit", self.index)
 
     def getNp(self):
         n = C.c_int64()
@@ -177,22 +180,52 @@ def _lib_consts():
     return {"AMU": deck.AMU, "QE": deck.QE}
 
 
+def speed_hint(deck, spec):
+    """Fixed-point velocity scale of a species (sf_species_desc.vmax) from the deck: twice the
+    drift plus three widened thermal speeds of its sources and preload (Utils.SampleMaxw3D widens
+    by 1.2).  Faster particles stay exact up to about 45 times this speed; 0 keeps the library's
+    1e5 m/s when the deck names no population."""
+    import math
+    from .deck import AMU, K_B
+    mass = spec.molwt * AMU
+    v = 0.0
+    for s in deck.sources:
+        if s.material.lower() == spec.name.lower():
+            v = max(v, s.v_drift + 3 * 1.2 * math.sqrt(2 * K_B * s.temperature / mass))
+    pre = getattr(deck, "preload", None)
+    if pre is not None and deck.materials and deck.materials[0] is spec:
+        _, drift, temp = pre
+        v = max(v, math.sqrt(sum(d * d for d in drift)) + 3 * 1.2 * math.sqrt(2 * K_B * temp / mass))
+    return 2 * v
+
+
 class PoissonHipSolver:
     """`<solver type="poisson_hip">`: PoissonSolver keys (n0, Te0, phi0, eps_r, linear, max_it,
-    nl_max_it, nl_tol; PoissonSolver.java:49-101, SolverModule.java:71-86) plus rel_tol."""
+    nl_max_it, nl_tol, skip; PoissonSolver.java:49-113, SolverModule.java:71-86) plus rel_tol.
+    The solve always runs to rel_tol; nl_tol adds the reference's Newton stop only when the deck
+    sets it.  SolverModule.updateFields applies initial_only (Simulation.update_fields)."""
 
     def __init__(self, sim, spec):
         self.sim = sim
         self.spec = spec
+        nl_tol = spec.nl_tol if getattr(spec, "nl_tol_set", False) else 0.0
         d = _lib.PoissonDesc(1 if spec.linear else 0, spec.n0, spec.Te0, spec.phi0, spec.eps_r, spec.max_it,
-                             spec.rel_tol, spec.nl_max_it, spec.nl_tol)
+                             spec.rel_tol, spec.nl_max_it, nl_tol)
         sim.ctx.call("sf_poisson_setup", C.byref(d))
+        self.skip = max(1, int(getattr(spec, "skip", 1)))
+        self.first = True
+        self.initial_only = bool(getattr(spec, "initial_only", False))
         self.last = _lib.SolveInfo()
+        self.solved = False                         # whether the last update() ran a solve
 
     def update(self):                               # PoissonSolver.update
         self.sim.ctx.call("sf_rho_update")
+        self.solved = False
+        if (self.sim.it - 1) % self.skip != 0:     # (Starfish.getIt()-1) % skip (PoissonSolver.java:113)
+            return 0
         self.last = _lib.SolveInfo()
         rc = self.sim.ctx.call("sf_poisson_solve", None, C.byref(self.last))
+        self.solved = True
         return rc
 
     def updateGradientField(self):                  # PotentialSolver.updateGradientField
@@ -292,6 +325,9 @@ class Simulation:
             src = C.c_int()
             self.ctx.call("sf_source_maxwellian", mat.index, len(b.points), _dp(pts), 1 if b.reverse else 0,
                           s.mdot, s.v_drift, s.temperature, C.byref(src))
+            v_az, start_it, stop_it = getattr(s, "v_az", 0.0), getattr(s, "start_it", 0), getattr(s, "stop_it", -1)
+            if v_az != 0.0 or start_it != 0 or stop_it >= 0:
+                self.ctx.call("sf_source_options", src.value, v_az, start_it, stop_it)
         self.it = 1
         self.solve_infos = []
 
@@ -306,18 +342,24 @@ class Simulation:
         self.ctx.call("sf_ctx_get_rng_state", C.byref(s))
         return s.value
 
-    def update_fields(self):                        # SolverModule.updateFields
+    def update_fields(self):                        # SolverModule.updateFields (SolverModule.java:123-139)
         if self.solver is None:
             return
+        if getattr(self.solver, "initial_only", False) and not self.solver.first:
+            self.ctx.call("sf_rho_update")
+            return 0
         rc = self.solver.update()
-        self.solve_infos.append(self.solver.last)
+        if getattr(self.solver, "solved", True):
+            self.solve_infos.append(self.solver.last)
         self.solver.updateGradientField()
+        self.solver.first = False
         return rc
 
     def initial_solve(self):
         return self.update_fields()
 
     def sample_sources(self):                       # SourceModule.sampleSources
+        self.ctx.call("sf_ctx_set_it", self.it)
         injected = C.c_int64()
         if self.device_sources:
             self.ctx.call("sf_sources_sample_device", C.c_uint64(self.deck.seed), C.byref(injected))
@@ -326,7 +368,7 @@ class Simulation:
         return injected.value
 
     def step(self):                                 # Starfish.MainLoop body
-        self.sample_sources()
+        self.sample_sources()                       # also sets the step counter of the library
         for m in self.materials:
             m.update()
         self.update_fields()

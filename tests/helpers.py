@@ -29,7 +29,9 @@ def oracle_for(deck, geo, rng_state=None, nthreads=1):
         O.set_rng_state(o, rng_state)
     for m in deck.materials:
         if m.kind.lower() in ("kinetic", "kinetic_hip"):
-            o.add_species(m.charge, m.molwt, m.spwt, m.nd_back)
+            k = o.add_species(m.charge, m.molwt, m.spwt, m.nd_back)
+            if getattr(m, "init_T", 0.0):
+                o.set_init_T(k, m.init_T)
     if deck.solver is not None:
         o.set_solver(deck.solver)
     return o

@@ -1,0 +1,109 @@
+"""Parity at the benchmark's full mesh (1024x512 XY nodes, dx = 1e-3 m, -100 V cylinder):
+
+* push + deposit: a 2e5-particle sample of the drifting Maxwellian in the device's own initial E
+  field, pushed by the device and by the FP64 oracle.  The first step runs the kernel for appended
+  particles and sorts them; the oracle's state is rounded to FP32 as the device stores it; the
+  second step runs the tiled kernels (the FP32 fast path and the queue of particles near the
+  cylinder and the faces).  Positions and velocities agree to 1e-5 relative, nd, u, v, w to 1e-5,
+  T (a difference of second moments) to 2e-4.
+* potential: the device's converged phi for the preloaded plasma, against the solution of the
+  reference's assembled FP64 system (Solver.solveNonLinearNR's F and J): one Newton step from
+  the device's phi with a direct sparse solve gives its error, since Newton converges
+  quadratically from there.  Bound: 1e-6 of max |phi|."""
+import numpy as np
+import pytest
+import scipy.sparse as sps
+import scipy.sparse.linalg as spla
+
+from helpers import geometry, open_cell_particles, oracle_for, rel_err
+from starfish_amd import deck as D
+
+EPS0 = 8.85418782e-12
+QE = 1.60217646e-19
+
+
+def _face_mask(geo, ni, nj):
+    nn = ni * nj
+    I, J = np.arange(nn) % ni, np.arange(nn) // ni
+    ft = geo.face_type
+    right, top, left, bottom = ft[:nj], ft[nj:nj + ni], ft[nj + ni:2 * nj + ni], ft[2 * nj + ni:]
+    return ((I == 0) & (left[J] != 4) & (left[J] != 0)) | ((I == ni - 1) & (right[J] != 4) & (right[J] != 0)) | \
+           ((J == 0) & (bottom[I] != 4) & (bottom[I] != 0)) | ((J == nj - 1) & (top[I] != 4) & (top[I] != 0))
+
+
+@pytest.mark.gpu
+def test_c2_mesh_push_deposit_and_potential_match_oracle():
+    from starfish_amd.plugin import Simulation
+    deck = D.config_c2(particles=2_000_000)
+    geo, st = geometry(deck)
+    m = deck.mesh
+    # the device's initial solve for a preloaded plasma gives E (and the potential checked below)
+    big = Simulation(deck, capacity=3_000_000)
+    try:
+        n, drift, temp = deck.preload
+        big.materials[0].preload(n, drift, temp, seed=3)
+        assert big.initial_solve() == 0
+        phi, rho = big.phi(), big.rho()
+        efi, efj = big.efield()
+    finally:
+        big.close()
+    o = oracle_for(deck, geo, st, nthreads=8)
+    # potential error: one FP64 Newton step on the reference's assembled system
+    A = o.operator(0).tocsr()
+    fixed = o.fixed_nodes().astype(bool)
+    face = _face_mask(geo, m.ni, m.nj)
+    s = deck.solver
+    eps = EPS0 * s.eps_r
+    b = -rho / eps
+    b[fixed] = geo.node_bc[fixed]
+    bx = np.where(fixed, 0, QE / eps * s.n0 * np.exp((phi - s.phi0) / s.Te0))
+    P = bx / s.Te0
+    b = b + bx
+    b[face & ~fixed] = 0
+    P[face & ~fixed] = 0
+    y = spla.spsolve((A - sps.diags(P)).tocsc(), A @ phi - b)
+    print("phi error (one Newton step): max %.2e V, rel %.2e" % (np.abs(y).max(), np.abs(y).max() / np.abs(phi).max()),
+          flush=True)
+    assert np.abs(y).max() <= 1e-6 * np.abs(phi).max()
+    # push + deposit from identical sets
+    ns = 200_000
+    x, yy, v, _, _ = open_cell_particles(deck, geo, ns, seed=17)
+    ids = np.arange(ns, dtype=np.int32)
+    o.set_field("efi", efi)
+    o.set_field("efj", efj)
+    o.load_particles(0, np.stack([x, yy, np.zeros(ns)], 1), v, ids=ids, rewind=False)
+    sim = Simulation(deck, capacity=2 * ns, track_ids=True)
+    try:
+        sim.set_efield(efi, efj)
+        mat = sim.materials[0]
+        mat.add_particles(x, yy, v[:, 0], v[:, 1], v[:, 2], ids=ids, rewind=False)
+        o.move()
+        o.update_species_fields()
+        o.round_state_fp32(0)
+        assert mat.update() == 0
+        o.move()
+        o.update_species_fields()
+        assert mat.update() == 0
+        q = np.zeros(2)
+        import ctypes as C
+        sim.ctx.call("sf_last_push_queue", q.ctypes.data_as(C.POINTER(C.c_double)))
+        print("queued to the reference loop on the tiled step: %d of %d" % (q[1], mat.getNp()), flush=True)
+        gp = mat.particles(ids=True)
+        op = o.particles(0)
+        gi, oi = np.argsort(gp["id"]), np.argsort(op["id"])
+        common = np.intersect1d(gp["id"], op["id"])
+        assert len(gp["id"]) + len(op["id"]) - 2 * len(common) <= 2
+        gs, os_ = np.isin(gp["id"][gi], common), np.isin(op["id"][oi], common)
+        ext = (m.ni - 1) * m.dh[0]
+        assert rel_err(gp["x"][gi][gs], op["pos"][oi][os_][:, 0], ext) < 1e-5
+        assert rel_err(gp["y"][gi][gs], op["pos"][oi][os_][:, 1], ext) < 1e-5
+        vmax = np.abs(op["vel"]).max()
+        for k, name in enumerate(("u", "v", "w")):
+            assert rel_err(gp[name][gi][gs], op["vel"][oi][os_][:, k], vmax) < 1e-5, name
+        for name in ("nd", "u", "v", "w", "t"):
+            err = rel_err(mat.field(name), o.field(name, 0))
+            print("%-3s rel err %.2e" % (name, err), flush=True)
+            assert err < (2e-4 if name == "t" else 1e-5), name
+    finally:
+        sim.close()
+        o.close()

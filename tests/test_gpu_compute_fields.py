@@ -1,0 +1,131 @@
+"""computeFields parity on the GPU (KineticMaterial.updateGasProperties / computeFields,
+core/materials/KineticMaterial.java:1553-1671): the same particle set is pushed by the device and
+by the FP64 oracle in the same fixed E field (the oracle rounds its state to FP32 after each step,
+as the device stores it), and t, t1-t3, nd-ave, u/v/w-ave, p and mpc are compared after steps 1
+(first update: computed), 9 (not refreshed since step 1), 10 (refreshed) and 12 (not refreshed).
+
+Tolerances: the sums agree to ~1e-7 relative, so nd-ave, the averages, p and mpc are held to
+1e-5 of their maximum.  A temperature is a difference of second moments, <v^2> - <v>^2, with a
+7 km/s drift over a ~1 km/s spread, so its relative error is about 50 times larger: 2e-4 of the
+largest temperature.  init_vals.T (here 77 K) must appear on exactly the nodes the oracle leaves
+empty."""
+import numpy as np
+import pytest
+
+from helpers import geometry, open_cell_particles, oracle_for, rel_err
+from starfish_amd import deck as D
+
+GAS = ("t", "t1", "t2", "t3", "nd-ave", "u-ave", "v-ave", "w-ave", "p", "mpc")
+
+
+@pytest.mark.gpu
+def test_compute_fields_match_oracle_across_the_refresh_steps():
+    from starfish_amd.plugin import Simulation
+    n = 30000
+    deck = D.config_c1()
+    deck.materials[0].init_T = 77.0
+    geo, st = geometry(deck)
+    o = oracle_for(deck, geo, st)
+    o.initial_solve()
+    efi, efj = o.field("efi"), o.field("efj")
+    x, y, v, _, _ = open_cell_particles(deck, geo, n, seed=11)
+    ids = np.arange(n, dtype=np.int32)
+    o.load_particles(0, np.stack([x, y, np.zeros(n)], 1), v, ids=ids, rewind=False)
+    sim = Simulation(deck, capacity=2 * n, track_ids=True)
+    try:
+        sim.set_efield(efi, efj)
+        mat = sim.materials[0]
+        mat.add_particles(x, y, v[:, 0], v[:, 1], v[:, 2], ids=ids, rewind=False)
+        # before any update the device holds Material.init's fields: t = init_vals.T, the rest 0
+        assert np.all(mat.field("t") == 77.0) and np.all(mat.field("nd-ave") == 0.0)
+        snap = {}
+        for it in range(1, 13):
+            o.it = it
+            sim.ctx.call("sf_ctx_set_it", it)
+            o.move()
+            o.update_species_fields()
+            o.round_state_fp32(0)
+            rc = mat.update()
+            assert rc == 0
+            if it in (1, 9, 10, 12):
+                assert abs(mat.getNp() - o.num_particles(0)) <= 2
+                for name in GAS:
+                    g, r = mat.field(name), o.field(name, 0)
+                    tol = 2e-4 if name.startswith("t") else 1e-5
+                    err = rel_err(g, r)
+                    print("it %2d %-6s rel err %.2e" % (it, name, err), flush=True)
+                    assert err < tol, (it, name, err)
+                empty = o.field("count-sum", 0) == 0
+                assert np.array_equal(mat.field("count-sum") == 0, empty)
+                assert np.all(mat.field("t")[empty] == 77.0)
+                snap[it] = {name: mat.field(name) for name in GAS}
+        # refresh rule: step 9 still shows step 1's fields, step 12 step 10's
+        for name in GAS:
+            assert np.array_equal(snap[9][name], snap[1][name]), name
+            assert np.array_equal(snap[12][name], snap[10][name]), name
+        assert not np.array_equal(snap[10]["nd-ave"], snap[1]["nd-ave"])
+    finally:
+        sim.close()
+
+
+@pytest.mark.gpu
+def test_running_sums_are_fp64_and_do_not_wrap():
+    """A node sampled far beyond 2^23 unit-weight visits: the device's FP64 running sums keep
+    growing linearly (the former int64 sums at 2^40 per unit weight wrapped there)."""
+    from starfish_amd.plugin import Simulation
+    deck = D.config_c1()
+    geo, _ = geometry(deck)
+    m = deck.mesh
+    sim = Simulation(deck, capacity=1 << 20, track_ids=False)
+    try:
+        mat = sim.materials[0]
+        sim.set_efield(np.zeros(sim.nn), np.zeros(sim.nn))
+        # 2^19 particles at rest on one node, 40 samples: 2^24.3 visits of that node
+        i, j = 60, 30
+        nprt = 1 << 19
+        x = np.full(nprt, m.x0[0] + (i + 0.5) * m.dh[0])
+        y = np.full(nprt, m.x0[1] + (j + 0.5) * m.dh[1])
+        z = np.zeros(nprt)
+        mat.add_particles(x, y, z, z + 10.0, z, rewind=False)
+        for _ in range(40):
+            assert mat.update() == 0
+        u = i + j * m.ni
+        cs = mat.field("count-sum")[u]
+        expect = 40 * nprt * 0.25 * deck.materials[0].spwt
+        assert abs(cs / expect - 1) < 1e-12, cs / expect
+        vs = mat.field("v-sum")[u]
+        assert abs(vs / (expect * 10.0) - 1) < 1e-9
+    finally:
+        sim.close()
+
+
+@pytest.mark.gpu
+def test_too_fast_particle_is_reported_not_wrapped():
+    """Speeds past the one-FMA quantisation range take the exact path (moments still right);
+    a speed whose second moment no longer fits 64 bits makes the push return
+    SF_WARN_MOMENT_RANGE instead of silently corrupting uu/vv/ww."""
+    from starfish_amd.plugin import Simulation
+    deck = D.config_c1()
+    m = deck.mesh
+    vmax = 1e4
+    sim = Simulation(deck, capacity=1024, track_ids=False, vmax=vmax)
+    try:
+        mat = sim.materials[0]
+        sim.set_efield(np.zeros(sim.nn), np.zeros(sim.nn))
+        i, j = 40, 20
+        xs = np.array([m.x0[0] + (i + 0.25) * m.dh[0]] * 2)
+        ys = np.array([m.x0[1] + (j + 0.5) * m.dh[1]] * 2)
+        # 100 vmax: beyond the fast range, within 64 bits; tiny dt keeps them in the cell
+        sim.ctx.call("sf_ctx_set_dt", 1e-12)
+        mat.add_particles(xs, ys, np.array([100 * vmax, -100 * vmax]), np.zeros(2), np.zeros(2), rewind=False)
+        assert mat.update() == 0
+        spwt = deck.materials[0].spwt
+        uu = mat.field("uu-sum")
+        assert abs(uu.sum() / (2 * spwt * (100 * vmax) ** 2) - 1) < 1e-9
+        assert abs(mat.field("u-sum").sum()) < 1e-6 * spwt * vmax
+        # 1e4 vmax overflows the int64 second-moment channel: reported
+        mat.add_particles(xs[:1], ys[:1], np.array([1e4 * vmax]), np.zeros(1), np.zeros(1), rewind=False)
+        from starfish_amd._lib import SF_WARN_MOMENT_RANGE
+        assert mat.update() == SF_WARN_MOMENT_RANGE
+    finally:
+        sim.close()
