@@ -27,8 +27,9 @@ extern "C" {
 
 #define SF_OK 0
 #define SF_WARN_NOT_CONVERGED 1
-/* sf_push_deposit: a particle's moment contribution exceeded the int64 fixed-point range (a speed
-   far beyond the species' vmax); the moment channels of that step are not valid */
+/* sf_push_deposit: a particle's moment contribution exceeded the int64 fixed-point range (a
+   velocity deviating from the species' mean by about 2^8 times the previous step's largest
+   deviation); the moment channels of that step are not valid */
 #define SF_WARN_MOMENT_RANGE 2
 #define SF_ERR_ARG (-1)
 #define SF_ERR_HIP (-2)
@@ -136,10 +137,12 @@ int sf_mesh_setup(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_type, c
 /* ------------------------------------------------------------------------------------------
  * Kinetic species (KineticMaterial, core/materials/KineticMaterial.java; Material keys
  * charge/molwt/spwt/init nd_back/init T, Material.java:62-80, 580-608).
- * vmax: expected maximum particle speed (m/s), sets the fixed-point scale of the velocity
- * moment channels (0 selects 1e5 m/s).  Faster particles are still deposited exactly (a slower
- * quantisation); beyond about 2000 vmax a contribution no longer fits 64 bits and the push
- * returns SF_WARN_MOMENT_RANGE.  capacity: maximum particles held on the device.
+ * vmax: expected maximum particle speed (m/s, 0 selects 1e5), sets the fixed-point scale of the
+ * mover sums (momentum, speed).  The moment channels carry v - c and (v - c)^2 about the species'
+ * mean velocity c, at scales set at every push from the particles' largest deviation from c
+ * (measured on the first push after a bulk load, then carried by each push); particles beyond
+ * the scales' fast range are deposited with an exact slower quantisation.
+ * capacity: maximum particles held on the device.
  * ------------------------------------------------------------------------------------------ */
 typedef struct {
     double charge;       /* in units of e (Material "charge") */
@@ -424,9 +427,12 @@ int sf_cnn_params(sf_cnn* cnn, int get, float* buf, int64_t* n_out);
 /* writes the network's phi for the current rho / boundary values of `target` into its phi,
    the starting point of the next sf_poisson_solve */
 int sf_cnn_predict(sf_cnn* cnn, sf_ctx* target);
-/* sf_poisson_solve started from the network's guess.  policy 0: always from the guess;
-   policy 1: from whichever of the current phi and the guess has the smaller nonlinear residual.
-   *used_guess (may be NULL) reports the choice.  Same tolerance and status as sf_poisson_solve. */
+/* sf_poisson_solve started from the network's guess.  policy 0: from the guess; policy 1: from
+   whichever of the current phi and the guess has the smaller nonlinear residual.  Under either
+   policy a guess with a non-finite residual is replaced by the current phi, and a solve that
+   diverges from the guess is repeated from the current phi, so the plain solve stays the
+   fallback.  *used_guess (may be NULL) reports whether the solve started from the guess.  Same
+   tolerance and status as sf_poisson_solve. */
 int sf_poisson_solve_learned(sf_ctx* ctx, sf_cnn* cnn, int policy, sf_solve_info* info, int* used_guess);
 /* diagnostics: activation `which` of batch slot 0 after the last forward pass, as FP32
    (0 input [H][W][32], 1 e1a, 2 u1 = up(d2)++e1, 3 p1, 4 e2a, 5 u2 = up(e3)++e2, 6 p2, 7 e3a,

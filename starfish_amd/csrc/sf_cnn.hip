@@ -941,21 +941,29 @@ int sf_poisson_solve_learned(sf_ctx* ctx, sf_cnn* c, int policy, sf_solve_info* 
     if (policy == 1) {
         int rc = poisson_rel_residual(ctx, &r_prev);
         if (rc < 0) return rc;
-        SF_HIP(ctx, hipMemcpyAsync(c->phi_save, ctx->d_phi, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
     }
+    SF_HIP(ctx, hipMemcpyAsync(c->phi_save, ctx->d_phi, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
     int rc = sf_cnn_predict(c, ctx);
     if (rc < 0) return sf_fail(ctx, rc, "sf_poisson_solve_learned: prediction failed: %s", c->ctx->err.c_str());
-    bool use = true;
-    if (policy == 1) {
-        rc = poisson_rel_residual(ctx, &r_guess);
-        if (rc < 0) return rc;
-        if (!(r_guess < r_prev)) {
-            use = false;
-            SF_HIP(ctx, hipMemcpyAsync(ctx->d_phi, c->phi_save, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
-        }
+    // The guess is only a starting point: one whose nonlinear residual is not finite (exp of a
+    // wild potential), or worse than the previous potential's under policy 1, is replaced by the
+    // previous potential, and a solve that diverges from the guess is repeated from it.
+    rc = poisson_rel_residual(ctx, &r_guess);
+    if (rc < 0) return rc;
+    bool use = std::isfinite(r_guess) && (policy == 0 || r_guess < r_prev);
+    auto restore = [&]() -> int {
+        SF_HIP(ctx, hipMemcpyAsync(ctx->d_phi, c->phi_save, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        return SF_OK;
+    };
+    if (!use && (rc = restore()) < 0) return rc;
+    rc = poisson_solve(ctx, info);
+    if (rc < 0 && use) {
+        use = false;
+        if ((rc = restore()) < 0) return rc;
+        rc = poisson_solve(ctx, info);
     }
     if (used_guess) *used_guess = use ? 1 : 0;
-    return poisson_solve(ctx, info);
+    return rc;
 }
 
 // Network output for the current state of `target` (rho, Dirichlet values) written as phi.

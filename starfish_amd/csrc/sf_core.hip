@@ -131,6 +131,8 @@ static void free_species(Species* sp) {
     dfree(sp->gas);
     dfree(sp->flags);
     dfree(sp->qcount);
+    dfree(sp->tile_nslow);
+    dfree(sp->slow_items);
     dfree(sp->den);
     dfree(sp->counts);
     dfree(sp->offsets);
@@ -290,18 +292,19 @@ int sf_mesh_setup(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_type, c
     TRY(dalloc(ctx, &ctx->d_nseg_idx, idx.size()));
     TRY(dalloc(ctx, &ctx->d_seg, (size_t)nseg));
     TRY(dalloc(ctx, &ctx->d_hasseg, nn));
-    // Cells whose particles can take the push's fast path: a move of less than one cell from
-    // cell (i, j) has a ProcessBoundary bounding box inside nodes [i-2, i+2] x [j-2, j+2] (one
-    // node of margin for rounding) and stays inside the mesh when 2 <= i <= nci-3 and
-    // 2 <= j <= ncj-3.  Any other cell, or one whose neighbourhood lists a DIRICHLET/SINK
-    // segment, is "slow": its particles run the full reference loop.
+    // Cells whose particles can take the push's fast path.  The fast path moves a particle by
+    // less than 0.99 cell per axis, so from cell (i, j) (lc in [i, i+1)) the new lc lies in
+    // (i - 0.99, i + 1.99): ProcessBoundary's bounding box (KineticMaterial.java:482-512) spans
+    // nodes [i-1, i+1] x [j-1, j+1], and the particle stays strictly inside the mesh when
+    // 1 <= i <= nci-2 and 1 <= j <= ncj-2 (no face is crossed).  Any other cell, or one whose
+    // box lists a DIRICHLET/SINK segment, is "slow": its particles run the full reference loop.
     const int nci = ni - 1, ncj = nj - 1;
     std::vector<uint8_t> cslow((size_t)nci * ncj, 1);
-    for (int j = 2; j <= ncj - 3; j++)
-        for (int i = 2; i <= nci - 3; i++) {
+    for (int j = 1; j <= ncj - 2; j++)
+        for (int i = 1; i <= nci - 2; i++) {
             uint8_t any = 0;
-            for (int b = j - 2; b <= j + 2 && !any; b++)
-                for (int a = i - 2; a <= i + 2; a++)
+            for (int b = j - 1; b <= j + 1 && !any; b++)
+                for (int a = i - 1; a <= i + 1; a++)
                     if (has[(size_t)b * ni + a]) { any = 1; break; }
             cslow[(size_t)j * nci + i] = any;
         }
@@ -349,12 +352,11 @@ int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species) {
     sp->qm = sp->charge / sp->mass;       // Material.init (Material.java:711)
     sp->capacity = desc->capacity;
     double vmax = desc->vmax > 0 ? desc->vmax : 1e5;
-    // fixed-point scales: a particle contributes at most ~2^40 per channel
-    int kv = (int)std::floor(std::log2(std::ldexp(1.0, 40) / vmax));
-    int kuu = (int)std::floor(std::log2(std::ldexp(1.0, 40) / (vmax * vmax)));
+    // fixed-point scale of the mover sums: a particle at vmax contributes ~2^40; the deposit
+    // channels take their scales from the particles at each push
+    sp->kstat = std::ldexp(1.0, (int)std::floor(std::log2(std::ldexp(1.0, 40) / vmax)));
     sp->kscale[0] = std::ldexp(1.0, 40);
-    for (int c = 1; c <= 3; c++) sp->kscale[c] = std::ldexp(1.0, kv);
-    for (int c = 4; c <= 6; c++) sp->kscale[c] = std::ldexp(1.0, kuu);
+    for (int c = 1; c < NCH; c++) sp->kscale[c] = 1.0;
     const size_t nn = (size_t)ctx->mp.ni * ctx->mp.nj;
     const size_t ncells = (size_t)ctx->mp.nci * ctx->mp.ncj;
     const size_t cap = (size_t)desc->capacity;
@@ -374,7 +376,11 @@ int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species) {
     if (!rc) rc = dalloc(ctx, &sp->mpc_sum, ncells);
     if (!rc) rc = dalloc(ctx, &sp->gas, NGAS * nn);
     if (!rc) rc = dalloc(ctx, &sp->flags, 1);
-    if (!rc) rc = dalloc(ctx, &sp->qcount, 1);
+    const size_t ntiles = (size_t)std::max(1, ctx->mp.ntx * ctx->mp.nty);
+    if (!rc) rc = dalloc(ctx, &sp->qcount, 2);
+    if (!rc) rc = dalloc(ctx, &sp->tile_nslow, ntiles);
+    // a tile of n slow particles makes ceil(n / 256) items: at most cap / 256 + tiles in all
+    if (!rc) rc = dalloc(ctx, &sp->slow_items, cap / 256 + ntiles + 1);
     if (!rc) rc = dalloc(ctx, &sp->den, 4 * nn);
     if (!rc) rc = dalloc(ctx, &sp->counts, ncells);
     if (!rc) rc = dalloc(ctx, &sp->offsets, ncells + 1);
@@ -418,6 +424,7 @@ int sf_particles_add(sf_ctx* ctx, int species, int64_t n, const double* x, const
     SF_HIP(ctx, hipSetDevice(ctx->device));
     return add_particles_device(ctx, sp, n, x, y, u, v, w, ids, mode);
 }
+
 
 int sf_particles_count(sf_ctx* ctx, int species, int64_t* n) {
     SF_CHECK_CTX(ctx);

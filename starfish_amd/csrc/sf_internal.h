@@ -93,12 +93,17 @@ struct PushC {
     double qm_dt;          // q/m * dt
     double qm;             // q/m
     double dt;
-    // fixed-point scales 2^k: density channel, the three velocity channels, the three second
-    // moments (one value per group, so the push holds three scalars instead of seven)
+    // fixed-point scales 2^k of this step: density channel, the three velocity channels, the
+    // three second moments (one value per group, so the push holds three scalars, not seven)
     double kn, kv, kvv;
     double spwt;
-    // speeds beyond vfast need the exact (slow) quantisation, beyond vover a contribution does
-    // not fit an int64 (the step's moments are then flagged as invalid)
+    // the velocity channels carry v - c and (v - c)^2 for the step's reference velocity c (the
+    // species' mean velocity), so their scales follow the spread of the velocities about it
+    double c0, c1, c2;
+    // scale of the mover sums (momentum and speed, physical velocities)
+    double kst;
+    // deviations max_k |v_k - c_k| beyond vfast need the exact (slow) quantisation, beyond vover
+    // a contribution does not fit an int64 (the step's moments are then flagged as invalid)
     double vfast, vover;
 };
 
@@ -112,8 +117,10 @@ struct Species {
     int since_sort = 0;
     int cur = 0;
     PState buf[2]{};
-    int32_t* key = nullptr;        // new cell key per particle (-1 dead); the push's slow-path queue
-    int* qcount = nullptr;         // length of the slow-path queue of the last push
+    int32_t* key = nullptr;        // new cell key per particle (-1 dead); the push's slow lists
+    int* qcount = nullptr;         // [2] particles the last push left to the reference loop, work items
+    int32_t* tile_nslow = nullptr; // [tiles] length of each push tile's slow list (kept in key)
+    int2* slow_items = nullptr;    // (tile, chunk) work items of the slow lists
     unsigned long long* acc = nullptr;   // [NCH][nn] int64 deposit accumulators
     double* sums = nullptr;        // [NCH][nn] running moment sums, physical units (FP64)
     long long* mpc_sum = nullptr;  // [ncells]
@@ -129,7 +136,13 @@ struct Species {
     int num_samples = 0;
     bool first_time = true;        // KineticMaterial.first_time: computeFields on the first update
     int64_t pushes = 0;            // sf_push_deposit calls (the step counter when none is set)
-    double kscale[NCH];
+    double kscale[NCH];            // fixed-point scales of the last push (set per push)
+    double kstat = 0;              // fixed-point scale of the mover sums (from the species' vmax)
+    // reference velocity and a bound of max_k |v_k - c_k| over the particles, from the last push
+    // (its mover sums and deviations); invalid until the first push or after particles are loaded
+    double vref[3] = {0, 0, 0};
+    double vbound = 0;
+    bool vrange_valid = false;
     double stats_host[5] = {0, 0, 0, 0, 0};
 };
 

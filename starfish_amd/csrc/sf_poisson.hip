@@ -51,6 +51,7 @@ struct PoissonState {
     double* scal = nullptr;       // device scalars
     int* istate = nullptr;        // [0] done (1 converged, 2 not finite), [1] iterations at done
     int nblk_tile = 0, nblk_elem = 0;
+    int last_its = 7;             // PCG iterations of the last linear solve (sizes the next batch)
     double* host_scal = nullptr;  // pinned
     int* host_istate = nullptr;
     hipEvent_t e0{}, e1{};
@@ -187,18 +188,21 @@ __device__ __forceinline__ double lred_apply64(const MeshP& m, uint8_t c, int i,
 }
 
 // Newton residual r = f + bfix - L x - W b(x), Jacobian diagonal D = diag(L) + W b'(x);
-// also ||r||^2 partials.  f = W * rho / eps.
+// also ||r||^2 partials.  f = W * rho / eps.  The accumulated correction y is zero at this point,
+// so r is also the first refinement residual: the FP32 PCG starts from d = 0, res = r.
 __global__ void newton_residual(MeshP m, const uint8_t* __restrict__ cls, const double* __restrict__ W,
                                 const double* __restrict__ bfix, const double* __restrict__ rho,
                                 const double* __restrict__ x, double* __restrict__ r, float* __restrict__ D,
-                                double* __restrict__ P, double inv_eps, double Cn0, double phi0, double inv_kTe,
-                                int linear, double* __restrict__ partial, const int* __restrict__ go) {
+                                double* __restrict__ P, float* __restrict__ d, float* __restrict__ res,
+                                double inv_eps, double Cn0, double phi0, double inv_kTe, int linear,
+                                double* __restrict__ partial, const int* __restrict__ go) {
     if (go != nullptr && go[0] != 1) return;   // queued behind a linear solve that has not converged
     const int64_t nn = (int64_t)m.ni * m.nj;
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
         uint8_t c = cls[u];
-        if (c == CLS_NOT_FREE) { r[u] = 0; D[u] = 1.0f; P[u] = 0; continue; }
+        d[u] = 0.0f;
+        if (c == CLS_NOT_FREE) { r[u] = 0; D[u] = 1.0f; P[u] = 0; res[u] = 0.0f; continue; }
         int i = (int)(u % m.ni), j = (int)(u / m.ni);
         double diagL;
         double Lx = lred_apply64(m, c, i, j, u, x, diagL);
@@ -210,6 +214,7 @@ __global__ void newton_residual(MeshP m, const uint8_t* __restrict__ cls, const 
         }
         double ru = Wu * rho[u] * inv_eps + bfix[u] - Lx - Wu * bx;
         r[u] = ru;
+        res[u] = (float)ru;
         P[u] = Wu * bp;
         D[u] = (float)(diagL + Wu * bp);
         acc += ru * ru;
@@ -278,12 +283,11 @@ __global__ void __launch_bounds__(256) pcg_spmv(MeshP m, const uint8_t* __restri
                                                 const float* __restrict__ pold, float* __restrict__ pnew,
                                                 float* __restrict__ q, const double* __restrict__ pz_cur,
                                                 const double* __restrict__ pz_prev, int nbz, int k,
-                                                double* __restrict__ ppq, const double* __restrict__ scal,
-                                                int* istate) {
+                                                double* __restrict__ ppq, double tol2, int* istate) {
     if (istate[0]) return;
     double cur[2];
     combine_partials<2>(pz_cur, nbz, cur);   // (rz_k, rr_k)
-    if (!(cur[1] > scal[S_TOL2])) {          // converged, or not finite
+    if (!(cur[1] > tol2)) {                  // converged, or not finite
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             istate[0] = (cur[1] == cur[1]) ? 1 : 2;
             istate[1] = k;
@@ -403,10 +407,16 @@ __device__ __forceinline__ void tile_sgs(const TileRows& T, int i0, int j0, cons
     }
 }
 
-// z = M^-1 res of the refinement residual and the (r.z, r.r) partials of iteration 0
+// z = M^-1 res of the refinement residual and the (r.z, r.r) partials of iteration 0; clears the
+// PCG state word for the iterations queued behind it.
 __global__ void __launch_bounds__(256) pcg_precond(MeshP m, const uint8_t* __restrict__ cls,
                                                    const float* __restrict__ D, const float* __restrict__ res,
-                                                   float* __restrict__ z, double* __restrict__ pz0) {
+                                                   float* __restrict__ z, double* __restrict__ pz0,
+                                                   int* __restrict__ istate) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        istate[0] = 0;
+        istate[1] = 0;
+    }
     __shared__ float rs[PT_H * PT_W], zs[PT_H * PT_W];
     const int nbx = (m.ni + PT_W - 1) / PT_W;
     const int i0 = (blockIdx.x % nbx) * PT_W, j0 = (blockIdx.x / nbx) * PT_H;
@@ -507,13 +517,16 @@ __global__ void absmax_kernel(int64_t nn, const double* __restrict__ y, double* 
 // x += limited y on the free nodes; y is cleared.  ypart receives per-block partial sums of y^2
 // (the Newton step before the limit), the quantity of the reference's convergence test
 // ||y||_2 / N < nl_tol (Solver.java:371-381, Vec.norm).
+// dlast (may be null): the last FP32 PCG correction, added to y first (y += d fused in).
 __global__ void __launch_bounds__(256) newton_update(int64_t nn, const uint8_t* __restrict__ cls,
-                                                     double* __restrict__ x, double* __restrict__ y, double kte,
+                                                     double* __restrict__ x, double* __restrict__ y,
+                                                     const float* __restrict__ dlast, double kte,
                                                      const int* __restrict__ go, double* __restrict__ ypart) {
     if (go != nullptr && go[0] != 1) return;
     double v2[1] = {0.0};
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
         double v = y[u];
+        if (dlast != nullptr) v += (double)dlast[u];
         if (cls[u] != CLS_NOT_FREE) v2[0] += v * v;
         if (kte > 0) {
             if (v > kte) v = kte * (1.0 + log(v / kte));
@@ -813,11 +826,8 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     auto launch_residual = [&](const int* go) {
         hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                            (const double*)P->W, (const double*)P->bfix, (const double*)ctx->d_rho,
-                           (const double*)P->x, P->r, P->D, P->P, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
-                           P->partial, go);
-        hipLaunchKernelGGL(refine_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls,
-                           (const double*)P->r, (const double*)P->y, (const double*)P->P, P->rr, P->d, P->res,
-                           P->partial, go);
+                           (const double*)P->x, P->r, P->D, P->P, P->d, P->res, inv_eps, Cn0, D.phi0, inv_kTe,
+                           D.linear, P->partial, go);
         finalize<1>(P, st, P->partial, ge, S_NORM2);
     };
     bool have_residual = false;   // the residual of the current x is already in host_scal
@@ -857,12 +867,9 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             if (rr <= lin_target || refine >= 8 || pcg_total >= max_it * nl_max) break;
             // inner FP32 tolerance: a relative reduction FP32 can deliver
             double inner_tol = std::max(lin_target, 1e-5 * rr);
-            double tol2 = inner_tol * inner_tol;
-            SF_HIP(ctx, hipMemcpyAsync(P->scal + S_TOL2, &tol2, 8, hipMemcpyHostToDevice, st));
-            int zero[2] = {0, 0};
-            SF_HIP(ctx, hipMemcpyAsync(P->istate, zero, 8, hipMemcpyHostToDevice, st));
+            const double tol2 = inner_tol * inner_tol;
             hipLaunchKernelGGL(pcg_precond, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const float*)P->D,
-                               (const float*)P->res, P->z, pz0);
+                               (const float*)P->res, P->z, pz0, P->istate);
             // When this linear solve is the Newton step's last (it aims at the step's own target),
             // the update and the next Newton residual are queued behind the first batch of PCG
             // iterations.  They run only if that batch converged (the kernels read istate), and
@@ -870,7 +877,9 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             const bool ahead = !debug && inner_tol == lin_target;
             float* pa = P->p0;
             float* pb = P->p1;
-            int k = 0, its = 0, batch = 8;   // iterations queued between host checks; grows while unconverged
+            // iterations queued between host checks: one more than the last linear solve needed,
+            // doubling while unconverged (a converged iteration turns the rest into no-ops)
+            int k = 0, its = 0, batch = std::min(8, std::max(3, P->last_its + 1));
             bool first_batch = true, done_in_first = false;
             while (true) {
                 const int nb = std::min(batch, max_it - k);
@@ -879,8 +888,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
                     double* nxt = (k & 1) ? pz0 : pz1;
                     hipLaunchKernelGGL(pcg_spmv, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                                        (const float*)P->D, (const float*)P->z, (const float*)pa, pb, P->q,
-                                       (const double*)cur, (const double*)nxt, gt, k, P->ppq,
-                                       (const double*)P->scal, P->istate);
+                                       (const double*)cur, (const double*)nxt, gt, k, P->ppq, tol2, P->istate);
                     hipLaunchKernelGGL(pcg_update, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
                                        (const float*)P->D, (const float*)pb, (const float*)P->q, P->d, P->res,
                                        P->z, (const double*)P->ppq, gt, (const double*)cur, gt, nxt,
@@ -890,10 +898,9 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
                 const bool queued = ahead && first_batch;
                 if (queued) {
                     const int* go = (const int*)P->istate;
-                    hipLaunchKernelGGL(accumulate_y, dim3(ge), dim3(256), 0, st, nn, (const float*)P->d, P->y, go);
                     hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x,
-                                       P->y, lim, go, ypart);
-                    finalize<1>(P, st, ypart, ge, S_YNORM2);
+                                       P->y, (const float*)P->d, lim, go, ypart);
+                    if (nl_tol > 0) finalize<1>(P, st, ypart, ge, S_YNORM2);
                     launch_residual(go);
                 }
                 SF_HIP(ctx, hipGetLastError());
@@ -910,6 +917,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
                 first_batch = false;
             }
             pcg_total += its;
+            P->last_its = its;
             if (ahead && done_in_first && P->host_istate[0] == 1) {
                 stepped = true;
                 break;
@@ -930,9 +938,9 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             hipLaunchKernelGGL(absmax_kernel, dim3(ge), dim3(256), 0, st, nn, (const double*)P->y, P->partial);
             finalize<1, true>(P, st, P->partial, ge, S_YMAX);
         }
-        hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y, lim,
-                           nullptr, ypart);
-        finalize<1>(P, st, ypart, ge, S_YNORM2);
+        hipLaunchKernelGGL(newton_update, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, P->x, P->y,
+                           (const float*)nullptr, lim, nullptr, ypart);
+        if (nl_tol > 0) finalize<1>(P, st, ypart, ge, S_YNORM2);
         if (nl_tol > 0) {
             rc = read_scalars(ctx, P);
             if (rc) return rc;
@@ -1196,7 +1204,7 @@ int poisson_rel_residual(sf_ctx* ctx, double* rel) {
     finalize<1>(P, st, P->partial, ge, S_R0);
     hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
                        (const double*)P->bfix, (const double*)ctx->d_rho, (const double*)P->x, P->r, P->D, P->P,
-                       inv_eps, Cn0, D.phi0, inv_kTe, D.linear, P->partial, nullptr);
+                       P->d, P->res, inv_eps, Cn0, D.phi0, inv_kTe, D.linear, P->partial, nullptr);
     finalize<1>(P, st, P->partial, ge, S_NORM2);
     int rc = read_scalars(ctx, P);
     if (rc) return rc;

@@ -13,6 +13,8 @@
 //   normalise (normalize_nodes): int64 channels -> nd, u, v, w; running sums += channels.
 // Particles are read coalesced in cell-sorted order.  The deposit and the sums are 64-bit
 // fixed point, so every density and moment is independent of the order of the additions.
+#include <cstring>
+
 #include "sf_particle_math.h"
 #include "sf_rng.h"
 
@@ -22,14 +24,18 @@ namespace sf {
 // deposit helpers
 // ----------------------------------------------------------------------------------------
 
+// The 7 channels of one particle at unit weight: 1, v - c and (v - c)^2 per component, each at
+// its fixed-point scale.  normalize_nodes turns the step's integer totals back into the
+// reference's sums of mpw, mpw v and mpw v^2.
 __device__ __forceinline__ void channel_values(const PushC& pc, double v0, double v1, double v2, double* val) {
+    const double d0 = v0 - pc.c0, d1 = v1 - pc.c1, d2 = v2 - pc.c2;
     val[0] = pc.kn;
-    val[1] = v0 * pc.kv;
-    val[2] = v1 * pc.kv;
-    val[3] = v2 * pc.kv;
-    val[4] = v0 * v0 * pc.kvv;
-    val[5] = v1 * v1 * pc.kvv;
-    val[6] = v2 * v2 * pc.kvv;
+    val[1] = d0 * pc.kv;
+    val[2] = d1 * pc.kv;
+    val[3] = d2 * pc.kv;
+    val[4] = d0 * d0 * pc.kvv;
+    val[5] = d1 * d1 * pc.kvv;
+    val[6] = d2 * d2 * pc.kvv;
 }
 
 // Global-memory deposit (used for the unsorted tail and for particles that left the tile).
@@ -66,9 +72,10 @@ __device__ __forceinline__ void deposit_global_exact(const MeshP& m, unsigned lo
     }
 }
 
-// Largest velocity component: the one-FMA quantisation needs it below pc.vfast.
-__device__ __forceinline__ double vmax_component(double v0, double v1, double v2) {
-    return fmax(fabs(v0), fmax(fabs(v1), fabs(v2)));
+// Largest deviation of a velocity component from the reference velocity: the one-FMA
+// quantisation needs it below pc.vfast.
+__device__ __forceinline__ double vdev(const PushC& pc, double v0, double v1, double v2) {
+    return fmax(fabs(v0 - pc.c0), fmax(fabs(v1 - pc.c1), fabs(v2 - pc.c2)));
 }
 
 // Load of the FP32 particle state, conversion to logical coordinates.
@@ -117,16 +124,20 @@ __device__ __forceinline__ void cell_offsets(const MeshP& m, double lc0, double 
     }
 }
 
+// Mover sums of the surviving particles (count, momentum and speed at the scale kst) and the
+// largest deviation max_k |v_k - c_k|, which bounds the next step's channel scales.
 struct StatAcc {
     long long n, p0, p1, p2, e;
+    double dmax;
 };
 
 __device__ __forceinline__ void stat_add(StatAcc& s, const PushC& pc, double v0, double v1, double v2) {
     s.n += 1;
-    s.p0 += quant(1.0, v0 * pc.kv);
-    s.p1 += quant(1.0, v1 * pc.kv);
-    s.p2 += quant(1.0, v2 * pc.kv);
-    s.e += quant(1.0, sqrt(v0 * v0 + v1 * v1 + v2 * v2) * pc.kv);
+    s.p0 += quant(1.0, v0 * pc.kst);
+    s.p1 += quant(1.0, v1 * pc.kst);
+    s.p2 += quant(1.0, v2 * pc.kst);
+    s.e += quant(1.0, sqrt(v0 * v0 + v1 * v1 + v2 * v2) * pc.kst);
+    s.dmax = fmax(s.dmax, vdev(pc, v0, v1, v2));
 }
 
 __device__ __forceinline__ long long wave_sum_ll(long long v) {
@@ -142,6 +153,7 @@ __device__ void stat_flush(StatAcc s, long long* stats) {
     s.p1 = wave_sum_ll(s.p1);
     s.p2 = wave_sum_ll(s.p2);
     s.e = wave_sum_ll(s.e);
+    for (int o = 32; o > 0; o >>= 1) s.dmax = fmax(s.dmax, __shfl_xor(s.dmax, o));
     if ((threadIdx.x & 63) == 0) {
         unsigned long long* slot = (unsigned long long*)stats + (size_t)(blockIdx.x % STAT_SLOTS) * STAT_STRIDE;
         atomicAdd(&slot[0], (unsigned long long)s.n);
@@ -149,6 +161,8 @@ __device__ void stat_flush(StatAcc s, long long* stats) {
         atomicAdd(&slot[2], (unsigned long long)s.p1);
         atomicAdd(&slot[3], (unsigned long long)s.p2);
         atomicAdd(&slot[4], (unsigned long long)s.e);
+        // a non-negative double orders like its bit pattern
+        if (s.dmax > 0) atomicMax(&slot[5], (unsigned long long)__double_as_longlong(s.dmax));
     }
 }
 
@@ -196,7 +210,7 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
                                                        int* flags) {
     __shared__ TileLds L;
     const int t = threadIdx.x;
-    StatAcc st{0, 0, 0, 0, 0};
+    StatAcc st{0, 0, 0, 0, 0, 0.0};
     // One workgroup per tile.  The dispatcher starts a waiting tile as soon as a workgroup slot
     // frees up, so tiles that take longer (bounces near objects, particles that left the LDS
     // window) do not leave the rest of the device idle at the end of the launch; this balances
@@ -333,7 +347,7 @@ __device__ __forceinline__ void push_deposit_tile_body(const MeshP& m, const Mes
                     double val[NCH];
                     channel_values(pc, vxn, vyn, vzn, val);
                     int li = ci - n_i0, lj = cj - n_j0;
-                    const double vm = vmax_component(vxn, vyn, vzn);
+                    const double vm = vdev(pc, vxn, vyn, vzn);
                     if (vm >= pc.vfast) {
                         deposit_global_exact(m, acc, nn, ci, cj, di, dj, val);
                         atomicOr(flags, vm >= pc.vover ? (PUSH_FLAG_FAST | PUSH_FLAG_OVERFLOW) : PUSH_FLAG_FAST);
@@ -459,7 +473,7 @@ __device__ __forceinline__ void push_one_global(const MeshP& m, const MeshD& md,
         if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
             double val[NCH];
             channel_values(pc, vxn, vyn, vzn, val);
-            const double vm = vmax_component(vxn, vyn, vzn);
+            const double vm = vdev(pc, vxn, vyn, vzn);
             if (vm >= pc.vfast) {
                 deposit_global_exact(m, acc, nn, ci, cj, di, dj, val);
                 atomicOr(flags, vm >= pc.vover ? (PUSH_FLAG_FAST | PUSH_FLAG_OVERFLOW) : PUSH_FLAG_FAST);
@@ -481,26 +495,12 @@ __global__ void __launch_bounds__(256) push_deposit_tail(MeshP m, MeshD md, Push
                                                          int64_t p1, unsigned long long* __restrict__ acc,
                                                          int32_t* __restrict__ counts, long long* stats,
                                                          int* flags) {
-    StatAcc st{0, 0, 0, 0, 0};
+    StatAcc st{0, 0, 0, 0, 0, 0.0};
     for (int64_t p = p0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < p1;
          p += (int64_t)gridDim.x * blockDim.x) {
         if (s.cell[p] < 0) continue;   // hole left by a dead particle
         push_one_global<HAS_B>(m, md, pc, s, p, acc, counts, flags, st);
     }
-    stat_flush(st, stats);
-}
-
-// The particles the fast tile kernel queued (slots of the cell-sorted array).
-__global__ void __launch_bounds__(256) push_deposit_queue(MeshP m, MeshD md, PushC pc, PState s,
-                                                          const int32_t* __restrict__ queue,
-                                                          const int* __restrict__ qcount,
-                                                          unsigned long long* __restrict__ acc,
-                                                          int32_t* __restrict__ counts, long long* stats,
-                                                          int* flags) {
-    StatAcc st{0, 0, 0, 0, 0};
-    const int nq = *qcount;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nq; k += gridDim.x * blockDim.x)
-        push_one_global<false>(m, md, pc, s, queue[k], acc, counts, flags, st);
     stat_flush(st, stats);
 }
 
@@ -519,8 +519,10 @@ __global__ void __launch_bounds__(256) push_deposit_queue(MeshP m, MeshD md, Pus
 // tile's cell histogram.  Integer sums modulo 2^64 keep this exact and order independent.
 // ----------------------------------------------------------------------------------------
 struct FastC {
-    float qmdt, cdx, cdy, vfast;   // q/m dt, dt/dx, dt/dy, fast range of the quantisation
-    double kn, kv, kvv;            // fixed-point scales
+    float qmdt, cdx, cdy;          // q/m dt, dt/dx, dt/dy
+    double vfast;                  // fast range of the quantisation (deviation from c)
+    double kn, kv, kvv, kst;       // fixed-point scales of the channels and of the mover sums
+    double c0, c1, c2;             // reference velocity of the channels
     int ni, nj, nci, ncj, ntx;
 };
 
@@ -530,14 +532,79 @@ struct FastLds {
     float2 ef[TILE_NODES];
     uint8_t slow[TILE_CELLS];
     int seg_start[TILE_H], seg_pre[TILE_H + 1];
-    int32_t o_cell[PUSH_THREADS];
-    float o_fx[PUSH_THREADS], o_fy[PUSH_THREADS], o_vx[PUSH_THREADS], o_vy[PUSH_THREADS], o_vz[PUSH_THREADS];
+    int nslow;
 };
 
-constexpr int32_t SLOT_KEEP = -2;   // staging marker: the slot is not written by this kernel
+constexpr int32_t SLOT_KEEP = -2;   // marker: the slot is not written by the fast kernel
+constexpr int SLOW_CHUNK = 256;     // slow-list particles per work item of push_tile_slow
 
 __device__ __forceinline__ unsigned long long qbits(double w, double v) {
     return (unsigned long long)__double_as_longlong(fma(w, v, QMAGIC));
+}
+
+// Row segments of a tile's particles: row r of the tile (cells ti0 .. ti0+TILE_W-1 of row
+// tj0 + r) holds slots [seg_start[r], seg_start[r] + len_r) of the cell-sorted array, and
+// seg_pre[r] = len_0 + ... + len_{r-1}.  The concatenated segments number the tile's slots
+// 0 .. seg_pre[TILE_H)-1; tile_slot maps such an index back to its slot.
+__device__ __forceinline__ void tile_segments(const int32_t* __restrict__ offsets, int nci, int ncj, int ti0, int tj0,
+                                              int* seg_start, int* seg_pre) {
+    int pre = 0;
+#pragma unroll
+    for (int r = 0; r < TILE_H; r++) {
+        const int row = tj0 + r;
+        int st = 0, len = 0;
+        if (row < ncj) {
+            st = offsets[row * nci + ti0];
+            len = offsets[row * nci + min(ti0 + TILE_W, nci)] - st;
+        }
+        seg_start[r] = st;
+        seg_pre[r] = pre;
+        pre += len;
+    }
+    seg_pre[TILE_H] = pre;
+}
+__device__ __forceinline__ int64_t tile_slot(const int* seg_start, const int* seg_pre, int k) {
+    int r = 0;
+#pragma unroll
+    for (int q = 1; q < TILE_H; q++) r += k >= seg_pre[q] ? 1 : 0;
+    return (int64_t)seg_start[r] + (k - seg_pre[r]);
+}
+
+// Flush of a tile's LDS accumulators (raw patterns fma(w, value, 1.5 2^52), one per channel
+// and corner from every particle counted in hist): node (a, b) of the window received one
+// pattern per channel from every particle of window cells (a, b), (a-1, b), (a, b-1),
+// (a-1, b-1), so it subtracts that many bit patterns of 1.5 2^52 (integer sums modulo 2^64
+// keep this exact and order independent).  The cell histogram goes to the global counts.
+__device__ __forceinline__ void flush_tile(const unsigned long long (*lacc)[TILE_NODES], const int* hist, int w_i0,
+                                           int w_j0, int ni, int nj, int nci, unsigned long long* __restrict__ acc,
+                                           int32_t* __restrict__ counts, int nthreads) {
+    const int t = threadIdx.x;
+    const size_t nn = (size_t)ni * nj;
+    const unsigned long long MAGIC_BITS = (unsigned long long)__double_as_longlong(QMAGIC);
+    for (int k = t; k < TILE_NODES; k += nthreads) {
+        const int a = k % TILE_NW, b = k / TILE_NW;
+        const int i = w_i0 + a, j = w_j0 + b;
+        if (i < 0 || j < 0 || i >= ni || j >= nj) continue;
+        int np = 0;
+        if (a < TILE_CW && b < TILE_CH) np += hist[b * TILE_CW + a];
+        if (a > 0 && b < TILE_CH) np += hist[b * TILE_CW + a - 1];
+        if (a < TILE_CW && b > 0) np += hist[(b - 1) * TILE_CW + a];
+        if (a > 0 && b > 0) np += hist[(b - 1) * TILE_CW + a - 1];
+        if (np == 0) continue;
+        const unsigned long long bias = (unsigned long long)np * MAGIC_BITS;
+        const size_t u = (size_t)j * ni + i;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) {
+            const unsigned long long v = lacc[ch][k] - bias;
+            if (v != 0ull) atomicAdd(acc + ch * nn + u, v);
+        }
+    }
+    for (int k = t; k < TILE_CELLS; k += nthreads) {
+        const int h = hist[k];
+        if (h == 0) continue;
+        const int ci = w_i0 + k % TILE_CW, cj = w_j0 + k / TILE_CW;
+        atomicAdd(&counts[cj * nci + ci], h);
+    }
 }
 
 __global__ void __launch_bounds__(PUSH_THREADS, 4) push_tile_fast_xy(FastC fc, const float2* __restrict__ ef,
@@ -546,7 +613,9 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_tile_fast_xy(FastC fc, c
                                                                     unsigned long long* __restrict__ acc,
                                                                     int32_t* __restrict__ counts,
                                                                     long long* __restrict__ stats,
-                                                                    int32_t* __restrict__ queue,
+                                                                    int32_t* __restrict__ slow_list,
+                                                                    int32_t* __restrict__ tile_nslow,
+                                                                    int2* __restrict__ slow_items,
                                                                     int* __restrict__ qcount) {
     __shared__ FastLds L;
     const int t = threadIdx.x;
@@ -561,6 +630,7 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_tile_fast_xy(FastC fc, c
     const unsigned long long MAGIC_BITS = (unsigned long long)__double_as_longlong(QMAGIC);
 
     for (int k = t; k < NCH * TILE_NODES; k += PUSH_THREADS) (&L.acc[0][0])[k] = 0ull;
+    if (t == 0) L.nslow = 0;
     for (int k = t; k < TILE_CELLS; k += PUSH_THREADS) {
         L.hist[k] = 0;
         const int i = w_i0 + k % TILE_CW, j = w_j0 + k / TILE_CW;
@@ -572,63 +642,47 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_tile_fast_xy(FastC fc, c
         j = min(max(j, 0), fc.nj - 1);
         L.ef[k] = ef[(size_t)j * fc.ni + i];
     }
-    int first_row = TILE_H, first_start = 0, first_len = 0;
-    {
-        int pre = 0;
-#pragma unroll
-        for (int r = 0; r < TILE_H; r++) {
-            const int row = tj0 + r;
-            int st = 0, len = 0;
-            if (row < fc.ncj) {
-                const int k0 = row * fc.nci + ti0;
-                const int k1 = row * fc.nci + min(ti0 + TILE_W, fc.nci);
-                st = offsets[k0];
-                len = offsets[k1] - st;
-            }
-            if (first_row == TILE_H && len > 0) {
-                first_row = r;
-                first_start = st;
-                first_len = len;
-            }
-            if (t == 0) {
-                L.seg_start[r] = st;
-                L.seg_pre[r] = pre;
-            }
-            pre += len;
-        }
-        if (t == 0) L.seg_pre[TILE_H] = pre;
-    }
-    PReg cur;
-    if (pt < first_len) {
-        cur.g = first_start + pt;
-        cur.c = s.cell[cur.g];
-        cur.fx = s.fx[cur.g];
-        cur.fy = s.fy[cur.g];
-        cur.vx = s.vx[cur.g];
-        cur.vy = s.vy[cur.g];
-        cur.vz = s.vz[cur.g];
-    } else {
-        cur.g = 0;
-        cur.c = -1;
-        cur.fx = cur.fy = cur.vx = cur.vy = cur.vz = 0.0f;
-    }
+    if (t == 0) tile_segments(offsets, fc.nci, fc.ncj, ti0, tj0, L.seg_start, L.seg_pre);
     __syncthreads();
-
-    auto row_len = [&](int r) { return L.seg_pre[r + 1] - L.seg_pre[r]; };
+    // The tile's particles, numbered 0 .. total-1 across its row segments, are taken in windows of
+    // PUSH_THREADS; within a window thread t takes particle pt.  The waves run through the windows
+    // independently (no barrier): each loads its next window's particles before pushing the
+    // current ones and stores each pushed particle straight back to its slot.
+    const int total = L.seg_pre[TILE_H];
+    const int sg0 = L.seg_start[0], sg1 = L.seg_start[1], sg2 = L.seg_start[2], sg3 = L.seg_start[3];
+    const int pr1 = L.seg_pre[1], pr2 = L.seg_pre[2], pr3 = L.seg_pre[3];
+    auto slot_of = [&](int k) -> int64_t {
+        int b = sg0, pr = 0;
+        if (k >= pr1) { b = sg1; pr = pr1; }
+        if (k >= pr2) { b = sg2; pr = pr2; }
+        if (k >= pr3) { b = sg3; pr = pr3; }
+        return (int64_t)b + (k - pr);
+    };
+    auto load = [&](PReg& r, int k) {
+        if (k < total) {
+            r.g = slot_of(k);
+            r.c = s.cell[r.g];
+            r.fx = s.fx[r.g];
+            r.fy = s.fy[r.g];
+            r.vx = s.vx[r.g];
+            r.vy = s.vy[r.g];
+            r.vz = s.vz[r.g];
+        } else {
+            r.g = 0;
+            r.c = -1;
+            r.fx = r.fy = r.vx = r.vy = r.vz = 0.0f;
+        }
+    };
     // per-lane mover sums: raw FMA bit patterns, the bias removed with the lane's count at the end
     long long sn = 0;
     unsigned long long sp0 = 0, sp1 = 0, sp2 = 0, se = 0;
-    int cr = first_row, cb = 0;
-    while (cr < TILE_H) {
-        const int rlen = row_len(cr);
-        const int64_t rst = L.seg_start[cr];
-        const bool ok = cb + pt < rlen;
-        int nr = cr, nb = cb + PUSH_THREADS;
-        if (nb >= rlen) {
-            nr = cr + 1;
-            nb = 0;
-            while (nr < TILE_H && row_len(nr) == 0) nr++;
-        }
+    double dmax = 0.0;
+    PReg nxt;
+    load(nxt, pt);
+    for (int w0 = 0; w0 < total; w0 += PUSH_THREADS) {
+        const PReg cur = nxt;
+        const bool ok = w0 + pt < total;
+        load(nxt, w0 + PUSH_THREADS + pt);
         int32_t key = SLOT_KEEP;
         float fxn = 0.f, fyn = 0.f, vxn = 0.f, vyn = 0.f, vzn = 0.f;
         bool slow = false;
@@ -648,8 +702,10 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_tile_fast_xy(FastC fc, c
                 vyn = cur.vy + fc.qmdt * ey;
                 vzn = cur.vz;
                 const float dx = vxn * fc.cdx, dy = vyn * fc.cdy;
-                fast = fabsf(dx) < 1.0f && fabsf(dy) < 1.0f &&
-                       fmaxf(fabsf(vxn), fmaxf(fabsf(vyn), fabsf(vzn))) < fc.vfast;
+                const double d0 = (double)vxn - fc.c0, d1 = (double)vyn - fc.c1, d2 = (double)vzn - fc.c2;
+                const double dev = fmax(fabs(d0), fmax(fabs(d1), fabs(d2)));
+                // |d| < 0.99 keeps f + d strictly inside (-1, 2) after FP32 rounding
+                fast = fabsf(dx) < 0.99f && fabsf(dy) < 0.99f && dev < fc.vfast;
                 if (fast) {
                     const float xn = cur.fx + dx, yn = cur.fy + dy;
                     const float flx = floorf(xn), fly = floorf(yn);
@@ -663,17 +719,17 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_tile_fast_xy(FastC fc, c
                     key = pack_cell(nci, ncj);
                     // mover sums (KineticMaterial.java:247-259, 406-413)
                     sn++;
-                    sp0 += qbits(1.0, (double)vxn * fc.kv);
-                    sp1 += qbits(1.0, (double)vyn * fc.kv);
-                    sp2 += qbits(1.0, (double)vzn * fc.kv);
-                    se += qbits(1.0, (double)sqrtf(vxn * vxn + vyn * vyn + vzn * vzn) * fc.kv);
+                    sp0 += qbits(1.0, (double)vxn * fc.kst);
+                    sp1 += qbits(1.0, (double)vyn * fc.kst);
+                    sp2 += qbits(1.0, (double)vzn * fc.kst);
+                    se += qbits(1.0, (double)sqrtf(vxn * vxn + vyn * vyn + vzn * vzn) * fc.kst);
+                    dmax = fmax(dmax, dev);
                     // deposit of the 7 channels from the stored FP32 state (Field2D.scatter)
                     const double dfx = fxn, dfy = fyn;
                     const double w00 = (1.0 - dfx) * (1.0 - dfy), w10 = dfx * (1.0 - dfy), w11 = dfx * dfy,
                                  w01 = (1.0 - dfx) * dfy;
-                    const double v0 = vxn, v1 = vyn, v2 = vzn;
-                    const double val[NCH] = {fc.kn, v0 * fc.kv, v1 * fc.kv, v2 * fc.kv, v0 * v0 * fc.kvv,
-                                             v1 * v1 * fc.kvv, v2 * v2 * fc.kvv};
+                    const double val[NCH] = {fc.kn, d0 * fc.kv, d1 * fc.kv, d2 * fc.kv, d0 * d0 * fc.kvv,
+                                             d1 * d1 * fc.kvv, d2 * d2 * fc.kvv};
                     const int hn = nci - w_i0, hm = ncj - w_j0;
                     if ((unsigned)hn < (unsigned)TILE_CW && (unsigned)hm < (unsigned)TILE_CH) {
                         atomicAdd(&L.hist[hm * TILE_CW + hn], 1);
@@ -711,87 +767,139 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_tile_fast_xy(FastC fc, c
             }
             slow = !fast;
         }
-        // slow particles: the slot goes to the queue kernel, which writes it
+        // slow particles: the slot goes to the tile's slow list (stored in the tile's own slots
+        // of slow_list); push_tile_slow pushes and writes it
         {
             const unsigned long long mk = __ballot(slow);
             if (mk) {
                 const int lane = t & 63;
                 const int leader = __ffsll((long long)mk) - 1;
                 int base = 0;
-                if (lane == leader) base = atomicAdd(qcount, __popcll(mk));
+                if (lane == leader) base = atomicAdd(&L.nslow, __popcll(mk));
                 base = __shfl(base, leader);
-                if (slow) queue[base + __popcll(mk & ((1ull << lane) - 1ull))] = (int32_t)cur.g;
+                if (slow)
+                    slow_list[tile_slot(L.seg_start, L.seg_pre, base + __popcll(mk & ((1ull << lane) - 1ull)))] =
+                        (int32_t)cur.g;
             }
         }
-        if (ok) {
-            L.o_cell[pt] = key;
-            L.o_fx[pt] = fxn;
-            L.o_fy[pt] = fyn;
-            L.o_vx[pt] = vxn;
-            L.o_vy[pt] = vyn;
-            L.o_vz[pt] = vzn;
+        if (key != SLOT_KEEP) {
+            s.cell[cur.g] = key;
+            s.fx[cur.g] = fxn;
+            s.fy[cur.g] = fyn;
+            s.vx[cur.g] = vxn;
+            s.vy[cur.g] = vyn;
+            s.vz[cur.g] = vzn;
         }
-        lds_barrier();
-        const int wlen = min(PUSH_THREADS, rlen - cb);
-        if (t < wlen) {
-            const int32_t kc = L.o_cell[t];
-            if (kc != SLOT_KEEP) {
-                const int64_t g = rst + cb + t;
-                s.cell[g] = kc;
-                s.fx[g] = L.o_fx[t];
-                s.fy[g] = L.o_fy[t];
-                s.vx[g] = L.o_vx[t];
-                s.vy[g] = L.o_vy[t];
-                s.vz[g] = L.o_vz[t];
-            }
-        }
-        if (nr < TILE_H) {
-            if (nb + pt < row_len(nr)) {
-                cur.g = L.seg_start[nr] + nb + pt;
-                cur.c = s.cell[cur.g];
-                cur.fx = s.fx[cur.g];
-                cur.fy = s.fy[cur.g];
-                cur.vx = s.vx[cur.g];
-                cur.vy = s.vy[cur.g];
-                cur.vz = s.vz[cur.g];
-            } else {
-                cur.g = 0;
-                cur.c = -1;
-            }
-        }
-        lds_barrier();
-        cr = nr;
-        cb = nb;
     }
     __syncthreads();
-    // flush: node (a, b) of the window received one pattern per channel from every particle of
-    // window cells (a, b), (a-1, b), (a, b-1), (a-1, b-1)
-    for (int k = t; k < TILE_NODES; k += PUSH_THREADS) {
-        const int a = k % TILE_NW, b = k / TILE_NW;
-        const int i = w_i0 + a, j = w_j0 + b;
-        if (i < 0 || j < 0 || i >= fc.ni || j >= fc.nj) continue;
-        int np = 0;
-        if (a < TILE_CW && b < TILE_CH) np += L.hist[b * TILE_CW + a];
-        if (a > 0 && b < TILE_CH) np += L.hist[b * TILE_CW + a - 1];
-        if (a < TILE_CW && b > 0) np += L.hist[(b - 1) * TILE_CW + a];
-        if (a > 0 && b > 0) np += L.hist[(b - 1) * TILE_CW + a - 1];
-        if (np == 0) continue;
-        const unsigned long long bias = (unsigned long long)np * MAGIC_BITS;
-        const size_t u = (size_t)j * fc.ni + i;
-#pragma unroll
-        for (int ch = 0; ch < NCH; ch++) {
-            const unsigned long long v = L.acc[ch][k] - bias;
-            if (v != 0ull) atomicAdd(acc + ch * nn + u, v);
+    if (t == 0) {
+        // the slow list goes to push_tile_slow as work items of SLOW_CHUNK particles
+        const int n = L.nslow;
+        tile_nslow[tile] = n;
+        if (n) {
+            atomicAdd(&qcount[0], n);
+            const int nch = (n + SLOW_CHUNK - 1) / SLOW_CHUNK;
+            const int base = atomicAdd(&qcount[1], nch);
+            for (int c = 0; c < nch; c++) slow_items[base + c] = make_int2(tile, c);
         }
     }
-    for (int k = t; k < TILE_CELLS; k += PUSH_THREADS) {
-        const int h = L.hist[k];
-        if (h == 0) continue;
-        const int ci = w_i0 + k % TILE_CW, cj = w_j0 + k / TILE_CW;
-        atomicAdd(&counts[cj * fc.nci + ci], h);
-    }
+    flush_tile(L.acc, L.hist, w_i0, w_j0, fc.ni, fc.nj, fc.nci, acc, counts, PUSH_THREADS);
     const unsigned long long sb = (unsigned long long)sn * MAGIC_BITS;
-    StatAcc st{sn, (long long)(sp0 - sb), (long long)(sp1 - sb), (long long)(sp2 - sb), (long long)(se - sb)};
+    StatAcc st{sn, (long long)(sp0 - sb), (long long)(sp1 - sb), (long long)(sp2 - sb), (long long)(se - sb), dmax};
+    stat_flush(st, stats);
+}
+
+// The particles push_tile_fast_xy left to the reference loop (cells next to segments or mesh
+// faces, moves of 0.99 cell or more, deviations beyond the fast quantisation range): the full
+// FP64 push with ProcessBoundary and the deposit into the tile's LDS window (raw patterns and
+// histogram as in the fast kernel), into global memory for particles that land outside it, or
+// with the exact quantisation past the fast range.  The work is a list of (tile, chunk) items of
+// SLOW_CHUNK particles each, taken by the workgroups in turn, so a tile with a long slow list
+// (a row of cells along a mesh face, or the cells around an object) is spread over many of them.
+struct SlowLds {
+    unsigned long long acc[NCH][TILE_NODES];
+    int hist[TILE_CELLS];
+    int seg_start[TILE_H], seg_pre[TILE_H + 1];
+};
+
+__global__ void __launch_bounds__(SLOW_CHUNK) push_tile_slow(MeshP m, MeshD md, PushC pc, PState s,
+                                                             const int32_t* __restrict__ offsets,
+                                                             const int32_t* __restrict__ slow_list,
+                                                             const int32_t* __restrict__ tile_nslow,
+                                                             const int2* __restrict__ slow_items,
+                                                             const int* __restrict__ qcount,
+                                                             unsigned long long* __restrict__ acc,
+                                                             int32_t* __restrict__ counts, long long* stats,
+                                                             int* flags) {
+    __shared__ SlowLds L;
+    const int t = threadIdx.x;
+    const int nitems = qcount[1];
+    const size_t nn = (size_t)m.ni * m.nj;
+    auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
+    auto hasseg = [&](int i, int j) -> int { return md.node_hasseg[(size_t)j * m.ni + i]; };
+    StatAcc st{0, 0, 0, 0, 0, 0.0};
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int2 item = slow_items[it];
+        const int tile = item.x;
+        const int tx = tile % m.ntx, ty = tile / m.ntx;
+        const int ti0 = tx * TILE_W, tj0 = ty * TILE_H;
+        const int w_i0 = ti0 - TILE_HALO, w_j0 = tj0 - TILE_HALO;
+        for (int k = t; k < NCH * TILE_NODES; k += SLOW_CHUNK) (&L.acc[0][0])[k] = 0ull;
+        for (int k = t; k < TILE_CELLS; k += SLOW_CHUNK) L.hist[k] = 0;
+        if (t == 0) tile_segments(offsets, m.nci, m.ncj, ti0, tj0, L.seg_start, L.seg_pre);
+        __syncthreads();
+        const int k = item.y * SLOW_CHUNK + t;
+        if (k < tile_nslow[tile]) {
+            const int64_t g = slow_list[tile_slot(L.seg_start, L.seg_pre, k)];
+            PIn q = load_particle(s, g);
+            const bool alive = push_particle<false>(m, md, pc, load_e, hasseg, q.lc0, q.lc1, q.v0, q.v1, q.v2);
+            int32_t key = -1;
+            if (alive) {
+                int ci, cj;
+                float fxn, fyn;
+                cell_offsets(m, q.lc0, q.lc1, ci, cj, fxn, fyn);
+                key = pack_cell(ci, cj);
+                const float vxn = (float)q.v0, vyn = (float)q.v1, vzn = (float)q.v2;
+                stat_add(st, pc, vxn, vyn, vzn);
+                const double di = fxn, dj = fyn;
+                const int hn = ci - w_i0, hm = cj - w_j0;
+                bool counted = false;
+                if (!(ci + di >= m.ni - 1 || cj + dj >= m.nj - 1)) {
+                    double val[NCH];
+                    channel_values(pc, vxn, vyn, vzn, val);
+                    const double vm = vdev(pc, vxn, vyn, vzn);
+                    if (vm >= pc.vfast) {
+                        deposit_global_exact(m, acc, nn, ci, cj, di, dj, val);
+                        atomicOr(flags, vm >= pc.vover ? (PUSH_FLAG_FAST | PUSH_FLAG_OVERFLOW) : PUSH_FLAG_FAST);
+                    } else if ((unsigned)hn < (unsigned)TILE_CW && (unsigned)hm < (unsigned)TILE_CH) {
+                        double w4[4];
+                        scatter_weights(m, ci, cj, di, dj, w4);
+                        const int l0 = hm * TILE_NW + hn;
+                        const int off4[4] = {0, 1, TILE_NW + 1, TILE_NW};
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+#pragma unroll
+                            for (int ch = 0; ch < NCH; ch++)
+                                atomicAdd(&L.acc[ch][l0 + off4[c]], qbits(w4[c], val[ch]));
+                        atomicAdd(&L.hist[hm * TILE_CW + hn], 1);
+                        counted = true;
+                    } else {
+                        deposit_global(m, acc, nn, ci, cj, di, dj, val);
+                    }
+                }
+                if (!counted) atomicAdd(&counts[cj * m.nci + ci], 1);
+                s.fx[g] = fxn;
+                s.fy[g] = fyn;
+                s.vx[g] = vxn;
+                s.vy[g] = vyn;
+                s.vz[g] = vzn;
+            }
+            s.cell[g] = key;
+        }
+        __syncthreads();
+        flush_tile(L.acc, L.hist, w_i0, w_j0, m.ni, m.nj, m.nci, acc, counts, SLOW_CHUNK);
+        __syncthreads();   // the next item reuses the LDS window
+    }
     stat_flush(st, stats);
 }
 
@@ -973,11 +1081,14 @@ __global__ void __launch_bounds__(256) copy_scatter(MeshP m, PState s, PState d,
 // The running sums are FP64 in physical units (mpw-weighted): each step adds its exact integer
 // totals, converted once, so they are deterministic and cannot wrap however long sampling runs.
 // fs_n, fs_v, fs_vv are spwt / 2^k of the density, velocity and second-moment channels.
+// The channels hold W = sum w, D_k = sum w (v_k - c_k) and Q_k = sum w (v_k - c_k)^2 (unit
+// weight); the reference's sums are W, D_k + c_k W and Q_k + 2 c_k D_k + c_k^2 W, times mpw.
 __global__ void __launch_bounds__(256) normalize_nodes(int64_t nn, unsigned long long* __restrict__ acc,
                                                        double* __restrict__ sums, const double* __restrict__ vol,
                                                        double* __restrict__ den, double inv_kw, double inv_kv,
-                                                       double spwt, double nd_back, int samples, double fs_n,
-                                                       double fs_v, double fs_vv) {
+                                                       double inv_kvv, double c0, double c1, double c2, double spwt,
+                                                       double nd_back, int samples) {
+    const double c[3] = {c0, c1, c2};
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
         long long a[NCH];
 #pragma unroll
@@ -985,25 +1096,31 @@ __global__ void __launch_bounds__(256) normalize_nodes(int64_t nn, unsigned long
             a[ch] = (long long)acc[ch * nn + u];
             acc[ch * nn + u] = 0ull;
         }
-        if (samples) {
+        const double W = (double)a[0] * inv_kw;
+        double D[3], Q[3];
 #pragma unroll
-            for (int ch = 0; ch < NCH; ch++) {
-                const double f = ch == 0 ? fs_n : (ch < 4 ? fs_v : fs_vv);
-                sums[ch * nn + u] += (double)a[ch] * f;
+        for (int k = 0; k < 3; k++) {
+            D[k] = (double)a[1 + k] * inv_kv;
+            Q[k] = (double)a[4 + k] * inv_kvv;
+        }
+        if (samples) {
+            sums[u] += W * spwt;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                sums[(1 + k) * nn + u] += (D[k] + c[k] * W) * spwt;
+                sums[(4 + k) * nn + u] += (Q[k] + 2.0 * c[k] * D[k] + c[k] * c[k] * W) * spwt;
             }
         }
-        double wsum = (double)a[0] * inv_kw;
-        double nd = wsum * spwt / vol[u];
-        double uu = 0, vv = 0, ww = 0;
+        const double nd = W * spwt / vol[u];
+        double vel[3] = {0, 0, 0};
         if (a[0] != 0) {
-            uu = ((double)a[1] * inv_kv) / wsum;
-            vv = ((double)a[2] * inv_kv) / wsum;
-            ww = ((double)a[3] * inv_kv) / wsum;
+#pragma unroll
+            for (int k = 0; k < 3; k++) vel[k] = D[k] / W + c[k];
         }
         den[u] = nd + nd_back;
-        den[nn + u] = uu;
-        den[2 * nn + u] = vv;
-        den[3 * nn + u] = ww;
+        den[nn + u] = vel[0];
+        den[2 * nn + u] = vel[1];
+        den[3 * nn + u] = vel[2];
     }
 }
 
@@ -1050,6 +1167,32 @@ __global__ void __launch_bounds__(256) compute_fields_kernel(MeshP m, const doub
 // ----------------------------------------------------------------------------------------
 // particle addition (KineticMaterial.addParticle) and device preload
 // ----------------------------------------------------------------------------------------
+// KineticMaterial.addParticle of one accepted particle into slot p of the state (XtoL, clamp,
+// half-step rewind with the current E when `rewind`, FP32 storage).
+__device__ __forceinline__ void add_one(const MeshP& m, const MeshD& md, const PState& d, int64_t p, double x,
+                                        double y, double u, double v, double w, double qm, double qm_half_dt,
+                                        double half_dt, int rewind) {
+    double lc0 = (x - m.x0[0]) * m.inv_dh[0];
+    double lc1 = (y - m.x0[1]) * m.inv_dh[1];
+    if (rewind) {
+        if (lc0 >= m.ni) lc0 = m.ni - 1;
+        if (lc1 >= m.nj) lc1 = m.nj - 1;
+        // part.dt = -dt/2 (KineticMaterial.java:777-794)
+        auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
+        if (md.bf) kick<true>(m, md, qm, qm_half_dt, half_dt, load_e, lc0, lc1, u, v, w);
+        else kick<false>(m, md, qm, qm_half_dt, half_dt, load_e, lc0, lc1, u, v, w);
+    }
+    int ci = (int)lc0, cj = (int)lc1;
+    ci = min(max(ci, 0), m.nci - 1);
+    cj = min(max(cj, 0), m.ncj - 1);
+    d.cell[p] = pack_cell(ci, cj);
+    d.fx[p] = (float)(lc0 - ci);
+    d.fy[p] = (float)(lc1 - cj);
+    d.vx[p] = (float)u;
+    d.vy[p] = (float)v;
+    d.vz[p] = (float)w;
+}
+
 // `in` holds the columns x, y, u, v, w, `ld` doubles apart (ld = n for a packed [5][n] block).
 __global__ void __launch_bounds__(256) add_particles_kernel_strided(MeshP m, MeshD md, PState d, int64_t base,
                                                                     int64_t n, const double* __restrict__ in,
@@ -1058,27 +1201,9 @@ __global__ void __launch_bounds__(256) add_particles_kernel_strided(MeshP m, Mes
                                                                     const int32_t* __restrict__ ids, int id0,
                                                                     int with_ids) {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        double x = in[k], y = in[ld + k], u = in[2 * ld + k], v = in[3 * ld + k], w = in[4 * ld + k];
-        double lc0 = (x - m.x0[0]) * m.inv_dh[0];
-        double lc1 = (y - m.x0[1]) * m.inv_dh[1];
-        if (rewind) {
-            if (lc0 >= m.ni) lc0 = m.ni - 1;
-            if (lc1 >= m.nj) lc1 = m.nj - 1;
-            // part.dt = -dt/2 (KineticMaterial.java:777-794)
-            auto load_e = [&](int i, int j) -> float2 { return md.ef[(size_t)j * m.ni + i]; };
-            if (md.bf) kick<true>(m, md, qm, qm_half_dt, half_dt, load_e, lc0, lc1, u, v, w);
-            else kick<false>(m, md, qm, qm_half_dt, half_dt, load_e, lc0, lc1, u, v, w);
-        }
-        int ci = (int)lc0, cj = (int)lc1;
-        ci = min(max(ci, 0), m.nci - 1);
-        cj = min(max(cj, 0), m.ncj - 1);
-        int64_t p = base + k;
-        d.cell[p] = pack_cell(ci, cj);
-        d.fx[p] = (float)(lc0 - ci);
-        d.fy[p] = (float)(lc1 - cj);
-        d.vx[p] = (float)u;
-        d.vy[p] = (float)v;
-        d.vz[p] = (float)w;
+        const int64_t p = base + k;
+        add_one(m, md, d, p, in[k], in[ld + k], in[2 * ld + k], in[3 * ld + k], in[4 * ld + k], qm, qm_half_dt,
+                half_dt, rewind);
         if (with_ids) d.id[p] = ids ? ids[k] : id0 + (int)k;
     }
 }
@@ -1272,6 +1397,22 @@ static int grid_for(int64_t n, int threads, int max_blocks = 65536) {
     return (int)b;
 }
 
+// Channel scales of a step: a particle whose velocity components deviate from c by at most B
+// contributes at most 2^46 to a velocity channel (a corner weight is at most 1), so a node holds
+// 2^17 such contributions per step before its int64 total wraps (a contribution's typical size
+// is a few percent of that).  A temperature is <(v-c)^2> - <v-c>^2 over a node's weight W, and
+// the absolute quantum 2^-k of a contribution enters it as 2^-k / W: the scales are as fine as
+// that margin allows, since W can be a small fraction of one particle's weight.
+static void set_channel_scales(Species* sp, const double c[3], double B) {
+    B = std::max(B, 1.0);
+    const int kv = (int)std::floor(std::log2(std::ldexp(1.0, 46) / B));
+    const int kvv = (int)std::floor(std::log2(std::ldexp(1.0, 46) / (B * B)));
+    sp->kscale[0] = std::ldexp(1.0, 40);
+    for (int k = 1; k <= 3; k++) sp->kscale[k] = std::ldexp(1.0, kv);
+    for (int k = 4; k <= 6; k++) sp->kscale[k] = std::ldexp(1.0, kvv);
+    for (int k = 0; k < 3; k++) sp->vref[k] = c[k];
+}
+
 static PushC make_pushc(sf_ctx* ctx, Species* sp) {
     PushC pc;
     pc.qm_dt = sp->qm * ctx->dt;
@@ -1281,12 +1422,75 @@ static PushC make_pushc(sf_ctx* ctx, Species* sp) {
     pc.kv = sp->kscale[1];    // = kscale[2] = kscale[3]
     pc.kvv = sp->kscale[4];   // = kscale[5] = kscale[6]
     pc.spwt = sp->desc.spwt;
-    // A corner weight is at most 1, so a contribution is at most |v| kv or v^2 kvv: the one-FMA
-    // rounding holds below 2^51 (margin 2 kept), an int64 below 2^63 (margin 2); the mover sums
-    // round |v| kv <= sqrt(3) max|v_c| kv with the same FMA.
+    pc.c0 = sp->vref[0];
+    pc.c1 = sp->vref[1];
+    pc.c2 = sp->vref[2];
+    pc.kst = sp->kstat;
+    // A contribution is at most d kv or d^2 kvv for a deviation d: the one-FMA rounding holds
+    // below 2^51 and the fast path stops at 2^50 (a deviation of 16 B in the velocity channels,
+    // 4 B in the second moments); beyond 2^62 a contribution does not fit an int64 with margin.
+    // The mover sums round |v| kst <= sqrt(3) max|v_c| kst with the same FMA.
     pc.vfast = std::min(std::ldexp(1.0, 50) / pc.kv, std::sqrt(std::ldexp(1.0, 50) / pc.kvv));
-    pc.vover = std::min(std::sqrt(std::ldexp(1.0, 62) / pc.kvv), std::ldexp(1.0, 51) / (std::sqrt(3.0) * pc.kv));
+    pc.vover = std::min(std::sqrt(std::ldexp(1.0, 62) / pc.kvv), std::ldexp(1.0, 62) / pc.kv);
     return pc;
+}
+
+// Mover sums and largest deviation from c of the particles in the array (holes skipped): the
+// first push of a species (or the first after a load) takes its reference velocity and scales
+// from them.
+__global__ void __launch_bounds__(256) velocity_range_kernel(PState s, int64_t n, PushC pc, long long* stats) {
+    StatAcc st{0, 0, 0, 0, 0, 0.0};
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        if (s.cell[p] < 0) continue;
+        stat_add(st, pc, s.vx[p], s.vy[p], s.vz[p]);
+    }
+    stat_flush(st, stats);
+}
+
+// Adds the STAT_SLOTS copies of the mover sums into out[0..4] and returns the largest deviation.
+static double combine_stat_slots(const long long* slots, long long* out) {
+    for (int q = 0; q < 5; q++) out[q] = 0;
+    double dmax = 0;
+    for (int k = 0; k < STAT_SLOTS; k++) {
+        for (int q = 0; q < 5; q++) out[q] += slots[k * STAT_STRIDE + q];
+        double d;
+        std::memcpy(&d, &slots[k * STAT_STRIDE + 5], sizeof(double));
+        dmax = std::max(dmax, d);
+    }
+    return dmax;
+}
+
+// Reference velocity and channel scales of the coming push.  After a push they follow from its
+// mover sums (the mean velocity) and its largest deviation; otherwise two passes over the
+// particles measure them (mean, then deviation about the mean).
+static int prepare_channel_scales(sf_ctx* ctx, Species* sp) {
+    if (sp->vrange_valid) return SF_OK;
+    const double zero[3] = {0, 0, 0};
+    const double vmax = sp->desc.vmax > 0 ? sp->desc.vmax : 1e5;
+    set_channel_scales(sp, zero, vmax);
+    if (sp->np == 0) return SF_OK;
+    long long slots[STAT_SLOTS * STAT_STRIDE], st[5];
+    double c[3] = {0, 0, 0};
+    for (int pass = 0; pass < 2; pass++) {
+        PushC pc = make_pushc(ctx, sp);
+        SF_HIP(ctx, hipMemsetAsync(sp->stats, 0, sizeof(slots), ctx->stream));
+        hipLaunchKernelGGL(velocity_range_kernel, dim3(grid_for(sp->np, 256, 2048)), dim3(256), 0, ctx->stream,
+                           sp->buf[sp->cur], sp->np, pc, sp->stats);
+        SF_HIP(ctx, hipGetLastError());
+        SF_HIP(ctx, hipMemcpyAsync(slots, sp->stats, sizeof(slots), hipMemcpyDeviceToHost, ctx->stream));
+        SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const double dmax = combine_stat_slots(slots, st);
+        if (st[0] == 0) return SF_OK;
+        if (pass == 0) {
+            for (int k = 0; k < 3; k++) c[k] = (double)st[1 + k] / sp->kstat / (double)st[0];
+            set_channel_scales(sp, c, vmax);
+        } else {
+            sp->vbound = dmax;
+            set_channel_scales(sp, c, 1.25 * dmax);
+            sp->vrange_valid = true;
+        }
+    }
+    return SF_OK;
 }
 
 static int run_scan(sf_ctx* ctx, Species* sp, bool samples) {
@@ -1342,6 +1546,8 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
     const MeshP& m = ctx->mp;
     MeshD md = ctx->meshd();
+    const int rc_scales = prepare_channel_scales(ctx, sp);
+    if (rc_scales) return rc_scales;
     PushC pc = make_pushc(ctx, sp);
     PState s = sp->buf[sp->cur], d = sp->buf[1 - sp->cur];
     const int64_t nn = (int64_t)m.ni * m.nj;
@@ -1362,17 +1568,21 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     const dim3 g_tail(grid_for(sp->np - sp->nsorted, 256, 4096)), b256(256);
     // Every step pushes in place (a dead particle becomes a hole, cell = -1).  A sort step then
     // moves the pushed particles into cell order by their new cell, compacting the holes.
-    SF_HIP(ctx, hipMemsetAsync(sp->qcount, 0, sizeof(int), ctx->stream));
+    SF_HIP(ctx, hipMemsetAsync(sp->qcount, 0, 2 * sizeof(int), ctx->stream));
     if (sp->nsorted > 0) {
         if (fastk) {
             FastC fc;
             fc.qmdt = (float)pc.qm_dt;
             fc.cdx = (float)(pc.dt * m.inv_dh[0]);
             fc.cdy = (float)(pc.dt * m.inv_dh[1]);
-            fc.vfast = (float)pc.vfast;
+            fc.vfast = pc.vfast;
             fc.kn = pc.kn;
             fc.kv = pc.kv;
             fc.kvv = pc.kvv;
+            fc.kst = pc.kst;
+            fc.c0 = pc.c0;
+            fc.c1 = pc.c1;
+            fc.c2 = pc.c2;
             fc.ni = m.ni;
             fc.nj = m.nj;
             fc.nci = m.nci;
@@ -1380,7 +1590,7 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
             fc.ntx = m.ntx;
             hipLaunchKernelGGL(push_tile_fast_xy, tg, tb, 0, ctx->stream, fc, (const float2*)ctx->d_ef32,
                                (const uint8_t*)ctx->d_cellslow, s, (const int32_t*)sp->offsets, sp->acc, sp->counts,
-                               sp->stats, sp->key, sp->qcount);
+                               sp->stats, sp->key, sp->tile_nslow, sp->slow_items, sp->qcount);
         } else if (hb) {
             hipLaunchKernelGGL(push_deposit_tile_inplace_b, tg, tb, 0, ctx->stream, m, md, pc, s, sp->offsets,
                                sp->acc, sp->counts, sp->stats, sp->flags);
@@ -1390,13 +1600,11 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
         }
     }
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-    if (sp->nsorted > 0 && fastk) {
-        // the queue is at most the sorted particles; the grid strides over the count on the device
-        const int gq = (int)std::min<int64_t>(2048, std::max<int64_t>(1, sp->nsorted / 4096));
-        hipLaunchKernelGGL(push_deposit_queue, dim3(gq), b256, 0, ctx->stream, m, md, pc, s,
-                           (const int32_t*)sp->key, (const int*)sp->qcount, sp->acc, sp->counts, sp->stats,
+    if (sp->nsorted > 0 && fastk)   // the item count is on the device; idle workgroups exit
+        hipLaunchKernelGGL(push_tile_slow, dim3(2048), dim3(SLOW_CHUNK), 0, ctx->stream, m, md, pc, s,
+                           (const int32_t*)sp->offsets, (const int32_t*)sp->key, (const int32_t*)sp->tile_nslow,
+                           (const int2*)sp->slow_items, (const int*)sp->qcount, sp->acc, sp->counts, sp->stats,
                            sp->flags);
-    }
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
     if (sp->np > sp->nsorted) {
         if (hb)
@@ -1430,7 +1638,7 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     const double spwt = sp->desc.spwt;
     hipLaunchKernelGGL(normalize_nodes, dim3(grid_for(nn, 256, 8192)), dim3(256), 0, ctx->stream, nn, sp->acc,
                        sp->sums, (const double*)ctx->d_node_vol, sp->den, 1.0 / sp->kscale[0], 1.0 / sp->kscale[1],
-                       spwt, sp->desc.nd_back, 1, spwt / sp->kscale[0], spwt / sp->kscale[1], spwt / sp->kscale[4]);
+                       1.0 / sp->kscale[4], pc.c0, pc.c1, pc.c2, spwt, sp->desc.nd_back, 1);
     SF_HIP(ctx, hipGetLastError());
     if (ctx->timing) SF_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     // KineticMaterial.updateGasProperties (KineticMaterial.java:1553-1563): one more sample, and
@@ -1455,14 +1663,25 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     SF_HIP(ctx, hipMemcpyAsync(&flags, sp->flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     SF_HIP(ctx, hipMemcpyAsync(&queued, sp->qcount, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int k = 0; k < STAT_SLOTS; k++)
-        for (int q = 0; q < 5; q++) st[q] += slots[k * STAT_STRIDE + q];
-    double inv_kv = 1.0 / sp->kscale[1];
+    const double dmax = combine_stat_slots(slots, st);
+    const double inv_kst = 1.0 / sp->kstat;
     sp->stats_host[0] = (double)st[0] * sp->desc.spwt * sp->mass;
-    sp->stats_host[1] = (double)st[1] * inv_kv * sp->desc.spwt * sp->mass;
-    sp->stats_host[2] = (double)st[2] * inv_kv * sp->desc.spwt * sp->mass;
-    sp->stats_host[3] = (double)st[3] * inv_kv * sp->desc.spwt * sp->mass;
-    sp->stats_host[4] = (double)st[4] * inv_kv * sp->desc.spwt * sp->mass;
+    sp->stats_host[1] = (double)st[1] * inv_kst * sp->desc.spwt * sp->mass;
+    sp->stats_host[2] = (double)st[2] * inv_kst * sp->desc.spwt * sp->mass;
+    sp->stats_host[3] = (double)st[3] * inv_kst * sp->desc.spwt * sp->mass;
+    sp->stats_host[4] = (double)st[4] * inv_kst * sp->desc.spwt * sp->mass;
+    // the next push's reference velocity (the survivors' mean) and scales: the survivors deviate
+    // from the new c by at most dmax + |c_new - c|
+    if (st[0] > 0) {
+        double c[3], shift = 0;
+        for (int k = 0; k < 3; k++) {
+            c[k] = (double)st[1 + k] * inv_kst / (double)st[0];
+            shift = std::max(shift, std::fabs(c[k] - sp->vref[k]));
+        }
+        sp->vbound = dmax + shift;
+        set_channel_scales(sp, c, 1.25 * sp->vbound);
+        sp->vrange_valid = true;
+    }
     ctx->last_queue[1] = queued;
     if (ctx->timing) {
         float a, b, c, e, t0 = 0.f, t1 = 0.f;
@@ -1495,8 +1714,8 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     }
     if (flags & PUSH_FLAG_OVERFLOW)
         return sf_fail(ctx, SF_WARN_MOMENT_RANGE,
-                       "a particle speed beyond %.3g m/s overflowed the 64-bit moment channels of this step "
-                       "(raise the species' vmax)", pc.vover);
+                       "a particle velocity deviating by more than %.3g m/s from the species' mean overflowed "
+                       "the 64-bit moment channels of this step", pc.vover);
     return SF_OK;
 }
 
@@ -1549,6 +1768,7 @@ int add_particles_device(sf_ctx* ctx, Species* sp, int64_t n, const double* x, c
     if (d_ids) SF_HIP(ctx, hipFreeAsync(d_ids, ctx->stream));
     SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (!(ids && sp->desc.track_ids)) sp->id_counter += (int)kept;
+    if (kept * 8 >= sp->nalive) sp->vrange_valid = false;   // a bulk load: re-measure the velocities
     sp->np += kept;
     sp->nalive += kept;
     return SF_OK;
@@ -1592,6 +1812,7 @@ int preload(sf_ctx* ctx, Species* sp, int64_t n, const double* drift, double tem
     sp->id_counter += (int)n;
     sp->np += n;
     sp->nalive += n;
+    sp->vrange_valid = false;   // re-measure the velocities at the next push
     return SF_OK;
 }
 

@@ -181,10 +181,11 @@ def _lib_consts():
 
 
 def speed_hint(deck, spec):
-    """Fixed-point velocity scale of a species (sf_species_desc.vmax) from the deck: twice the
-    drift plus three widened thermal speeds of its sources and preload (Utils.SampleMaxw3D widens
-    by 1.2).  Faster particles stay exact up to about 45 times this speed; 0 keeps the library's
-    1e5 m/s when the deck names no population."""
+    """Speed hint of a species (sf_species_desc.vmax) from the deck: twice the drift plus three
+    widened thermal speeds of its sources and preload (Utils.SampleMaxw3D widens by 1.2).  It sets
+    the fixed-point scale of the mover sums (momentum and speed); the deposit channels take their
+    scales from the particles at every push.  0 keeps the library's 1e5 m/s when the deck names
+    no population."""
     import math
     from .deck import AMU, K_B
     mass = spec.molwt * AMU

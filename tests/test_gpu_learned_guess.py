@@ -8,7 +8,9 @@ from three starts: phi = 0 (plain), the previous step's phi (warm) and the netwo
 
 * every solve converges to the tolerance, whatever the start (plain PCG stays the judge);
 * the learned start needs at least 3x fewer PCG iterations than phi = 0 on the held-out states;
-* the learned solve ends on the same potential as the plain one (to the tolerance's scale);
+* the learned solve ends on the same potential as the plain one: both stop at a 1e-8 relative
+  nonlinear residual, which leaves up to ~2e-5 of the potential's scale between two solutions
+  (1e-4 is asserted);
 * a network with garbage weights still converges to the same potential, and policy 1 then keeps
   the previous potential as the start (it has the smaller residual).
 
@@ -107,7 +109,7 @@ def test_learned_guess_gains_over_plain_pcg_on_held_out_decks():
             phi_plain = _solved_phi(sim, "zero", net)
             phi_learn = _solved_phi(sim, "learned", net)
             scale = np.abs(phi_plain).max()
-            assert np.abs(phi_learn - phi_plain).max() <= 1e-5 * scale
+            assert np.abs(phi_learn - phi_plain).max() <= 1e-4 * scale
             # garbage weights: still converges to the same potential
             info_g, rc_g = _solve(sim, "learned", garbage)
             print("  garbage network: PCG %d, Newton %d, rel res %.1e" % (info_g.pcg_iterations,
@@ -115,7 +117,7 @@ def test_learned_guess_gains_over_plain_pcg_on_held_out_decks():
                                                                         info_g.rel_residual), flush=True)
             assert rc_g == 0 and info_g.rel_residual <= 1e-8
             phi_g = _solved_phi(sim, "learned", garbage)
-            assert np.abs(phi_g - phi_plain).max() <= 1e-5 * scale
+            assert np.abs(phi_g - phi_plain).max() <= 1e-4 * scale
             # policy 1 compares residuals and keeps the (warm) previous potential over garbage
             info_1, rc_1, used_guess = _solve(sim, "learned", garbage, policy=1)
             assert rc_1 == 0 and info_1.rel_residual <= 1e-8 and used_guess == 0
