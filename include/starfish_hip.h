@@ -163,6 +163,15 @@ int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species);
    ids may be NULL (then ids continue the species counter). */
 int sf_particles_add(sf_ctx* ctx, int species, int64_t n, const double* x, const double* y, const double* u,
                      const double* v, const double* w, const int* ids, int mode);
+/* No counterpart in the reference.  sf_particles_add for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+   the same arguments, acceptance rules, slots and ids, but the inputs are staged in a fixed
+   device array of the library (one per GPU, held until the process ends) instead of per-call
+   allocations, so at most SF_NOALLOC_MAX_ADD particles per call (SF_ERR_ARG beyond that).  Calls
+   that overlap on one GPU share that array and race. */
+#define SF_NOALLOC_MAX_ADD 65536
+int sf_particles_add_noalloc(sf_ctx* ctx, int species, int64_t n, const double* x, const double* y,
+                             const double* u, const double* v, const double* w, const int* ids, int mode);
 int sf_particles_count(sf_ctx* ctx, int species, int64_t* n);
 /* Downloads the sf_particles_count() live particles: physical position (x,y), logical coordinates
    (lc0, lc1), velocities, ids.  Any pointer may be NULL.  Order is the device order (cell order

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libstarfish_hip.so")
 SF_OK = 0
 SF_WARN_NOT_CONVERGED = 1
 SF_WARN_MOMENT_RANGE = 2
+SF_NOALLOC_MAX_ADD = 65536   # particles per sf_particles_add_noalloc call
 
 DOMAIN = {"XY": 0, "RZ": 1, "ZR": 2}
 FACE = {"OPEN": -1, "DIRICHLET": 0, "NEUMANN": 1, "PERIODIC": 2, "SYMMETRY": 3, "MESH": 4, "SINK": 5,
@@ -85,6 +86,7 @@ SIGNATURES = {
     "sf_mesh_setup": (C.c_int, [P, C.POINTER(MeshDesc), IP, DP, IP, DP, DP, C.c_int, DP, IP, IP, IP, IP]),
     "sf_species_create": (C.c_int, [P, C.POINTER(SpeciesDesc), IP]),
     "sf_particles_add": (C.c_int, [P, C.c_int, I64, DP, DP, DP, DP, DP, IP, C.c_int]),
+    "sf_particles_add_noalloc": (C.c_int, [P, C.c_int, I64, DP, DP, DP, DP, DP, IP, C.c_int]),
     "sf_particles_count": (C.c_int, [P, C.c_int, C.POINTER(I64)]),
     "sf_particles_download": (C.c_int, [P, C.c_int, DP, DP, DP, DP, DP, DP, DP, IP]),
     "sf_species_restart_size": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64)]),

@@ -110,6 +110,14 @@ int sf_ctx_create(int device, sf_ctx** out) {
     }
     for (auto& ev : c->ev) hipEventCreate(&ev);
     c->no_fast_push = getenv("SF_NO_FAST_PUSH") != nullptr;
+    // the fixed staging array of sf_particles_add_noalloc: its address on this device
+    const int rc = noalloc_scratch_init(c);
+    if (rc) {
+        for (auto& ev : c->ev) hipEventDestroy(ev);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return SF_OK;
 }
@@ -425,6 +433,17 @@ int sf_particles_add(sf_ctx* ctx, int species, int64_t n, const double* x, const
     return add_particles_device(ctx, sp, n, x, y, u, v, w, ids, mode);
 }
 
+int sf_particles_add_noalloc(sf_ctx* ctx, int species, int64_t n, const double* x, const double* y,
+                             const double* u, const double* v, const double* w, const int* ids, int mode) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp) return sf_fail(ctx, SF_ERR_ARG, "unknown species %d", species);
+    if (n < 0 || (n > 0 && (!x || !y || !u || !v || !w)))
+        return sf_fail(ctx, SF_ERR_ARG, "sf_particles_add_noalloc: bad arrays");
+    if (mode != 0 && mode != 1) return sf_fail(ctx, SF_ERR_ARG, "sf_particles_add_noalloc: mode must be 0 or 1");
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    return add_particles_noalloc(ctx, sp, n, x, y, u, v, w, ids, mode);
+}
 
 int sf_particles_count(sf_ctx* ctx, int species, int64_t* n) {
     SF_CHECK_CTX(ctx);

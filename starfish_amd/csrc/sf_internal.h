@@ -256,6 +256,9 @@ int push_deposit(sf_ctx* ctx, Species* sp);
 int sort_species(sf_ctx* ctx, Species* sp);
 int add_particles_device(sf_ctx* ctx, Species* sp, int64_t n, const double* x, const double* y, const double* u,
                          const double* v, const double* w, const int* ids, int mode);
+int noalloc_scratch_init(sf_ctx* ctx);
+int add_particles_noalloc(sf_ctx* ctx, Species* sp, int64_t n, const double* x, const double* y, const double* u,
+                          const double* v, const double* w, const int* ids, int mode);
 int species_field(sf_ctx* ctx, Species* sp, int which, double* out);
 int efield_update(sf_ctx* ctx);
 int ef32_refresh(sf_ctx* ctx);

@@ -14,6 +14,7 @@
 // Particles are read coalesced in cell-sorted order.  The deposit and the sums are 64-bit
 // fixed point, so every density and moment is independent of the order of the additions.
 #include <cstring>
+#include <mutex>
 
 #include "sf_particle_math.h"
 #include "sf_rng.h"
@@ -1208,6 +1209,63 @@ __global__ void __launch_bounds__(256) add_particles_kernel_strided(MeshP m, Mes
     }
 }
 
+// MeshData.addParticle's acceptance (KineticMaterial.java:1356-1361: finite velocity) and
+// DomainModule.getMesh's (DomainModule.java:106-117: inside the mesh, strictly or within
+// FLT_EPS in logical coordinates), the same tests as the host filter of add_particles_device.
+__device__ __forceinline__ bool accept_particle(const MeshP& m, double xd0, double xd1, double x, double y, double u,
+                                                double v, double w) {
+    if (!(isfinite(u) && isfinite(v) && isfinite(w))) return false;
+    const double lc0 = (x - m.x0[0]) / m.dh[0], lc1 = (y - m.x0[1]) / m.dh[1];
+    const bool strict = x >= m.x0[0] && x < xd0 && y >= m.x0[1] && y < xd1;
+    const bool loose = !(lc0 < -FLT_EPS || lc1 < -FLT_EPS || lc0 > (m.ni - 1 + FLT_EPS) || lc1 > (m.nj - 1 + FLT_EPS));
+    return strict || loose;
+}
+
+constexpr int FILTER_THREADS = 1024;
+
+// Acceptance and addParticle of n <= FILTER_THREADS * 64 particles in one workgroup.  Thread t
+// takes the contiguous chunk [t c, (t + 1) c) of the input; the workgroup's exclusive scan of the
+// accepted counts numbers every accepted particle by its rank among the accepted ones in input
+// order, as the host filter does, so slots and new ids are those add_particles_device gives.
+// `in` holds the columns x, y, u, v, w, `ld` doubles apart; the accepted count goes to *kept.
+__global__ void __launch_bounds__(FILTER_THREADS) add_particles_filtered(MeshP m, MeshD md, PState d, int64_t base,
+                                                                         int n, const double* __restrict__ in,
+                                                                         int64_t ld, double xd0, double xd1, double qm,
+                                                                         double qm_half_dt, double half_dt, int rewind,
+                                                                         const int32_t* __restrict__ ids, int id0,
+                                                                         int with_ids, int* __restrict__ kept) {
+    __shared__ int wave_tot[FILTER_THREADS / 64];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int c = (n + FILTER_THREADS - 1) / FILTER_THREADS;
+    const int k0 = min(n, t * c), k1 = min(n, k0 + c);
+    auto take = [&](int k) {
+        return accept_particle(m, xd0, xd1, in[k], in[ld + k], in[2 * ld + k], in[3 * ld + k], in[4 * ld + k]);
+    };
+    int cnt = 0;
+    for (int k = k0; k < k1; k++) cnt += take(k) ? 1 : 0;
+    // inclusive scan over the wave, then over the wave totals
+    int inc = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(inc, off);
+        if (lane >= off) inc += o;
+    }
+    if (lane == 63) wave_tot[wv] = inc;
+    __syncthreads();
+    int before = 0;
+    for (int q = 0; q < wv; q++) before += wave_tot[q];
+    int r = before + inc - cnt;   // accepted particles ahead of this chunk
+    for (int k = k0; k < k1; k++) {
+        if (!take(k)) continue;
+        const int64_t p = base + r;
+        add_one(m, md, d, p, in[k], in[ld + k], in[2 * ld + k], in[3 * ld + k], in[4 * ld + k], qm, qm_half_dt,
+                half_dt, rewind);
+        if (with_ids) d.id[p] = ids ? ids[k] : id0 + r;
+        r++;
+    }
+    if (t == FILTER_THREADS - 1) *kept = r;
+}
+
 // MaxwellianSource.sampleParticle + Source.sampleKinetic + KineticMaterial.addParticle for one
 // batch of a boundary source, sampled on the device (sources/MaxwellianSource.java:93-125,
 // core/source/Source.java:123-198, Spline.randomT for a polyline, Utils.SampleMaxw3D with its
@@ -1716,6 +1774,83 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
         return sf_fail(ctx, SF_WARN_MOMENT_RANGE,
                        "a particle velocity deviating by more than %.3g m/s from the species' mean overflowed "
                        "the 64-bit moment channels of this step", pc.vover);
+    return SF_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// sf_particles_add_noalloc: the staging of add_particles_device in a fixed device array
+// ----------------------------------------------------------------------------------------
+// Layout: the five FP64 input columns (x, y, u, v, w) SF_NOALLOC_MAX_ADD apart, the int32 ids,
+// then the accepted count.  Every GPU holds its own copy until the process ends; calls that
+// overlap on one GPU share it and race.
+constexpr size_t NOALLOC_COLS = (size_t)5 * SF_NOALLOC_MAX_ADD * sizeof(double);
+constexpr size_t NOALLOC_IDS = (size_t)SF_NOALLOC_MAX_ADD * sizeof(int32_t);
+constexpr size_t NOALLOC_BYTES = NOALLOC_COLS + NOALLOC_IDS + 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+static_assert(SF_NOALLOC_MAX_ADD <= FILTER_THREADS * 64, "the filter kernel takes 64 particles per thread at most");
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of this GPU's copy of `scratch`, looked up once per device with the device
+// current (the array's name is a host-side stand-in and must not reach a kernel).
+int noalloc_scratch_init(sf_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES)
+        return sf_fail(ctx, SF_ERR_ARG, "device %d: the fixed staging array supports devices 0-%d", ctx->device,
+                       NOALLOC_MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[ctx->device]) return SF_OK;
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    void* p = nullptr;
+    SF_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return sf_fail(ctx, SF_ERR_HIP, "hipGetSymbolAddress returned no address for the staging array");
+    g_scratch_ptr[ctx->device] = p;
+    return SF_OK;
+}
+
+int add_particles_noalloc(sf_ctx* ctx, Species* sp, int64_t n, const double* x, const double* y, const double* u,
+                          const double* v, const double* w, const int* ids, int mode) {
+    if (n <= 0) return SF_OK;
+    if (n > SF_NOALLOC_MAX_ADD)
+        return sf_fail(ctx, SF_ERR_ARG,
+                       "sf_particles_add_noalloc: %lld particles exceed the %d of its fixed staging array "
+                       "(split the call, or use sf_particles_add)", (long long)n, SF_NOALLOC_MAX_ADD);
+    if (sp->np + n > sp->capacity)
+        return sf_fail(ctx, SF_ERR_OOM, "species capacity %lld exceeded (have %lld, adding %lld)",
+                       (long long)sp->capacity, (long long)sp->np, (long long)n);
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = g_scratch_ptr[ctx->device];
+    }
+    if (!base) return sf_fail(ctx, SF_ERR_STATE, "sf_particles_add_noalloc: staging array not set up on device %d",
+                              ctx->device);
+    double* d_cols = (double*)base;
+    int32_t* d_ids = (int32_t*)((char*)base + NOALLOC_COLS);
+    int* d_kept = (int*)((char*)base + NOALLOC_COLS + NOALLOC_IDS);
+    const double* src[5] = {x, y, u, v, w};
+    for (int c = 0; c < 5; c++)
+        SF_HIP(ctx, hipMemcpyAsync(d_cols + (size_t)c * SF_NOALLOC_MAX_ADD, src[c], n * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    const bool with_ids = ids && sp->desc.track_ids;
+    if (with_ids)
+        SF_HIP(ctx, hipMemcpyAsync(d_ids, ids, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    const MeshP& m = ctx->mp;
+    const double xd0 = m.x0[0] + (m.ni - 1) * m.dh[0], xd1 = m.x0[1] + (m.nj - 1) * m.dh[1];
+    hipLaunchKernelGGL(add_particles_filtered, dim3(1), dim3(FILTER_THREADS), 0, ctx->stream, ctx->mp, ctx->meshd(),
+                       sp->buf[sp->cur], sp->np, (int)n, (const double*)d_cols, (int64_t)SF_NOALLOC_MAX_ADD, xd0, xd1,
+                       sp->qm, -0.5 * ctx->dt * sp->qm, -0.5 * ctx->dt, mode,
+                       with_ids ? (const int32_t*)d_ids : (const int32_t*)nullptr, sp->id_counter,
+                       sp->desc.track_ids, d_kept);
+    SF_HIP(ctx, hipGetLastError());
+    int kept = 0;
+    SF_HIP(ctx, hipMemcpyAsync(&kept, d_kept, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!with_ids) sp->id_counter += kept;
+    if ((int64_t)kept * 8 >= sp->nalive) sp->vrange_valid = false;   // a bulk load: re-measure the velocities
+    sp->np += kept;
+    sp->nalive += kept;
     return SF_OK;
 }
 
