@@ -337,7 +337,7 @@ def main():
     kern = np.zeros(4)
     pushed = 0
     # per-kernel totals: (ms, particles, launches): the tiled push (the particles it pushed itself),
-    # the queue of particles it left to the full reference loop, and the sort-step scatter
+    # the slow lists it left to the full reference loop (push_tile_slow), and the sort-step scatter
     k_push, k_queue, k_scatter = [0.0, 0, 0], [0.0, 0, 0], [0.0, 0, 0]
     solve_ms, pcg_its, newton_its = [], [], []
     worst_res = 0.0
@@ -396,10 +396,10 @@ def main():
                 "algorithmic_bytes_per_launch": round(BYTES_PUSH * per_launch),
                 "ms_per_launch": round(k_push[0] / max(1, k_push[2]), 4), "launches": k_push[2],
                 "per_kernel_GBs": {"push_tile_fast_xy": round(achieved, 1),
-                                   "push_deposit_queue": round(rate(k_queue, BYTES_PUSH), 1),
+                                   "push_tile_slow": round(rate(k_queue, BYTES_PUSH), 1),
                                    "copy_scatter": round(rate(k_scatter, BYTES_SCATTER), 1)},
-                "queued_per_step": round(k_queue[1] / max(1, k_queue[2])),
-                "queue_ms_per_step": round(k_queue[0] / max(1, k_queue[2]), 4),
+                "slow_particles_per_step": round(k_queue[1] / max(1, k_queue[2])),
+                "slow_ms_per_step": round(k_queue[0] / max(1, k_queue[2]), 4),
                 "particle_phase_state_GBs": round(BYTES_STATE * pushed / particle_s / 1e9, 1)}
     cpu = None
     if not args.no_cpu_baseline:
@@ -434,7 +434,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "full_step_particles_per_s": pushed / wall,
-        "particle_phase_ms": {"push_tiles": k_push[0] / args.steps, "push_queue": k_queue[0] / args.steps,
+        "particle_phase_ms": {"push_tiles": k_push[0] / args.steps, "push_slow": k_queue[0] / args.steps,
                               "push_appended": tail_ms / args.steps, "scan": kern[1] / args.steps,
                               "scatter": kern[2] / args.steps, "normalize": kern[3] / args.steps},
         "pcg": {"ms_per_solve": float(np.mean(solve_ms)), "pcg_iterations_per_solve": float(np.mean(pcg_its)),

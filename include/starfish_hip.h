@@ -318,9 +318,10 @@ int sf_last_timings(sf_ctx* ctx, double* out4);
  * second kernel covered. */
 int sf_last_push_timings(sf_ctx* ctx, double* out3);
 /* The XY tile kernel pushes in FP32 every particle whose cell is away from segments and mesh faces
- * and queues the others for the full reference loop: out2[0] the queue kernel's ms (with
- * sf_set_timing), out2[1] how many particles the last push queued.  The environment variable
- * SF_NO_FAST_PUSH (at context creation) makes every step run the full loop in the tile kernel. */
+ * and leaves the others to per-tile slow lists that a second kernel pushes with the full reference
+ * loop: out2[0] that kernel's ms (with sf_set_timing), out2[1] how many particles the last push
+ * left to it.  The environment variable SF_NO_FAST_PUSH (at context creation) makes every step
+ * run the full loop in the general tile kernel. */
 int sf_last_push_queue(sf_ctx* ctx, double* out2);
 int sf_set_timing(sf_ctx* ctx, int enabled);
 

@@ -28,6 +28,15 @@ namespace sf {
 enum { NB_SELF = 0, NB_FREE = 1, NB_FIXED = 2 };
 constexpr uint8_t CLS_NOT_FREE = 0xFF;
 
+constexpr int NK_RING = 8;
+struct NkState {
+    double tol2;   // PCG tolerance (squared) of the current linear solve
+    double r2;     // ||r||^2 of the Newton residual at the current linear solve's start (at stop: final)
+    int k;         // PCG iteration within the current linear solve
+    int newton;    // Newton updates so far
+    int pcg;       // PCG iterations so far
+    int done;      // 0 running, 1 converged, 2 Newton iteration cap, 3 residual not finite
+};
 struct PoissonState {
     sf_poisson_desc desc{};
     double eps = EPS0, C = 0;     // C = e/eps
@@ -52,6 +61,10 @@ struct PoissonState {
     int* istate = nullptr;        // [0] done (1 converged, 2 not finite), [1] iterations at done
     int nblk_tile = 0, nblk_elem = 0;
     int last_its = 7;             // PCG iterations of the last linear solve (sizes the next batch)
+    int last_pairs = 24;          // step pairs of the last device-driven solve (sizes its first batch)
+    bool host_newton = false;     // SF_HOST_NEWTON: keep every solve on the host-driven loop
+    NkState* nk = nullptr;        // [NK_RING] device-driven Newton-PCG state ring
+    NkState* host_nk = nullptr;   // pinned copy of the last state read
     double* host_scal = nullptr;  // pinned
     int* host_istate = nullptr;
     hipEvent_t e0{}, e1{};
@@ -130,6 +143,8 @@ void poisson_free(PoissonState* p) {
         if (q) hipFree(q);
     if (p->host_scal) hipHostFree(p->host_scal);
     if (p->host_istate) hipHostFree(p->host_istate);
+    if (p->host_nk) hipHostFree(p->host_nk);
+    if (p->nk) hipFree(p->nk);
     if (p->e0) hipEventDestroy(p->e0);
     if (p->e1) hipEventDestroy(p->e1);
     delete p;
@@ -495,6 +510,258 @@ __global__ void __launch_bounds__(256) pcg_update(MeshP m, const uint8_t* __rest
     block_partials<2>(v, pz_next);
 }
 
+// ------------------------------------------------------------------------------------------
+// Device-driven Newton-PCG (warm solves).  The host queues pairs of kernels (nk_step_a,
+// nk_step_b) without reading anything back in between; each pair is one PCG iteration or one
+// Newton update, decided by every block from the same partial sums and the state the previous
+// pair left in a small ring (state[i % NK_RING] is written by block 0 of pair i's second
+// kernel and read by pair i + 1).  Pair i keeps its (r.z, r.r) partials in pz[i & 1] and its
+// search direction in p[i & 1], so no kernel reads a buffer a block of the same kernel writes.
+//   A (tile kernel): at the start of a linear solve (k = 0) the partials hold ||r||^2 of the
+//     Newton residual: below (rel_tol ||r_ref||)^2 the solve is converged; otherwise the PCG
+//     tolerance is set and the first direction computed.  Later: if the PCG residual is below
+//     the tolerance (or k reached max_it), x += limited d (the Newton update, Solver.java:366-368
+//     with the step limit of newton_update); else p = z + beta p_old, q = J p, p.q partials.
+//   B (tile kernel): after a Newton update, the FP64 residual r = f + bfix - L x - W b(x), the
+//     Jacobian diagonal, d = 0, res = r, z = M^-1 r and the partials of iteration 0 (r.r in
+//     FP64); else the PCG update d += alpha p, res -= alpha q, z = M^-1 res and its partials.
+// ------------------------------------------------------------------------------------------
+struct NkParams {
+    double conv2;   // (rel_tol ||r_ref||)^2
+    double eta2;    // squared forcing term (max(eta, 1e-5))^2
+    double floor2;  // (0.5 rel_tol ||r_ref||)^2
+    double lim;     // Newton step limit (kTe)
+    int max_it, nl_max;
+};
+
+enum { NK_PCG = 0, NK_UPDATE = 1, NK_STOP = 2 };
+// The decision of pair i from the previous state and the partials of the previous pair.
+__device__ __forceinline__ int nk_decide(const NkState& S, const NkParams& np, const double cur[2], int& code) {
+    code = 0;
+    if (S.done) return NK_STOP;
+    if (S.k == 0) {
+        if (!(cur[1] == cur[1]) || cur[1] > 1e300) { code = 3; return NK_STOP; }
+        if (cur[1] <= np.conv2) { code = 1; return NK_STOP; }
+        if (S.newton >= np.nl_max) { code = 2; return NK_STOP; }
+        return NK_PCG;
+    }
+    if (!(cur[1] > S.tol2) || S.k >= np.max_it) return NK_UPDATE;
+    return NK_PCG;
+}
+
+__global__ void __launch_bounds__(256) nk_step_a(MeshP m, const uint8_t* __restrict__ cls,
+                                                 const float* __restrict__ D, const float* __restrict__ z,
+                                                 float* __restrict__ p_even, float* __restrict__ p_odd,
+                                                 float* __restrict__ q, double* __restrict__ x,
+                                                 const float* __restrict__ d, const double* __restrict__ pz_even,
+                                                 const double* __restrict__ pz_odd, int nbz,
+                                                 double* __restrict__ ppq, const NkState* __restrict__ state,
+                                                 NkParams np, int i) {
+    const NkState S = state[(i - 1) % NK_RING];
+    if (S.done) return;
+    double cur[2];
+    combine_partials<2>((i - 1) & 1 ? pz_odd : pz_even, nbz, cur);
+    int code;
+    const int act = nk_decide(S, np, cur, code);
+    if (act == NK_STOP) return;
+    const int nbx = (m.ni + PT_W - 1) / PT_W;
+    const int bx = blockIdx.x % nbx, by = blockIdx.x / nbx;
+    if (act == NK_UPDATE) {
+        // x += limited d on the tile's free nodes
+        const int li = threadIdx.x % PT_W;
+        for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += 256 / PT_W) {
+            const int ii = bx * PT_W + li, jj = by * PT_H + lj;
+            if (ii >= m.ni || jj >= m.nj) continue;
+            const size_t u = (size_t)jj * m.ni + ii;
+            if (cls[u] == CLS_NOT_FREE) continue;
+            double v = (double)d[u];
+            if (np.lim > 0) {
+                if (v > np.lim) v = np.lim * (1.0 + log(v / np.lim));
+                v = fmax(v, -10.0 * np.lim);
+            }
+            x[u] += v;
+        }
+        return;
+    }
+    float beta = 0.0f;
+    if (S.k > 0) {
+        double prev[2];
+        combine_partials<2>(i & 1 ? pz_odd : pz_even, nbz, prev);   // pair i-2's partials
+        beta = (float)(cur[0] / prev[0]);
+    }
+    const float* pold = (i - 1) & 1 ? p_odd : p_even;
+    float* pnew = i & 1 ? p_odd : p_even;
+    __shared__ float ps[(PT_H + 2) * (PT_W + 2)];
+    const int i0 = bx * PT_W - 1, j0 = by * PT_H - 1;
+    for (int t = threadIdx.x; t < (PT_H + 2) * (PT_W + 2); t += blockDim.x) {
+        const int ii = i0 + t % (PT_W + 2), jj = j0 + t / (PT_W + 2);
+        float v = 0.0f;
+        if (ii >= 0 && jj >= 0 && ii < m.ni && jj < m.nj) {
+            const size_t u = (size_t)jj * m.ni + ii;
+            v = S.k == 0 ? z[u] : z[u] + beta * pold[u];
+        }
+        ps[t] = v;
+    }
+    __syncthreads();
+    const int li = threadIdx.x % PT_W;
+    double acc = 0;
+    for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += 256 / PT_W) {
+        const int ii = i0 + 1 + li, jj = j0 + 1 + lj;
+        if (ii >= m.ni || jj >= m.nj) continue;
+        const size_t u = (size_t)jj * m.ni + ii;
+        const uint8_t c = cls[u];
+        const int l = (lj + 1) * (PT_W + 2) + (li + 1);
+        const float pv = ps[l];
+        float qv = 0.0f;
+        if (c != CLS_NOT_FREE) {
+            const Stencil st = stencil(m, ii, jj);
+            qv = D[u] * pv;
+            if (nbc(c, 0) == NB_FREE) qv -= (float)st.cE * ps[l + 1];
+            if (nbc(c, 1) == NB_FREE) qv -= (float)st.cW * ps[l - 1];
+            if (nbc(c, 2) == NB_FREE) qv -= (float)st.cN * ps[l + PT_W + 2];
+            if (nbc(c, 3) == NB_FREE) qv -= (float)st.cS * ps[l - PT_W - 2];
+        }
+        pnew[u] = pv;
+        q[u] = qv;
+        acc += (double)pv * (double)qv;
+    }
+    double v[1] = {acc};
+    block_partials<1>(v, ppq);
+}
+
+// init = 1: the residual branch unconditionally (the state of pair 0, written to state[0]).
+__global__ void __launch_bounds__(256) nk_step_b(MeshP m, const uint8_t* __restrict__ cls,
+                                                 const double* __restrict__ W, const double* __restrict__ bfix,
+                                                 const double* __restrict__ rho, const double* __restrict__ x,
+                                                 double* __restrict__ r, float* __restrict__ D,
+                                                 double* __restrict__ P, float* __restrict__ d,
+                                                 float* __restrict__ res, float* __restrict__ z,
+                                                 const float* __restrict__ p_even, const float* __restrict__ p_odd,
+                                                 const float* __restrict__ q, const double* __restrict__ ppq, int nbq,
+                                                 double* __restrict__ pz_even, double* __restrict__ pz_odd, int nbz,
+                                                 NkState* __restrict__ state, NkParams np, double inv_eps, double Cn0,
+                                                 double phi0, double inv_kTe, int i, int init) {
+    NkState S{0.0, 0.0, 0, 0, 0, 0};
+    int act = NK_UPDATE;   // init: compute the residual
+    double cur[2] = {0.0, 0.0};
+    if (!init) {
+        S = state[(i - 1) % NK_RING];
+        if (!S.done) combine_partials<2>((i - 1) & 1 ? pz_odd : pz_even, nbz, cur);
+        int code;
+        act = nk_decide(S, np, cur, code);
+        if (act == NK_STOP) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                NkState o = S;
+                if (!S.done) {
+                    o.done = code;
+                    o.r2 = cur[1];
+                }
+                state[i % NK_RING] = o;
+            }
+            return;
+        }
+    }
+    float alpha = 0.0f;
+    if (act == NK_PCG) {
+        double pq[1];
+        combine_partials<1>(ppq, nbq, pq);
+        alpha = (float)(cur[0] / pq[0]);
+    }
+    const float* p = i & 1 ? p_odd : p_even;
+    __shared__ float rs[PT_H * PT_W], zs[PT_H * PT_W];
+    const int nbx = (m.ni + PT_W - 1) / PT_W;
+    const int i0 = (blockIdx.x % nbx) * PT_W, j0 = (blockIdx.x / nbx) * PT_H;
+    const int li = threadIdx.x % PT_W;
+    double rr64[4] = {0.0, 0.0, 0.0, 0.0};
+    float rv[4];
+    if (act == NK_UPDATE) {
+        // FP64 Newton residual and Jacobian diagonal of the tile (the newton_residual kernel's rows)
+#pragma unroll
+        for (int s4 = 0; s4 < 4; s4++) {
+            const int lj = threadIdx.x / PT_W + 4 * s4;
+            const int ii = i0 + li, jj = j0 + lj;
+            rv[s4] = 0.0f;
+            if (ii < m.ni && jj < m.nj) {
+                const size_t u = (size_t)jj * m.ni + ii;
+                const uint8_t c = cls[u];
+                d[u] = 0.0f;
+                if (c == CLS_NOT_FREE) {
+                    r[u] = 0;
+                    D[u] = 1.0f;
+                    P[u] = 0;
+                    res[u] = 0.0f;
+                } else {
+                    double diagL;
+                    const double Lx = lred_apply64(m, c, ii, jj, u, x, diagL);
+                    const double Wu = W[u];
+                    const double bx = Cn0 * exp((x[u] - phi0) * inv_kTe);
+                    const double bp = bx * inv_kTe;
+                    const double ru = Wu * rho[u] * inv_eps + bfix[u] - Lx - Wu * bx;
+                    r[u] = ru;
+                    P[u] = Wu * bp;
+                    D[u] = (float)(diagL + Wu * bp);
+                    rv[s4] = (float)ru;
+                    res[u] = rv[s4];
+                    rr64[s4] = ru;
+                }
+            }
+            rs[lj * PT_W + li] = rv[s4];
+        }
+        __syncthreads();   // the tile's D values, read by tile_rows below
+    }
+    TileRows T;
+    tile_rows(m, cls, D, i0, j0, T);
+    if (act == NK_PCG) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; s4++) {
+            const int lj = threadIdx.x / PT_W + 4 * s4;
+            rv[s4] = 0.0f;
+            if (T.in[s4]) {
+                const size_t u = (size_t)(j0 + lj) * m.ni + i0 + li;
+                const float pv = p[u], qv = q[u];
+                d[u] = d[u] + alpha * pv;
+                rv[s4] = res[u] - alpha * qv;
+                res[u] = rv[s4];
+                if (T.c[s4] == CLS_NOT_FREE) rv[s4] = 0.0f;
+            }
+            rs[lj * PT_W + li] = rv[s4];
+        }
+    }
+    tile_sgs(T, i0, j0, rs, zs);
+    double a_rz = 0, a_rr = 0;
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+        const int lj = threadIdx.x / PT_W + 4 * s4;
+        if (!T.in[s4]) continue;
+        const float zv = zs[lj * PT_W + li];
+        z[(size_t)(j0 + lj) * m.ni + i0 + li] = zv;
+        if (act == NK_UPDATE) {
+            a_rz += rr64[s4] * (double)zv;
+            a_rr += rr64[s4] * rr64[s4];
+        } else {
+            a_rz += (double)rv[s4] * (double)zv;
+            a_rr += (double)rv[s4] * (double)rv[s4];
+        }
+    }
+    double v[2] = {a_rz, a_rr};
+    block_partials<2>(v, i & 1 ? pz_odd : pz_even);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        NkState o = S;
+        if (act == NK_UPDATE) {
+            if (!init) o.newton = S.newton + 1;
+            o.k = 0;
+        } else {
+            if (S.k == 0) {   // the linear solve's tolerance from its first residual
+                o.r2 = cur[1];
+                o.tol2 = fmax(np.eta2 * cur[1], np.floor2);
+            }
+            o.k = S.k + 1;
+            o.pcg = S.pcg + 1;
+        }
+        state[i % NK_RING] = o;
+    }
+}
+
 // y += d (FP64 accumulation of the FP32 correction)
 __global__ void accumulate_y(int64_t nn, const float* __restrict__ d, double* __restrict__ y,
                              const int* __restrict__ go) {
@@ -725,10 +992,14 @@ int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* dsc) {
         if (!e) e = A((void**)&P->istate, 16);
         if (!e) e = hipHostMalloc((void**)&P->host_scal, S_NSCAL * 8 * 2);
         if (!e) e = hipHostMalloc((void**)&P->host_istate, 16);
+        if (!e) e = A((void**)&P->nk, NK_RING * sizeof(NkState));
+        if (!e) e = hipHostMalloc((void**)&P->host_nk, sizeof(NkState));
         if (!e) e = hipEventCreate(&P->e0);
         if (!e) e = hipEventCreate(&P->e1);
         if (e) return sf_fail(ctx, SF_ERR_OOM, "poisson allocation failed: %s", hipGetErrorString(e));
         hipMemset(P->istate, 0, 16);
+        hipMemset(P->nk, 0, NK_RING * sizeof(NkState));
+        P->host_newton = getenv("SF_HOST_NEWTON") != nullptr;
         hipMemset(P->pz, 0, (size_t)4 * std::max(P->nblk_elem, nb_tile) * 8);
         hipMemset(P->ppq, 0, (size_t)P->nblk_tile * 8);
         hipMemset(P->y, 0, nn * 8);
@@ -847,6 +1118,49 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         if (!eta_chosen) {
             eta = rel < 0.5 ? 1e-2 : 1e-3;
             eta_chosen = true;
+        }
+        // A warm nonlinear solve (from near its solution) runs as device-driven Newton-PCG: the
+        // pairs of step kernels are queued in batches and the host reads one state word per batch.
+        if (it == 0 && rel < 0.5 && !D.linear && !(nl_tol > 0) && !debug && !P->host_newton) {
+            const double rt = rel_tol * ref;
+            const double ee = std::max(eta, 1e-5);
+            NkParams np{rt * rt, ee * ee, 0.25 * rt * rt, lim, max_it, nl_max};
+            auto step_b = [&](int i, int init) {
+                hipLaunchKernelGGL(nk_step_b, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+                                   (const double*)P->W, (const double*)P->bfix, (const double*)ctx->d_rho,
+                                   (const double*)P->x, P->r, P->D, P->P, P->d, P->res, P->z, (const float*)P->p0,
+                                   (const float*)P->p1, (const float*)P->q, (const double*)P->ppq, gt, pz0, pz1, gt,
+                                   P->nk, np, inv_eps, Cn0, D.phi0, inv_kTe, i, init);
+            };
+            step_b(0, 1);
+            int i = 1, batch = std::min(64, std::max(8, P->last_pairs + 2));
+            while (true) {
+                for (int b = 0; b < batch; b++, i++) {
+                    hipLaunchKernelGGL(nk_step_a, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+                                       (const float*)P->D, (const float*)P->z, P->p0, P->p1, P->q, P->x,
+                                       (const float*)P->d, (const double*)pz0, (const double*)pz1, gt, P->ppq,
+                                       (const NkState*)P->nk, np, i);
+                    step_b(i, 0);
+                }
+                SF_HIP(ctx, hipGetLastError());
+                SF_HIP(ctx, hipMemcpyAsync(P->host_nk, P->nk + (i - 1) % NK_RING, sizeof(NkState),
+                                           hipMemcpyDeviceToHost, st));
+                SF_HIP(ctx, hipStreamSynchronize(st));
+                if (P->host_nk->done) break;
+                if (i > 4 * (max_it + 1) * (nl_max + 1))
+                    return sf_fail(ctx, SF_ERR_STATE, "device Newton-PCG did not stop");
+                batch = std::min(2 * batch, 256);
+            }
+            const NkState fin = *P->host_nk;
+            P->last_pairs = fin.newton + fin.pcg + 1;
+            newton = fin.newton;
+            pcg_total = fin.pcg;
+            rnorm = std::sqrt(fin.r2);
+            rel = rnorm / ref;
+            if (fin.done == 3 || !std::isfinite(rel))
+                return sf_fail(ctx, SF_ERR_STATE, "Newton residual is not finite (diverged)");
+            converged = fin.done == 1;
+            break;
         }
         newton++;
         // inexact Newton: solve J y = r to max(eta*||r||, 0.5*tol*ref) with FP32 PCG + FP64 refinement
