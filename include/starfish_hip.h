@@ -120,6 +120,8 @@ int sf_geometry_build(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_typ
 int sf_geometry_sizes(const sf_geometry* g, int* nseg, int* nlist);
 int sf_geometry_export(const sf_geometry* g, int* node_type, double* node_bc, double* node_vol, double* seg_xy,
                        int* seg_btype, int* seg_absorb, int* node_seg_ptr, int* node_seg_idx);
+/* seg_bnd: [nseg] index of the boundary each segment belongs to (Segment.getBoundary) */
+int sf_geometry_segment_boundaries(const sf_geometry* g, int* seg_bnd);
 int sf_geometry_free(sf_geometry* g);
 
 /* ------------------------------------------------------------------------------------------
@@ -133,6 +135,12 @@ int sf_mesh_setup(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_type, c
                   const int* node_type, const double* node_bc, const double* node_vol, int nseg,
                   const double* seg_xy, const int* seg_btype, const int* seg_absorb, const int* node_seg_ptr,
                   const int* node_seg_idx);
+/* Boundary index of each uploaded segment (seg_bnd [nseg], -1 = not tallied; the order of
+   Starfish.getBoundaryList()).  From then on every species counts, per boundary, the particles
+   a surface absorbs (Material.addSurfaceMassDeposit, core/materials/Material.java:784-795),
+   which StatsModule divides by boundary area and time for its flux columns
+   (Material.getFluxInst, Material.java:806-813; core/diagnostics/StatsModule.java:141-147). */
+int sf_mesh_segment_boundaries(sf_ctx* ctx, int nseg, const int* seg_bnd, int nbnd);
 
 /* ------------------------------------------------------------------------------------------
  * Kinetic species (KineticMaterial, core/materials/KineticMaterial.java; Material keys
@@ -223,6 +231,9 @@ int sf_species_clear_samples(sf_ctx* ctx, int species);
 /* mover sums of the last push, multiplied by mass: mass, momentum[3], "energy" = mass*sum(mpw*|v|)
    (KineticMaterial.java:247-259, 406-413) */
 int sf_species_stats(sf_ctx* ctx, int species, double* out5);
+/* counts[nbnd]: particles of the species absorbed on each boundary since the last call with
+   clear != 0 (deprate_collection_inst; multiply by spwt for Material.getFluxInst's sum) */
+int sf_species_boundary_hits(sf_ctx* ctx, int species, int nbnd, int64_t* counts, int clear);
 
 /* ------------------------------------------------------------------------------------------
  * Field solve: PoissonSolver (pic/PoissonSolver.java) with SolverModule's keys

@@ -21,7 +21,7 @@ class MeshDesc(C.Structure):
                 ("dh", C.c_double * 2), ("face_type", IP), ("face_value", DP), ("node_type", IP),
                 ("node_bc", DP), ("node_vol", DP), ("nseg", C.c_int), ("seg_xy", DP), ("seg_btype", IP),
                 ("seg_absorb", IP), ("node_seg_ptr", IP), ("node_seg_idx", IP), ("seed", C.c_int64),
-                ("nthreads", C.c_int), ("dt", C.c_double)]
+                ("nthreads", C.c_int), ("dt", C.c_double), ("seg_bnd", IP), ("nbnd", C.c_int)]
 
 
 def build():
@@ -44,6 +44,7 @@ def load():
         "orc_create": (P, [C.POINTER(MeshDesc)]),
         "orc_destroy": (None, [P]),
         "orc_add_species": (C.c_int, [P, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "orc_boundary_hits": (C.c_int, [P, C.c_int, C.POINTER(C.c_int64), C.c_int, C.c_int]),
         "orc_set_solver": (None, [P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                   C.c_double, C.c_int, C.c_double]),
         "orc_add_source": (C.c_int, [P, C.c_int, C.c_int, DP, C.c_int, C.c_double, C.c_double, C.c_double]),
@@ -105,7 +106,7 @@ class Oracle:
     """One oracle simulation over a mesh geometry (arrays as produced by Starfish's Mesh)."""
 
     def __init__(self, mesh, face_type, face_value, node_type, node_bc, node_vol, seg_xy, seg_btype, seg_absorb,
-                 node_seg_ptr, node_seg_idx, nseg, seed=0, nthreads=1, dt=1e-7):
+                 node_seg_ptr, node_seg_idx, nseg, seed=0, nthreads=1, dt=1e-7, seg_bnd=None, nbnd=0):
         self.lib = load()
         dom = {"XY": 0, "RZ": 1, "ZR": 2}[mesh.domain.upper()]
         self._keep = [np.ascontiguousarray(a) for a in (face_type, face_value, node_type, node_bc, node_vol, seg_xy,
@@ -116,6 +117,11 @@ class Oracle:
         d = MeshDesc(dom, mesh.ni, mesh.nj, (C.c_double * 2)(*mesh.x0), (C.c_double * 2)(*mesh.dh),
                      _ip(k[0].astype(np.int32)), _dp(k[1]), _ip(k[2]), _dp(k[3]), _dp(k[4]), nseg, _dp(k[5]),
                      _ip(k[6]), _ip(k[7]), _ip(k[8]), _ip(k[9]), seed, nthreads, dt)
+        if seg_bnd is not None:
+            self._seg_bnd = np.ascontiguousarray(seg_bnd, dtype=np.int32)
+            d.seg_bnd = _ip(self._seg_bnd)
+            d.nbnd = int(nbnd)
+        self.nbnd = int(nbnd) if seg_bnd is not None else 0
         self._ft = k[0].astype(np.int32)
         d.face_type = _ip(self._ft)
         self.h = self.lib.orc_create(C.byref(d))
@@ -123,7 +129,8 @@ class Oracle:
     @classmethod
     def from_geometry(cls, geo, seed=0, nthreads=1, dt=1e-7):
         return cls(geo.mesh, geo.face_type, geo.face_value, geo.node_type, geo.node_bc, geo.node_vol, geo.seg_xy,
-                   geo.seg_btype, geo.seg_absorb, geo.node_seg_ptr, geo.node_seg_idx, geo.nseg, seed, nthreads, dt)
+                   geo.seg_btype, geo.seg_absorb, geo.node_seg_ptr, geo.node_seg_idx, geo.nseg, seed, nthreads, dt,
+                   getattr(geo, "seg_bnd", None), getattr(geo, "nbnd", 0))
 
     def close(self):
         if self.h:
@@ -135,6 +142,13 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+    def boundary_hits(self, sp, clear=False):
+        """particles of species sp absorbed on each boundary since the last clear"""
+        out = np.zeros(max(1, self.nbnd), dtype=np.int64)
+        assert self.lib.orc_boundary_hits(self.h, sp, out.ctypes.data_as(C.POINTER(C.c_int64)), self.nbnd,
+                                          1 if clear else 0) == 0
+        return out[:self.nbnd]
 
     def add_species(self, charge, molwt, spwt, nd_back=0.0):
         return self.lib.orc_add_species(self.h, charge, molwt, spwt, nd_back)

@@ -90,6 +90,7 @@ struct Segment {
     double length;
     int btype;
     int absorb;
+    int bnd = -1;        /* index of the parent boundary */
     void init(const double* a, const double* b) {
         x1[0] = a[0]; x1[1] = a[1]; x2[0] = b[0]; x2[1] = b[1];
         double dx = x2[0] - x1[0], dy = x2[1] - x1[1];
@@ -367,6 +368,8 @@ double vdot(const std::vector<double>& a, const std::vector<double>& b) {
 struct Species {
     double charge, mass, spwt, qm, nd_back;
     std::vector<Particle> parts;
+    std::unique_ptr<std::atomic<long long>[]> bnd_hits;   /* absorbed particles per boundary (deprate_inst) */
+    int nbnd = 0;
     int id_counter = 0;
     Field den, U, V, W;
     Field count_sum, u_sum, v_sum, w_sum, uu_sum, vv_sum, ww_sum, mpc_sum;
@@ -405,6 +408,7 @@ struct orc_sim {
     Mesh mesh;
     JavaRandom rnd{0};
     int nthreads = 1;
+    int nbnd = 0;            /* boundaries of the impact tallies */
     double dt = 1e-7;
     int it = 1;
     double time = 0;
@@ -786,6 +790,9 @@ struct orc_sim {
             if (sg.absorb) alive = false;              /* Material.performSurfaceInteraction default: absorb */
             if (!alive) add_boundary_charge(part.mpw * sp.charge);
             if (sg.btype == BT_SINK) alive = false;
+            /* addSurfaceMassDeposit on the boundary hit (KineticMaterial.java:596-600); counted in
+               particles, the caller multiplies by spwt */
+            if (!alive && sg.bnd >= 0 && sg.bnd < sp.nbnd) sp.bnd_hits[sg.bnd].fetch_add(1);
             if (!alive) return alive;
         }
         if (part.lc[0] < 0 || part.lc[1] < 0 || part.lc[0] >= ni - 1 || part.lc[1] >= nj - 1) {
@@ -1172,11 +1179,13 @@ orc_sim* orc_create(const orc_mesh_desc* d) {
         m.seg[q].init(d->seg_xy + 4 * q, d->seg_xy + 4 * q + 2);
         m.seg[q].btype = d->seg_btype[q];
         m.seg[q].absorb = d->seg_absorb[q];
+        m.seg[q].bnd = d->seg_bnd ? d->seg_bnd[q] : -1;
     }
     m.nseg_ptr.assign(d->node_seg_ptr, d->node_seg_ptr + nn + 1);
     m.nseg_idx.assign(d->node_seg_idx, d->node_seg_idx + m.nseg_ptr[nn]);
     s->rnd.set_seed(d->seed);
     s->nthreads = d->nthreads > 0 ? d->nthreads : 1;
+    s->nbnd = d->seg_bnd ? std::max(0, d->nbnd) : 0;
     s->dt = d->dt;
     s->phi.init(m.ni, m.nj);
     s->rho.init(m.ni, m.nj);
@@ -1188,6 +1197,16 @@ orc_sim* orc_create(const orc_mesh_desc* d) {
 }
 
 void orc_destroy(orc_sim* s) { delete s; }
+
+int orc_boundary_hits(orc_sim* s, int sp, int64_t* out, int nbnd, int clear) {
+    if (sp < 0 || sp >= (int)s->species.size() || nbnd != s->nbnd) return -1;
+    Species& S = *s->species[sp];
+    for (int b = 0; b < nbnd; b++) {
+        out[b] = S.bnd_hits[b].load();
+        if (clear) S.bnd_hits[b] = 0;
+    }
+    return 0;
+}
 
 int orc_add_species(orc_sim* s, double charge_e, double molwt, double spwt, double nd_back) {
     auto sp = std::make_unique<Species>();
@@ -1202,6 +1221,9 @@ int orc_add_species(orc_sim* s, double charge_e, double molwt, double spwt, doub
                      &sp->w_ave, &sp->T, &sp->T1, &sp->T2, &sp->T3, &sp->p, &sp->mpc})
         f->init(ni, nj);
     for (double& a : sp->den.d) a += nd_back;    /* Material.init adds nd_back */
+    sp->nbnd = s->nbnd;
+    sp->bnd_hits.reset(new std::atomic<long long>[std::max(1, s->nbnd)]);
+    for (int b = 0; b < std::max(1, s->nbnd); b++) sp->bnd_hits[b] = 0;
     s->species.push_back(std::move(sp));
     return (int)s->species.size() - 1;
 }

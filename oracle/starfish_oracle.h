@@ -41,10 +41,15 @@ typedef struct {
     int64_t seed;            /* java.util.Random seed of the shared RNG */
     int nthreads;            /* mover threads (max_cores) */
     double dt;               /* time step */
+    const int* seg_bnd;      /* [nseg] boundary index of each segment (Segment.getBoundary), may be NULL */
+    int nbnd;                /* number of boundaries */
 } orc_mesh_desc;
 
 orc_sim* orc_create(const orc_mesh_desc* desc);
 void orc_destroy(orc_sim* s);
+/* particles of species sp absorbed on each boundary since the last call with clear != 0
+   (Material.addSurfaceMassDeposit / getFluxInst, core/materials/Material.java:784-813) */
+int orc_boundary_hits(orc_sim* s, int sp, int64_t* out, int nbnd, int clear);
 
 int orc_add_species(orc_sim* s, double charge_e, double molwt, double spwt, double nd_back);
 void orc_set_solver(orc_sim* s, int method /*0 GS, 1 PCG*/, int linear, double n0, double Te0, double phi0,

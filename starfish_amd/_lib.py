@@ -82,6 +82,8 @@ SIGNATURES = {
                                     C.POINTER(P)]),
     "sf_geometry_sizes": (C.c_int, [P, IP, IP]),
     "sf_geometry_export": (C.c_int, [P, IP, DP, DP, DP, IP, IP, IP, IP]),
+    "sf_geometry_segment_boundaries": (C.c_int, [P, IP]),
+    "sf_mesh_segment_boundaries": (C.c_int, [P, C.c_int, IP, C.c_int]),
     "sf_geometry_free": (C.c_int, [P]),
     "sf_mesh_setup": (C.c_int, [P, C.POINTER(MeshDesc), IP, DP, IP, DP, DP, C.c_int, DP, IP, IP, IP, IP]),
     "sf_species_create": (C.c_int, [P, C.POINTER(SpeciesDesc), IP]),
@@ -98,6 +100,7 @@ SIGNATURES = {
     "sf_species_field": (C.c_int, [P, C.c_int, C.c_int, DP]),
     "sf_species_clear_samples": (C.c_int, [P, C.c_int]),
     "sf_species_stats": (C.c_int, [P, C.c_int, DP]),
+    "sf_species_boundary_hits": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int]),
     "sf_poisson_setup": (C.c_int, [P, C.POINTER(PoissonDesc)]),
     "sf_rho_update": (C.c_int, [P]),
     "sf_rho_set": (C.c_int, [P, DP]),

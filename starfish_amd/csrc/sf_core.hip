@@ -141,6 +141,7 @@ static void free_species(Species* sp) {
     dfree(sp->qcount);
     dfree(sp->tile_nslow);
     dfree(sp->slow_items);
+    dfree(sp->bnd_hits);
     dfree(sp->den);
     dfree(sp->counts);
     dfree(sp->offsets);
@@ -294,11 +295,14 @@ int sf_mesh_setup(sf_ctx* ctx, const sf_mesh_desc* mesh, const int* face_type, c
         g.nrm[1] = dx;
         g.btype = seg_btype[s];
         g.absorb = seg_absorb[s];
+        g.bnd = -1;
     }
     TRY(dalloc(ctx, &ctx->d_face, nface));
     TRY(dalloc(ctx, &ctx->d_nseg_ptr, nn + 1));
     TRY(dalloc(ctx, &ctx->d_nseg_idx, idx.size()));
     TRY(dalloc(ctx, &ctx->d_seg, (size_t)nseg));
+    ctx->nseg = nseg;
+    ctx->nbnd = 0;
     TRY(dalloc(ctx, &ctx->d_hasseg, nn));
     // Cells whose particles can take the push's fast path.  The fast path moves a particle by
     // less than 0.99 cell per axis, so from cell (i, j) (lc in [i, i+1)) the new lc lies in
@@ -389,6 +393,10 @@ int sf_species_create(sf_ctx* ctx, const sf_species_desc* desc, int* species) {
     if (!rc) rc = dalloc(ctx, &sp->tile_nslow, ntiles);
     // a tile of n slow particles makes ceil(n / 256) items: at most cap / 256 + tiles in all
     if (!rc) rc = dalloc(ctx, &sp->slow_items, cap / 256 + ntiles + 1);
+    if (!rc && ctx->nbnd > 0) {
+        rc = dalloc(ctx, &sp->bnd_hits, (size_t)ctx->nbnd);
+        if (!rc) SF_HIP(ctx, hipMemset(sp->bnd_hits, 0, ctx->nbnd * sizeof(unsigned long long)));
+    }
     if (!rc) rc = dalloc(ctx, &sp->den, 4 * nn);
     if (!rc) rc = dalloc(ctx, &sp->counts, ncells);
     if (!rc) rc = dalloc(ctx, &sp->offsets, ncells + 1);
@@ -644,6 +652,49 @@ int sf_species_clear_samples(sf_ctx* ctx, int species) {
     SF_HIP(ctx, hipMemsetAsync(sp->mpc_sum, 0, ncells * sizeof(long long), ctx->stream));
     SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     sp->num_samples = 0;
+    return SF_OK;
+}
+
+int sf_mesh_segment_boundaries(sf_ctx* ctx, int nseg, const int* seg_bnd, int nbnd) {
+    SF_CHECK_CTX(ctx);
+    if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "sf_mesh_segment_boundaries: mesh not set up");
+    if (nseg != ctx->nseg || nbnd < 0 || (nseg > 0 && !seg_bnd))
+        return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_segment_boundaries: %d segments given, the mesh has %d", nseg,
+                       ctx->nseg);
+    for (int s = 0; s < nseg; s++)
+        if (seg_bnd[s] < -1 || seg_bnd[s] >= nbnd)
+            return sf_fail(ctx, SF_ERR_ARG, "sf_mesh_segment_boundaries: segment %d names boundary %d of %d", s,
+                           seg_bnd[s], nbnd);
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<DevSeg> segs(nseg);
+    if (nseg > 0) {
+        SF_HIP(ctx, hipMemcpy(segs.data(), ctx->d_seg, nseg * sizeof(DevSeg), hipMemcpyDeviceToHost));
+        for (int s = 0; s < nseg; s++) segs[s].bnd = seg_bnd[s];
+        SF_HIP(ctx, hipMemcpy(ctx->d_seg, segs.data(), nseg * sizeof(DevSeg), hipMemcpyHostToDevice));
+    }
+    ctx->nbnd = nbnd;
+    for (Species* sp : ctx->species) {
+        dfree(sp->bnd_hits);
+        if (nbnd > 0) {
+            TRY(dalloc(ctx, &sp->bnd_hits, (size_t)nbnd));
+            SF_HIP(ctx, hipMemset(sp->bnd_hits, 0, nbnd * sizeof(unsigned long long)));
+        }
+    }
+    return SF_OK;
+}
+
+int sf_species_boundary_hits(sf_ctx* ctx, int species, int nbnd, int64_t* counts, int clear) {
+    SF_CHECK_CTX(ctx);
+    Species* sp = get_species(ctx, species);
+    if (!sp || nbnd != ctx->nbnd || (nbnd > 0 && !counts))
+        return sf_fail(ctx, SF_ERR_ARG, "sf_species_boundary_hits: bad argument (the mesh tallies %d boundaries)",
+                       ctx->nbnd);
+    if (nbnd == 0) return SF_OK;
+    SF_HIP(ctx, hipSetDevice(ctx->device));
+    SF_HIP(ctx, hipMemcpyAsync(counts, sp->bnd_hits, nbnd * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (clear) SF_HIP(ctx, hipMemsetAsync(sp->bnd_hits, 0, nbnd * sizeof(unsigned long long), ctx->stream));
+    SF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return SF_OK;
 }
 

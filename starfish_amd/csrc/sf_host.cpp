@@ -414,6 +414,12 @@ int sf_geometry_sizes(const sf_geometry* G, int* nseg, int* nlist) {
     return SF_OK;
 }
 
+int sf_geometry_segment_boundaries(const sf_geometry* G, int* seg_bnd) {
+    if (!G || !seg_bnd) return SF_ERR_ARG;
+    for (size_t s = 0; s < G->g.segs.size(); s++) seg_bnd[s] = G->g.segs[s].bnd;
+    return SF_OK;
+}
+
 int sf_geometry_export(const sf_geometry* G, int* node_type, double* node_bc, double* node_vol, double* seg_xy,
                        int* seg_btype, int* seg_absorb, int* node_seg_ptr, int* node_seg_idx) {
     if (!G) return SF_ERR_ARG;

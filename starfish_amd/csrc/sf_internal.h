@@ -75,6 +75,7 @@ struct DevSeg {
     double nrm[2];
     int btype;
     int absorb;
+    int bnd;       // boundary index for the impact tallies (-1: not tallied)
 };
 
 // Device pointers of the mesh.
@@ -86,6 +87,7 @@ struct MeshD {
     const uint8_t* node_hasseg;// [ni*nj] 1 if the node lists a DIRICHLET/SINK segment
     const float2* ef;          // [ni*nj] E field (i, j components)
     const float2* bf;          // [ni*nj] B field (i, j components); null when no field is loaded
+    unsigned long long* bnd_hits;   // [nbnd] particles of the pushed species absorbed per boundary (or null)
 };
 
 // Per-species push constants.
@@ -144,6 +146,7 @@ struct Species {
     double vbound = 0;
     bool vrange_valid = false;
     double stats_host[5] = {0, 0, 0, 0, 0};
+    unsigned long long* bnd_hits = nullptr;   // [ctx nbnd] particles absorbed per boundary since the last read
 };
 
 // Host-side java.util.Random (JDK-specified 48-bit LCG), the shared Starfish generator
@@ -219,6 +222,8 @@ struct sf_ctx {
     double last_push_ms[3] = {0, 0, 0};   // tile kernel ms, tail kernel ms, tail slots
     double last_queue[2] = {0, 0};        // queue kernel ms, particles the fast tile kernel queued
     bool no_fast_push = false;            // SF_NO_FAST_PUSH set: every step runs the full tile kernel
+    int nbnd = 0;                         // boundaries of the impact tallies (sf_mesh_segment_boundaries)
+    int nseg = 0;                         // segments uploaded by sf_mesh_setup
     sf::MeshD meshd() const {
         sf::MeshD m;
         m.face = d_face;
@@ -228,6 +233,7 @@ struct sf_ctx {
         m.node_hasseg = d_hasseg;
         m.ef = d_ef32;
         m.bf = d_bf32;
+        m.bnd_hits = nullptr;
         return m;
     }
 };

@@ -131,8 +131,13 @@ __device__ __forceinline__ bool process_boundary(const MeshP& m, const MeshD& md
             if (lc0 < 0 && lc0 > -FLT_EPS) lc0 = 0;
             if (lc1 < 0 && lc1 > -FLT_EPS) lc1 = 0;
             const DevSeg sg = md.seg[seg_min];
-            // default surface interaction of a boundary with a material absorbs; SINK always does
-            if (sg.absorb || sg.btype == SF_BND_SINK) return false;
+            // default surface interaction of a boundary with a material absorbs; SINK always does.
+            // An absorbed particle is tallied on its boundary (Material.addSurfaceMassDeposit,
+            // Material.java:784-795, feeding StatsModule's flux columns through getFluxInst).
+            if (sg.absorb || sg.btype == SF_BND_SINK) {
+                if (md.bnd_hits != nullptr && sg.bnd >= 0) atomicAdd(md.bnd_hits + sg.bnd, 1ull);
+                return false;
+            }
         }
     }
     if (lc0 < 0 || lc1 < 0 || lc0 >= ni - 1 || lc1 >= nj - 1) {

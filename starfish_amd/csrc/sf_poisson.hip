@@ -63,6 +63,7 @@ struct PoissonState {
     int last_its = 7;             // PCG iterations of the last linear solve (sizes the next batch)
     int last_pairs = 24;          // step pairs of the last device-driven solve (sizes its first batch)
     bool host_newton = false;     // SF_HOST_NEWTON: keep every solve on the host-driven loop
+    bool warm_hint = false;       // the last solve converged (the next one likely starts warm)
     NkState* nk = nullptr;        // [NK_RING] device-driven Newton-PCG state ring
     NkState* host_nk = nullptr;   // pinned copy of the last state read
     double* host_scal = nullptr;  // pinned
@@ -527,22 +528,32 @@ __global__ void __launch_bounds__(256) pcg_update(MeshP m, const uint8_t* __rest
 //     FP64); else the PCG update d += alpha p, res -= alpha q, z = M^-1 res and its partials.
 // ------------------------------------------------------------------------------------------
 struct NkParams {
-    double conv2;   // (rel_tol ||r_ref||)^2
-    double eta2;    // squared forcing term (max(eta, 1e-5))^2
-    double floor2;  // (0.5 rel_tol ||r_ref||)^2
-    double lim;     // Newton step limit (kTe)
+    const double* ref2;   // ||r_ref||^2 on the device (the finalize of ref_norm_kernel)
+    double rel2;          // rel_tol^2
+    double eta2;          // squared forcing term (max(eta, 1e-5))^2
+    double lim;           // Newton step limit (kTe)
     int max_it, nl_max;
 };
 
 enum { NK_PCG = 0, NK_UPDATE = 1, NK_STOP = 2 };
-// The decision of pair i from the previous state and the partials of the previous pair.
+enum { NK_CONVERGED = 1, NK_CAP = 2, NK_NOT_FINITE = 3, NK_COLD = 4 };
+// ||r_ref||^2 as the host path uses it (1 when the reference residual vanishes)
+__device__ __forceinline__ double nk_ref2(const NkParams& np) {
+    const double r = *np.ref2;
+    return r > 0 ? r : 1.0;
+}
+// The decision of pair i from the previous state and the partials of the previous pair.  A solve
+// whose first residual is above half the reference residual is not warm: it stops with NK_COLD
+// before any update, and the host runs it on its own loop instead.
 __device__ __forceinline__ int nk_decide(const NkState& S, const NkParams& np, const double cur[2], int& code) {
     code = 0;
     if (S.done) return NK_STOP;
     if (S.k == 0) {
-        if (!(cur[1] == cur[1]) || cur[1] > 1e300) { code = 3; return NK_STOP; }
-        if (cur[1] <= np.conv2) { code = 1; return NK_STOP; }
-        if (S.newton >= np.nl_max) { code = 2; return NK_STOP; }
+        const double R = nk_ref2(np);
+        if (!(cur[1] == cur[1]) || cur[1] > 1e300) { code = NK_NOT_FINITE; return NK_STOP; }
+        if (cur[1] <= np.rel2 * R) { code = NK_CONVERGED; return NK_STOP; }
+        if (S.newton == 0 && cur[1] > 0.25 * R) { code = NK_COLD; return NK_STOP; }
+        if (S.newton >= np.nl_max) { code = NK_CAP; return NK_STOP; }
         return NK_PCG;
     }
     if (!(cur[1] > S.tol2) || S.k >= np.max_it) return NK_UPDATE;
@@ -753,7 +764,7 @@ __global__ void __launch_bounds__(256) nk_step_b(MeshP m, const uint8_t* __restr
         } else {
             if (S.k == 0) {   // the linear solve's tolerance from its first residual
                 o.r2 = cur[1];
-                o.tol2 = fmax(np.eta2 * cur[1], np.floor2);
+                o.tol2 = fmax(np.eta2 * cur[1], 0.25 * np.rel2 * nk_ref2(np));
             }
             o.k = S.k + 1;
             o.pcg = S.pcg + 1;
@@ -811,7 +822,7 @@ __global__ void expand_phi(int64_t nn, const int32_t* __restrict__ src, const do
                            const double* __restrict__ x, double* __restrict__ phi) {
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
         int32_t s = src[u];
-        phi[u] = s < 0 ? fixval[u] : x[s];
+        phi[u] = s < 0 ? fixval[u] : x[s] + fixval[u];
     }
 }
 __global__ void gather_free(int64_t nn, const uint8_t* __restrict__ cls, const double* __restrict__ phi,
@@ -922,6 +933,23 @@ int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* dsc) {
         }
         return -2;
     };
+    // A one-sided face row reads (phi_in - phi_f)/dh = b_f (left, bottom) or (phi_f - phi_in)/dh = b_f
+    // (right, top), so phi_f = phi_in + off_f.  In linear mode b_f is the node's bc_value
+    // (PoissonSolver.update, PoissonSolver.java:140-153); in nonlinear mode Solver.solveNonLinearNR
+    // zeroes b on these nodes (Solver.java:325-342) and every tie is exact (off = 0).  fixval
+    // holds the value of fixed nodes and the offset of tied ones.
+    const bool lin = dsc->linear != 0;
+    auto step_off = [&](size_t v) -> double {
+        if (!lin) return 0.0;
+        const double b = ctx->h_node_bc[v];
+        switch (rt[v]) {
+            case 1: return -m.dh[0] * b;
+            case 2: return m.dh[0] * b;
+            case 3: return -m.dh[1] * b;
+            case 4: return m.dh[1] * b;
+        }
+        return 0.0;
+    };
     std::vector<double> fixval(nn, 0.0);
     for (size_t u = 0; u < nn; u++)
         if (rt[u] == 0) fixval[u] = ctx->h_node_bc[u];
@@ -929,10 +957,14 @@ int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* dsc) {
         if (rt[u] == 0 || rt[u] == 5) continue;
         size_t v = u;
         int guard = 0;
-        while (rt[v] >= 1 && rt[v] <= 4 && guard++ < 8) v = (size_t)tie(v);
+        double off = 0.0;
+        while (rt[v] >= 1 && rt[v] <= 4 && guard++ < 8) {
+            off += step_off(v);
+            v = (size_t)tie(v);
+        }
         if (rt[v] >= 1 && rt[v] <= 4) return sf_fail(ctx, SF_ERR_UNSUPPORTED, "cyclic face rows at node %zu", u);
-        if (rt[v] == 0) { src[u] = -1; fixval[u] = ctx->h_node_bc[v]; }
-        else src[u] = (int32_t)v;
+        if (rt[v] == 0) { src[u] = -1; fixval[u] = ctx->h_node_bc[v] + off; }
+        else { src[u] = (int32_t)v; fixval[u] = off; }
     }
     std::vector<uint8_t> cls(nn, CLS_NOT_FREE);
     std::vector<double> W(nn, 0.0), bfix(nn, 0.0);
@@ -951,7 +983,7 @@ int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* dsc) {
                 int kind;
                 if (rt[v] == 5) kind = NB_FREE;
                 else if (src[v] < 0) { kind = NB_FIXED; bfix[u] += cc[d] * fixval[v]; }
-                else if ((size_t)src[v] == u) kind = NB_SELF;
+                else if ((size_t)src[v] == u) { kind = NB_SELF; bfix[u] += cc[d] * fixval[v]; }
                 else return sf_fail(ctx, SF_ERR_UNSUPPORTED, "node %zu couples to a face row of another node", u);
                 c |= (uint8_t)(kind << (2 * d));
             }
@@ -1068,15 +1100,74 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
                        (const double*)P->bfix, (const double*)ctx->d_rho, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
                        P->partial);
     finalize<1>(P, st, P->partial, ge, S_R0);
-    int rc = read_scalars(ctx, P);
-    if (rc) return rc;
-    double ref2 = P->host_scal[S_R0];
-    double ref = std::sqrt(ref2);
-    if (!(ref > 0)) ref = 1.0;
     int newton = 0, pcg_total = 0;
-    double rel = 0, rnorm = 0;
+    double rel = 0, rnorm = 0, ref = 1.0;
     bool converged = false;
     const bool debug = getenv("SF_DEBUG_SOLVE") != nullptr;
+    const char* eta_env0 = getenv("SF_NEWTON_ETA");
+    const double lim0 = D.linear ? 0.0 : D.Te0;
+    // A warm nonlinear solve (from near its solution) runs as device-driven Newton-PCG: the pairs
+    // of step kernels are queued in batches and the host reads one state word per batch.  Returns
+    // the stop code (NK_COLD: the solve was not warm and nothing was changed) or an error.
+    const bool device_ok = !D.linear && !(nl_tol > 0) && !debug && !P->host_newton;
+    auto device_newton = [&](double eta_d) -> int {
+        const double ee = std::max(eta_d, 1e-5);
+        NkParams np{(const double*)(P->scal + S_R0), rel_tol * rel_tol, ee * ee, lim0, max_it, nl_max};
+        auto step_b = [&](int i, int init) {
+            hipLaunchKernelGGL(nk_step_b, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
+                               (const double*)P->bfix, (const double*)ctx->d_rho, (const double*)P->x, P->r, P->D,
+                               P->P, P->d, P->res, P->z, (const float*)P->p0, (const float*)P->p1,
+                               (const float*)P->q, (const double*)P->ppq, gt, pz0, pz1, gt, P->nk, np, inv_eps, Cn0,
+                               D.phi0, inv_kTe, i, init);
+        };
+        step_b(0, 1);
+        int i = 1, batch = std::min(64, std::max(8, P->last_pairs + 2));
+        while (true) {
+            for (int b = 0; b < batch; b++, i++) {
+                hipLaunchKernelGGL(nk_step_a, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+                                   (const float*)P->D, (const float*)P->z, P->p0, P->p1, P->q, P->x,
+                                   (const float*)P->d, (const double*)pz0, (const double*)pz1, gt, P->ppq,
+                                   (const NkState*)P->nk, np, i);
+                step_b(i, 0);
+            }
+            SF_HIP(ctx, hipGetLastError());
+            SF_HIP(ctx, hipMemcpyAsync(P->host_nk, P->nk + (i - 1) % NK_RING, sizeof(NkState),
+                                       hipMemcpyDeviceToHost, st));
+            SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
+            SF_HIP(ctx, hipStreamSynchronize(st));
+            if (P->host_nk->done) break;
+            if (i > 4 * (max_it + 1) * (nl_max + 1)) return sf_fail(ctx, SF_ERR_STATE, "device Newton-PCG did not stop");
+            batch = std::min(2 * batch, 256);
+        }
+        const NkState fin = *P->host_nk;
+        if (fin.done == NK_COLD) return NK_COLD;
+        P->last_pairs = fin.newton + fin.pcg + 1;
+        ref = std::sqrt(P->host_scal[S_R0]);
+        if (!(ref > 0)) ref = 1.0;
+        newton = fin.newton;
+        pcg_total = fin.pcg;
+        rnorm = std::sqrt(fin.r2);
+        rel = rnorm / ref;
+        if (fin.done == NK_NOT_FINITE || !std::isfinite(rel))
+            return sf_fail(ctx, SF_ERR_STATE, "Newton residual is not finite (diverged)");
+        converged = fin.done == NK_CONVERGED;
+        return fin.done;
+    };
+    int rc = 0;
+    // the last solve converged and this one starts from its potential: go straight to the device
+    // loop (it hands a cold start back untouched)
+    bool device_done = false;
+    if (device_ok && P->warm_hint) {
+        const int code = device_newton(eta_env0 ? atof(eta_env0) : 1e-2);
+        if (code < 0) return code;
+        device_done = code != NK_COLD;
+    }
+    if (!device_done) {
+    rc = read_scalars(ctx, P);
+    if (rc) return rc;
+    double ref2 = P->host_scal[S_R0];
+    ref = std::sqrt(ref2);
+    if (!(ref > 0)) ref = 1.0;
     // Forcing term of the inexact Newton steps: 1e-3 for a solve that starts far from its solution
     // (first residual at least half the reference residual, as from phi = 0), 1e-2 for a solve
     // warm-started from the previous step.  Measured on the cylinder deck: the warm solve takes
@@ -1119,48 +1210,10 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             eta = rel < 0.5 ? 1e-2 : 1e-3;
             eta_chosen = true;
         }
-        // A warm nonlinear solve (from near its solution) runs as device-driven Newton-PCG: the
-        // pairs of step kernels are queued in batches and the host reads one state word per batch.
-        if (it == 0 && rel < 0.5 && !D.linear && !(nl_tol > 0) && !debug && !P->host_newton) {
-            const double rt = rel_tol * ref;
-            const double ee = std::max(eta, 1e-5);
-            NkParams np{rt * rt, ee * ee, 0.25 * rt * rt, lim, max_it, nl_max};
-            auto step_b = [&](int i, int init) {
-                hipLaunchKernelGGL(nk_step_b, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
-                                   (const double*)P->W, (const double*)P->bfix, (const double*)ctx->d_rho,
-                                   (const double*)P->x, P->r, P->D, P->P, P->d, P->res, P->z, (const float*)P->p0,
-                                   (const float*)P->p1, (const float*)P->q, (const double*)P->ppq, gt, pz0, pz1, gt,
-                                   P->nk, np, inv_eps, Cn0, D.phi0, inv_kTe, i, init);
-            };
-            step_b(0, 1);
-            int i = 1, batch = std::min(64, std::max(8, P->last_pairs + 2));
-            while (true) {
-                for (int b = 0; b < batch; b++, i++) {
-                    hipLaunchKernelGGL(nk_step_a, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
-                                       (const float*)P->D, (const float*)P->z, P->p0, P->p1, P->q, P->x,
-                                       (const float*)P->d, (const double*)pz0, (const double*)pz1, gt, P->ppq,
-                                       (const NkState*)P->nk, np, i);
-                    step_b(i, 0);
-                }
-                SF_HIP(ctx, hipGetLastError());
-                SF_HIP(ctx, hipMemcpyAsync(P->host_nk, P->nk + (i - 1) % NK_RING, sizeof(NkState),
-                                           hipMemcpyDeviceToHost, st));
-                SF_HIP(ctx, hipStreamSynchronize(st));
-                if (P->host_nk->done) break;
-                if (i > 4 * (max_it + 1) * (nl_max + 1))
-                    return sf_fail(ctx, SF_ERR_STATE, "device Newton-PCG did not stop");
-                batch = std::min(2 * batch, 256);
-            }
-            const NkState fin = *P->host_nk;
-            P->last_pairs = fin.newton + fin.pcg + 1;
-            newton = fin.newton;
-            pcg_total = fin.pcg;
-            rnorm = std::sqrt(fin.r2);
-            rel = rnorm / ref;
-            if (fin.done == 3 || !std::isfinite(rel))
-                return sf_fail(ctx, SF_ERR_STATE, "Newton residual is not finite (diverged)");
-            converged = fin.done == 1;
-            break;
+        if (it == 0 && rel < 0.5 && device_ok) {
+            const int code = device_newton(eta);
+            if (code < 0) return code;
+            if (code != NK_COLD) break;
         }
         newton++;
         // inexact Newton: solve J y = r to max(eta*||r||, 0.5*tol*ref) with FP32 PCG + FP64 refinement
@@ -1268,6 +1321,8 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         }
         SF_HIP(ctx, hipGetLastError());
     }
+    }   // !device_done
+    P->warm_hint = converged;
     hipLaunchKernelGGL(expand_phi, dim3(ge), dim3(256), 0, st, nn, (const int32_t*)P->src, (const double*)P->fixval,
                        (const double*)P->x, ctx->d_phi);
     SF_HIP(ctx, hipGetLastError());

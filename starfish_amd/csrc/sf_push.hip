@@ -1604,6 +1604,7 @@ int push_deposit(sf_ctx* ctx, Species* sp) {
     if (!ctx->mesh_ready) return sf_fail(ctx, SF_ERR_STATE, "mesh not set up");
     const MeshP& m = ctx->mp;
     MeshD md = ctx->meshd();
+    md.bnd_hits = sp->bnd_hits;
     const int rc_scales = prepare_channel_scales(ctx, sp);
     if (rc_scales) return rc_scales;
     PushC pc = make_pushc(ctx, sp);

@@ -129,11 +129,11 @@ def write_vtk_particles(sim, material, file_name, particle_count=None, animation
 class StatsWriter:
     """StatsModule: one CSV line per `skip` steps with, per material, the mover sums (mass,
     momentum, energy) and the macroparticle count; per source, the mass it added since the last
-    line and the matching current density; per boundary and material, the flux.
-
-    The device push does not tally particles per boundary, so the flux columns carry NaN.  Mass,
-    momentum and energy are those of the last push (sf_species_stats), the reference's
-    getMassSum / getMomentumSum / getEnergySum."""
+    line and the matching current density; per boundary and material, the flux of particles the
+    boundary absorbed since the last line over its area and the interval (Material.getFluxInst:
+    the device tallies absorbed particles per boundary, sf_species_boundary_hits, and the count
+    times spwt is the reference's deposit sum).  Mass, momentum and energy are those of the last
+    push (sf_species_stats), the reference's getMassSum / getMomentumSum / getEnergySum."""
 
     def __init__(self, sim, file_name="starfish_stats.csv", skip=1):
         self.sim = sim
@@ -190,9 +190,20 @@ class StatsWriter:
             cur = mass_gen * mat.charge / (mat.mass * area * self.skip * dt) if area > 0 else float("nan")
             line += "," + java_format_g(mass_gen, 4)
             line += "," + java_format_g(cur, 4)
-        for _b in sim.deck.boundaries:
-            for _m in sim.materials:
-                line += "," + java_format_g(float("nan"), 4)
+        nb = len(sim.deck.boundaries)
+        hits = {}
+        for m in sim.materials:
+            if _kinetic(m):
+                h = np.zeros(max(1, nb), dtype=np.int64)
+                sim.ctx.call("sf_species_boundary_hits", m.index, nb, h.ctypes.data_as(C.POINTER(C.c_int64)), 1)
+                hits[m.name] = h
+        t = self.skip * dt
+        for b_i, b in enumerate(sim.deck.boundaries):
+            area = _boundary_area(b, sim.deck.mesh.domain)
+            for m in sim.materials:
+                dep = float(hits[m.name][b_i]) * m.spec.spwt if m.name in hits else 0.0
+                flux = dep / (area * t) if area > 0 else (float("nan") if dep == 0 else float("inf"))
+                line += "," + java_format_g(flux, 4)
         self.f.write(line + "\n")
         self.f.flush()
 

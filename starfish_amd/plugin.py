@@ -93,6 +93,9 @@ class Geometry:
         ctx.lib.sf_geometry_export(g, _ip(self.node_type), _dp(self.node_bc), _dp(self.node_vol),
                                    _dp(self.seg_xy), _ip(self.seg_btype), _ip(self.seg_absorb),
                                    _ip(self.node_seg_ptr), _ip(self.node_seg_idx))
+        self.seg_bnd = np.zeros(max(1, self.nseg), np.int32)     # boundary of each segment (deck order)
+        ctx.lib.sf_geometry_segment_boundaries(g, _ip(self.seg_bnd))
+        self.nbnd = len(deck.boundaries)
         ctx.lib.sf_geometry_free(g)
 
 
@@ -310,6 +313,8 @@ class Simulation:
         self.ctx.call("sf_mesh_setup", C.byref(g.md), _ip(g.face_type), _dp(g.face_value), _ip(g.node_type),
                       _dp(g.node_bc), _dp(g.node_vol), g.nseg, _dp(g.seg_xy), _ip(g.seg_btype), _ip(g.seg_absorb),
                       _ip(g.node_seg_ptr), _ip(g.node_seg_idx))
+        # per-boundary impact tallies of every species (StatsModule's flux columns)
+        self.ctx.call("sf_mesh_segment_boundaries", g.nseg, _ip(g.seg_bnd), g.nbnd)
         self.materials = []
         for spec in deck.materials:
             kind = spec.kind.upper()
