@@ -57,10 +57,53 @@ def parse():
                     help="C2-style runs (random cylinder radius / bias) that provide training snapshots")
     ap.add_argument("--learned-train-steps", type=int, default=600)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--cnn-only", action="store_true",
+                    help="time only the learned guess's network (training steps at 1024x512, batch 4, on "
+                         "synthetic samples) and print that JSON line")
     ap.add_argument("--host-sources", action="store_true",
                     help="sample the boundary injection on the host with the reference's random stream "
                          "(default: sampled on the device)")
     return ap.parse_args()
+
+
+# (input channels, output channels, grid divisor) of the network's 3x3 convolutions, input channels
+# padded to multiples of 32 as the kernels run them (sf_cnn.hip: conv_init)
+CNN_LAYERS = [(32, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2), (64, 128, 4), (128, 128, 4),
+              (192, 64, 2), (64, 64, 2), (96, 32, 1), (32, 32, 1)]
+
+
+def cnn_flops(H, W, batch):
+    """Matrix-core flops of one forward pass and of one training step (forward, backward-data for
+    every layer but the first, backward-weights) at grid H x W and the given batch."""
+    fwd = sum(2.0 * 9 * k * n * (H // d) * (W // d) * batch for k, n, d in CNN_LAYERS)
+    first = 2.0 * 9 * 32 * 32 * H * W * batch
+    return fwd, 3.0 * fwd - first
+
+
+def cnn_throughput(ctx, ni, nj, batch=4, steps=20):
+    """Training steps of the learned guess's network on synthetic samples at the benchmark's grid:
+    device time per step (HIP events on the library's stream) and matrix-core TFLOP/s."""
+    import ctypes as C
+    import numpy as np
+    from starfish_amd.learned import LearnedGuess
+    net = LearnedGuess(ctx, ni=ni, nj=nj, max_batch=batch, seed=7)
+    try:
+        rng = np.random.default_rng(5)
+        for _ in range(batch):
+            net.add_sample(rng.normal(size=(net.H, net.W, 4)).astype(np.float32),
+                           rng.normal(size=(net.H, net.W)).astype(np.float32), np.ones((net.H, net.W), np.uint8))
+        net.train(2, batch=batch, lr=1e-4)
+        ctx.call("sf_ctx_synchronize")
+        t0 = time.perf_counter()
+        net.train(steps, batch=batch, lr=1e-4)
+        ctx.call("sf_ctx_synchronize")
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        fwd, train = cnn_flops(net.H, net.W, batch)
+        return {"grid": [net.H, net.W], "batch": batch, "steps": steps, "train_step_ms": ms,
+                "train_step_tflops": train / (ms * 1e-3) / 1e12, "forward_gflop": fwd / 1e9,
+                "train_step_gflop": train / 1e9, "peak_bf16_tflops": 2500.0}
+    finally:
+        net.close()
 
 
 def learned_guess(sim, deck, args, rho0):
@@ -133,6 +176,7 @@ def learned_guess(sim, deck, args, rho0):
                "all_converged": all(r[k][4] == 0 for r in rows for k in r) and cold["zero"]["converged"]
                and cold["learned"]["converged"]}
         out["iteration_ratio_plain_over_learned"] = out["plain"]["pcg_iterations"] / max(1.0, out["learned"]["pcg_iterations"])
+        out["network_throughput"] = cnn_throughput(sim.ctx, m.ni, m.nj)
         return out, net
     except Exception:
         net.close()
@@ -317,6 +361,14 @@ def main():
     from starfish_amd import deck as D
     from starfish_amd.plugin import Simulation
 
+    if args.cnn_only:
+        from starfish_amd.plugin import Context
+        ctx = Context(0)
+        try:
+            print(json.dumps({"cnn": cnn_throughput(ctx, 1024, 512)}), flush=True)
+        finally:
+            ctx.close()
+        return
     npart = int(args.particles)
     deck = D.config_c2(steps=args.steps, particles=npart)
     sim = Simulation(deck, capacity=int(npart * 1.6) + 1_000_000, device_sources=not args.host_sources)

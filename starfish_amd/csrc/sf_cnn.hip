@@ -53,7 +53,6 @@ __device__ __forceinline__ bf16x8 zero8() {
 //   forward:       out[p][n] = sum_tap sum_k in[p + d(tap)][k] * w[tap][n][k]   (+ bias, ReLU)
 //   backward data: din[p][k'] = sum_tap sum_n' g[p - d(tap)][n'] * w[tap][n'][k'] with the weight
 //                  stored transposed as wt[tap][k'][n'] -> the same kernel with sgn = -1.
-// One workgroup: 64 pixels of one row x 64 output channels; wave w owns pixels 16w..16w+15.
 // Requirements (checked on the host): K multiple of 32, N multiple of 16, channel strides and
 // offsets multiples of 8 (16-byte operand loads).
 // ------------------------------------------------------------------------------------------
@@ -67,114 +66,234 @@ struct FView {
     int cs, co;
 };
 
-template <int MODE>   // 0: forward BF16 out (+bias, optional ReLU); 1: backward data, FP32 out (+= or =)
-__global__ void __launch_bounds__(256) conv3x3_mfma(const uint16_t* __restrict__ in, int in_cs, int in_co, int K,
-                                                    const uint16_t* __restrict__ w, const float* __restrict__ bias,
-                                                    int N, uint16_t* __restrict__ out, float* __restrict__ fout,
-                                                    int out_cs, int out_co, int H, int Wd, int relu, int sgn,
-                                                    int accumulate) {
-    const int ntn = (N + 63) / 64;
-    const int x0 = blockIdx.x * 64, y = blockIdx.y;
-    const int b = blockIdx.z / ntn, n0 = (blockIdx.z % ntn) * 64;
+// The operands are staged in LDS.  One workgroup: CONV_TP pixels of one row
+// x TN output channels (TN = 64: waves 2 x 2 over pixels x channels; TN = 32: 4 x 1).  For each
+// chunk of 32 input channels the 3 x (CONV_TP + 2) input pixels and the 9 x TN x 32 weights go to
+// LDS once, and every wave runs its 16 x 16 tiles over the 9 taps from there: an input fragment is
+// read once per tap for two channel tiles, a weight fragment once per tap for all the wave's pixel
+// tiles.  The next chunk's operands are loaded into registers while the current one is multiplied.
+constexpr int CONV_TP = 128;
+template <int MODE, int TN>
+__global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ in, int in_cs, int in_co, int K,
+                                                   const uint16_t* __restrict__ w, const float* __restrict__ bias,
+                                                   int N, uint16_t* __restrict__ out, float* __restrict__ fout,
+                                                   int out_cs, int out_co, int H, int Wd, int relu, int sgn,
+                                                   int accumulate) {
+    constexpr int WN = TN / 32;                 // waves along the channels (two 16-channel tiles each)
+    constexpr int WP = 4 / WN;                  // waves along the pixels
+    constexpr int PT = CONV_TP / 16 / WP;       // 16-pixel tiles per wave
+    constexpr int IW = CONV_TP + 2;             // staged pixels per row
+    constexpr int NIN = 3 * IW * 4;             // 16-byte groups of the input stage (8 channels each)
+    constexpr int NWT = 9 * TN * 4;             // 16-byte groups of the weight stage
+    constexpr int RIN = (NIN + 255) / 256, RWT = (NWT + 255) / 256;
+    __shared__ uint4 s_in[NIN];                 // [row][pixel][group]
+    __shared__ uint4 s_w[NWT];                  // [tap][channel][group]
+    const int ntn = (N + TN - 1) / TN;
+    const int x0 = blockIdx.x * CONV_TP, y = blockIdx.y;
+    const int b = blockIdx.z / ntn, n0 = (blockIdx.z % ntn) * TN;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int prow = lane & 15, kq = (lane >> 4) * 8;
-    const int px = x0 + wave * 16 + prow;
-    f32x4 acc[4];
+    const int wp = wave % WP, wn = wave / WP;
+    const int prow = lane & 15, kq = lane >> 4;
+    const uint4 zero = make_uint4(0, 0, 0, 0);
+    uint4 rin[RIN], rwt[RWT];
+    auto fetch = [&](int k0) {
 #pragma unroll
-    for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < 9; tap++) {
-        const int dy = sgn * (tap / 3 - 1), dx = sgn * (tap % 3 - 1);
-        const int yy = y + dy, xx = px + dx;
-        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < Wd;
-        const uint16_t* arow = in + (((size_t)b * H + (ok ? yy : 0)) * Wd + (ok ? xx : 0)) * in_cs + in_co;
-        for (int k0 = 0; k0 < K; k0 += 32) {
-            bf16x8 a = ok ? load8(arow + k0 + kq) : zero8();
+        for (int q = 0; q < RIN; q++) {
+            const int e = threadIdx.x + 256 * q;
+            rin[q] = zero;
+            if (e < NIN) {
+                const int g = e & 3, px = (e >> 2) % IW, r = (e >> 2) / IW;
+                const int yy = y - 1 + r, xx = x0 - 1 + px;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
+                    rin[q] = *reinterpret_cast<const uint4*>(in + (((size_t)b * H + yy) * Wd + xx) * in_cs + in_co +
+                                                             k0 + g * 8);
+            }
+        }
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int n = n0 + t * 16 + prow;
-                bf16x8 bv = (n < N) ? load8(w + ((size_t)tap * N + n) * K + k0 + kq) : zero8();
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
+        for (int q = 0; q < RWT; q++) {
+            const int e = threadIdx.x + 256 * q;
+            rwt[q] = zero;
+            if (e < NWT) {
+                const int g = e & 3, n = (e >> 2) % TN, tap = (e >> 2) / TN;
+                if (n0 + n < N)
+                    rwt[q] = *reinterpret_cast<const uint4*>(w + ((size_t)tap * N + n0 + n) * K + k0 + g * 8);
+            }
+        }
+    };
+    f32x4 acc[PT][2];
+#pragma unroll
+    for (int p = 0; p < PT; p++) acc[p][0] = acc[p][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    fetch(0);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        __syncthreads();   // the previous chunk's fragments are read
+#pragma unroll
+        for (int q = 0; q < RIN; q++)
+            if (threadIdx.x + 256 * q < NIN) s_in[threadIdx.x + 256 * q] = rin[q];
+#pragma unroll
+        for (int q = 0; q < RWT; q++)
+            if (threadIdx.x + 256 * q < NWT) s_w[threadIdx.x + 256 * q] = rwt[q];
+        __syncthreads();
+        if (k0 + 32 < K) fetch(k0 + 32);
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = sgn * (tap / 3 - 1), dx = sgn * (tap % 3 - 1);
+            bf16x8 bw[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const uint4 v = s_w[(tap * TN + wn * 32 + t * 16 + prow) * 4 + kq];
+                memcpy(&bw[t], &v, 16);
+            }
+#pragma unroll
+            for (int p = 0; p < PT; p++) {
+                const int px = (wp * PT + p) * 16 + prow;
+                const uint4 v = s_in[((1 + dy) * IW + px + 1 + dx) * 4 + kq];
+                bf16x8 a;
+                memcpy(&a, &v, 16);
+                acc[p][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[0], acc[p][0], 0, 0, 0);
+                acc[p][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[1], acc[p][1], 0, 0, 0);
             }
         }
     }
     // D layout: lane holds rows (lane/16)*4 + i (pixels), column lane%16 (channel)
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const int n = n0 + t * 16 + prow;
-        if (n >= N) continue;
+    for (int p = 0; p < PT; p++)
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int n = n0 + wn * 32 + t * 16 + prow;
+            if (n >= N) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int xo = x0 + (wp * PT + p) * 16 + (lane >> 4) * 4 + i;
+                if (xo >= Wd) continue;
+                const size_t o = (((size_t)b * H + y) * Wd + xo) * out_cs + out_co + n;
+                float v = acc[p][t][i];
+                if (MODE == 0) {
+                    v += bias[n];
+                    if (relu) v = fmaxf(v, 0.f);
+                    out[o] = f2bf(v);
+                } else {
+                    fout[o] = accumulate ? fout[o] + v : v;
+                }
+            }
+        }
+}
+
+// Weight gradient with the operands staged in LDS.  One workgroup: 32 input x 32 output channels
+// for all 9 taps over a band of 64-pixel row chunks; wave w owns the 16 x 16 sub-tile
+// (w & 1, w >> 1) of every tap.  Each chunk's three input rows (with the three horizontal shifts)
+// and its gradient row are staged transposed to [channel][pixel], so an operand fragment
+// (8 consecutive pixels of one channel) is one 16-byte LDS read, and the gradient fragment serves
+// the 9 taps.  Rows hold 16 four-pixel entries plus 2 of padding, so the 16 channels of a
+// fragment read start on distinct banks.  The next chunk's pixels are loaded into registers while
+// the current one is multiplied.  The workgroups of the first input-channel tile also sum the
+// gradient over the pixels (the bias gradient) with one more MFMA per step against an all-ones
+// operand.  Requirements: K and N multiples of 32, strides and offsets multiples of 8.
+constexpr int WG_PX = 64;           // pixels per chunk (two MFMA reduction steps)
+constexpr int WG_Q = WG_PX / 4 + 2; // four-pixel entries per staged row (with padding)
+__device__ __forceinline__ uint32_t bf_at(const uint4& v, int c) {
+    const uint32_t w = c < 2 ? v.x : (c < 4 ? v.y : (c < 6 ? v.z : v.w));
+    return (c & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+__global__ void __launch_bounds__(256) conv3x3_wgrad_lds(const uint16_t* __restrict__ in, int in_cs, int in_co,
+                                                         int K, const uint16_t* __restrict__ g, int g_cs, int N,
+                                                         float* __restrict__ dw, float* __restrict__ db, int H,
+                                                         int Wd, int B, int chunks_per_block) {
+    __shared__ uint2 s_in[3][3][32][WG_Q];   // [dx + 1][dy + 1][input channel][pixel / 4]
+    __shared__ uint2 s_g[32][WG_Q];          // [output channel][pixel / 4]
+    const int nkt = K / 32;
+    const int k0 = (blockIdx.y % nkt) * 32, n0 = (blockIdx.y / nkt) * 32;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int tk = wave & 1, tn = wave >> 1;
+    const int prow = lane & 15, kq = lane >> 4;
+    const int cpr = (Wd + WG_PX - 1) / WG_PX;
+    const int nchunks = B * H * cpr;
+    const int c_begin = blockIdx.x * chunks_per_block;
+    const int c_end = min(c_begin + chunks_per_block, nchunks);
+    // staging items: threads 0..191 one input item (row r, pixel quad q, channel group gk: pixels
+    // 4q - 1 .. 4q + 4 of 8 channels), threads 192..255 one gradient item (quad q, group gn)
+    const bool is_in = t < 192;
+    const int it = is_in ? t : t - 192;
+    const int r = it / 64, q = (it % 64) / 4, gq = it % 4;
+    uint4 v[6];
+    // (the same loads for both kinds of item, so that v stays in registers)
+    const uint16_t* const src = is_in ? in : g;
+    const int scs = is_in ? in_cs : g_cs, sco = is_in ? in_co + k0 + gq * 8 : n0 + gq * 8;
+    const int sdy = is_in ? r - 1 : 0, sdx = is_in ? -1 : 0, sj = is_in ? 6 : 4;
+    auto fetch = [&](int c) {
+        const int row = c / cpr, x0 = (c % cpr) * WG_PX;
+        const int b = row / H, yy = row % H + sdy;
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            const int xx = x0 + 4 * q + sdx + j;
+            const bool ok = j < sj && yy >= 0 && yy < H && xx >= 0 && xx < Wd;
+            const size_t o = ok ? (((size_t)b * H + yy) * Wd + xx) * scs + sco : 0;
+            const uint4 u = *reinterpret_cast<const uint4*>(src + o);
+            v[j] = ok ? u : make_uint4(0, 0, 0, 0);
+        }
+    };
+    f32x4 acc[9];
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) acc[tap] = f32x4{0.f, 0.f, 0.f, 0.f};
+    f32x4 accb = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bool do_bias = k0 == 0 && tk == 0;   // uniform per wave
+    bf16x8 ones;
+    {
+        const uint32_t o2 = 0x3F803F80u;   // two BF16 1.0
+        const uint4 u = make_uint4(o2, o2, o2, o2);
+        memcpy(&ones, &u, 16);
+    }
+    if (c_begin < c_end) fetch(c_begin);
+    for (int c = c_begin; c < c_end; c++) {
+        __syncthreads();   // the previous chunk's fragments are read
+        if (is_in) {
+#pragma unroll
+            for (int d = 0; d < 3; d++)   // copy d holds pixels 4q + d - 1 + i: v[d + i]
+#pragma unroll
+                for (int ch = 0; ch < 8; ch++)
+                    s_in[d][r][gq * 8 + ch][q] = make_uint2(bf_at(v[d], ch) | (bf_at(v[d + 1], ch) << 16),
+                                                            bf_at(v[d + 2], ch) | (bf_at(v[d + 3], ch) << 16));
+        } else {
+#pragma unroll
+            for (int ch = 0; ch < 8; ch++)
+                s_g[gq * 8 + ch][q] = make_uint2(bf_at(v[0], ch) | (bf_at(v[1], ch) << 16),
+                                                 bf_at(v[2], ch) | (bf_at(v[3], ch) << 16));
+        }
+        __syncthreads();
+        if (c + 1 < c_end) fetch(c + 1);
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            bf16x8 bg;
+            {
+                const uint2 lo = s_g[tn * 16 + prow][8 * s + 2 * kq], hi = s_g[tn * 16 + prow][8 * s + 2 * kq + 1];
+                const uint4 u = make_uint4(lo.x, lo.y, hi.x, hi.y);
+                memcpy(&bg, &u, 16);
+            }
+            if (do_bias) accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ones, bg, accb, 0, 0, 0);
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++) {
+                const int dy = tap / 3, dx = tap % 3;   // copy indices (offset + 1)
+                const uint2 lo = s_in[dx][dy][tk * 16 + prow][8 * s + 2 * kq];
+                const uint2 hi = s_in[dx][dy][tk * 16 + prow][8 * s + 2 * kq + 1];
+                const uint4 u = make_uint4(lo.x, lo.y, hi.x, hi.y);
+                bf16x8 a;
+                memcpy(&a, &u, 16);
+                acc[tap] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bg, acc[tap], 0, 0, 0);
+            }
+        }
+    }
+    // D rows (lane/16)*4+i = input channel, column lane%16 = output channel
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int xo = x0 + wave * 16 + (lane >> 4) * 4 + i;
-            if (xo >= Wd) continue;
-            const size_t o = (((size_t)b * H + y) * Wd + xo) * out_cs + out_co + n;
-            float v = acc[t][i];
-            if (MODE == 0) {
-                v += bias[n];
-                if (relu) v = fmaxf(v, 0.f);
-                out[o] = f2bf(v);
-            } else {
-                fout[o] = accumulate ? fout[o] + v : v;
-            }
+            const int k = k0 + tk * 16 + kq * 4 + i;
+            const int n = n0 + tn * 16 + prow;
+            atomicAdd(&dw[((size_t)tap * N + n) * K + k], acc[tap][i]);
         }
-    }
+    // every row of accb holds the column sums: lanes 0..15 add row 0
+    if (do_bias && kq == 0) atomicAdd(&db[n0 + tn * 16 + prow], accb[0]);
 }
 
-// Weight gradient: dw[tap][n][k] += sum_p in[p + d(tap)][k] * g[p][n] (FP32 atomics per block),
-// the MFMA reduction dimension being 32 consecutive pixels of one row.
-// Block: one tap, 32 k x 32 n (2x2 tiles over the 4 waves), a band of rows.
-__global__ void __launch_bounds__(256) conv3x3_wgrad(const uint16_t* __restrict__ in, int in_cs, int in_co, int K,
-                                                     const uint16_t* __restrict__ g, int g_cs, int N,
-                                                     float* __restrict__ dw, int H, int Wd, int B,
-                                                     int rows_per_block) {
-    const int tap = blockIdx.y;
-    const int nkt = K / 32;
-    const int k0 = (blockIdx.z % nkt) * 32, n0 = (blockIdx.z / nkt) * 32;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tk = wave & 1, tn = wave >> 1;           // this wave's 16x16 tile of the 32x32 block
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int mrow = lane & 15, pq = (lane >> 4) * 8;
-    const int kk = k0 + tk * 16 + mrow;                 // A row = input channel
-    const int nn = n0 + tn * 16 + mrow;                 // B column = output channel
-    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int row_begin = blockIdx.x * rows_per_block;
-    const int total_rows = B * H;
-    for (int r = row_begin; r < min(row_begin + rows_per_block, total_rows); r++) {
-        const int b = r / H, y = r % H;
-        const int yy = y + dy;
-        for (int x0 = 0; x0 < Wd; x0 += 32) {
-            uint16_t av[8], bv[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int x = x0 + pq + j;
-                const int xx = x + dx;
-                const bool okg = x < Wd;
-                const bool oki = okg && yy >= 0 && yy < H && xx >= 0 && xx < Wd;
-                av[j] = oki ? in[(((size_t)b * H + yy) * Wd + xx) * in_cs + in_co + kk] : (uint16_t)0;
-                bv[j] = okg ? g[(((size_t)b * H + y) * Wd + x) * g_cs + nn] : (uint16_t)0;
-            }
-            bf16x8 a, bb;
-            memcpy(&a, av, 16);
-            memcpy(&bb, bv, 16);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc, 0, 0, 0);
-        }
-    }
-    // D rows (lane/16)*4+i = k, column lane%16 = n
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int k = k0 + tk * 16 + (lane >> 4) * 4 + i;
-        const int n = n0 + tn * 16 + mrow;
-        atomicAdd(&dw[((size_t)tap * N + n) * K + k], acc[i]);
-    }
-}
-
-// bias gradient: db[n] += sum_p g[p][n]
-__global__ void bias_grad(const uint16_t* __restrict__ g, int g_cs, int N, int64_t npix, float* __restrict__ db) {
-    for (int n = threadIdx.x; n < N; n += blockDim.x) {
-        float s = 0;
-        for (int64_t p = blockIdx.x; p < npix; p += gridDim.x) s += bf2f(g[p * g_cs + n]);
-        atomicAdd(&db[n], s);
-    }
-}
 
 // ReLU backward into a BF16 operand: g = (act > 0) ? d : 0
 __global__ void relu_bwd(const uint16_t* __restrict__ act, int a_cs, int a_co, const float* __restrict__ d, int d_cs,
@@ -214,16 +333,18 @@ __global__ void avgpool_bwd(const float* __restrict__ dout, int d_cs, int d_co, 
         din[(((size_t)b * H + y) * Wd + x) * i_cs + i_co + c] += v;
     }
 }
-// nearest 2x upsampling (forward BF16 into a view, backward FP32 sum of the four children)
+// nearest 2x upsampling (forward BF16 into a view, 8 channels per 16-byte access; backward FP32
+// sum of the four children).  Forward: C, strides and offset multiples of 8.
 __global__ void upsample_fwd(const uint16_t* __restrict__ in, int in_cs, int C, int Hi, int Wi, int B,
                              uint16_t* __restrict__ out, int out_cs, int out_co) {
-    const int H = 2 * Hi, Wd = 2 * Wi;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * H * Wd * C;
+    const int H = 2 * Hi, Wd = 2 * Wi, G = C / 8;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * H * Wd * G;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int c = (int)(i % C);
-        int64_t q = i / C;
-        int x = (int)(q % Wd), y = (int)((q / Wd) % H), b = (int)(q / ((int64_t)Wd * H));
-        out[(((size_t)b * H + y) * Wd + x) * out_cs + out_co + c] = in[(((size_t)b * Hi + y / 2) * Wi + x / 2) * in_cs + c];
+        const int g = (int)(i % G);
+        const int64_t q = i / G;
+        const int x = (int)(q % Wd), y = (int)((q / Wd) % H), b = (int)(q / ((int64_t)Wd * H));
+        *reinterpret_cast<uint4*>(out + (((size_t)b * H + y) * Wd + x) * out_cs + out_co + 8 * g) =
+            *reinterpret_cast<const uint4*>(in + (((size_t)b * Hi + y / 2) * Wi + x / 2) * in_cs + 8 * g);
     }
 }
 __global__ void upsample_bwd(const float* __restrict__ dout, int d_cs, int d_co, int C, int Hi, int Wi, int B,
@@ -260,27 +381,60 @@ __global__ void mse_loss(const float* __restrict__ y, const float* __restrict__ 
         acc += e * e;
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, acc * inv_count);
-}
-// head backward: dw[c] += sum_p dy[p] act[p][c]; db += sum dy; dact[p][c] = dy[p] w[c] (FP32)
-__global__ void head_bwd(const uint16_t* __restrict__ act, int C, int64_t npix, const float* __restrict__ w,
-                         const float* __restrict__ dy, float* __restrict__ dw, float* __restrict__ db,
-                         float* __restrict__ dact) {
-    __shared__ float sw[128], sdb[4];
-    for (int c = threadIdx.x; c < C; c += blockDim.x) sw[c] = 0;
-    if (threadIdx.x == 0) sdb[0] = 0;
+    __shared__ float red[16];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
+    if (threadIdx.x == 0) {
+        float s = 0.f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
+        atomicAdd(loss, s * inv_count);   // one atomic per block
+    }
+}
+// head backward (C = HEAD_C channels): dw[c] += sum_p dy[p] act[p][c]; db += sum dy;
+// dact[p][c] = dy[p] w[c] (FP32).  Each thread keeps its channel sums in registers; the block
+// adds them with shuffles and one atomic per channel.
+constexpr int HEAD_C = 32;
+__global__ void __launch_bounds__(256) head_bwd(const uint16_t* __restrict__ act, int64_t npix,
+                                                const float* __restrict__ w, const float* __restrict__ dy,
+                                                float* __restrict__ dw, float* __restrict__ db,
+                                                float* __restrict__ dact) {
+    __shared__ float red[HEAD_C + 1][4];
+    float sw[HEAD_C], sdb = 0.f, wv[HEAD_C];
+#pragma unroll
+    for (int c = 0; c < HEAD_C; c++) {
+        sw[c] = 0.f;
+        wv[c] = w[c];
+    }
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
-        float d = dy[p];
-        for (int c = 0; c < C; c++) {
-            atomicAdd(&sw[c], d * bf2f(act[p * C + c]));
-            dact[p * C + c] = d * w[c];
+        const float d = dy[p];
+        const uint4* a4 = reinterpret_cast<const uint4*>(act + p * HEAD_C);
+        float4* o4 = reinterpret_cast<float4*>(dact + p * HEAD_C);
+#pragma unroll
+        for (int q = 0; q < HEAD_C / 8; q++) {
+            const uint4 u = a4[q];
+            const uint32_t uw[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                sw[8 * q + 2 * e] += d * bf2f((uint16_t)(uw[e] & 0xFFFFu));
+                sw[8 * q + 2 * e + 1] += d * bf2f((uint16_t)(uw[e] >> 16));
+            }
+            o4[2 * q] = make_float4(d * wv[8 * q], d * wv[8 * q + 1], d * wv[8 * q + 2], d * wv[8 * q + 3]);
+            o4[2 * q + 1] = make_float4(d * wv[8 * q + 4], d * wv[8 * q + 5], d * wv[8 * q + 6], d * wv[8 * q + 7]);
         }
-        atomicAdd(&sdb[0], d);
+        sdb += d;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c <= HEAD_C; c++) {
+        float v = c < HEAD_C ? sw[c] : sdb;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[c][wave] = v;
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&dw[c], sw[c]);
-    if (threadIdx.x == 0) atomicAdd(db, sdb[0]);
+    if (threadIdx.x <= HEAD_C) {
+        const float v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+        atomicAdd(threadIdx.x < HEAD_C ? &dw[threadIdx.x] : db, v);
+    }
 }
 // Adam (Kingma & Ba 2015) on FP32 master weights
 __global__ void adam_step(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -331,7 +485,7 @@ struct sf_cnn {
     int ni = 0, nj = 0;       // mesh
     int H = 0, W = 0;         // network grid (mesh padded to multiples of 4)
     int Bmax = 0;
-    int C1 = 32, C2 = 64, C3 = 128;
+    int C1 = sf::HEAD_C, C2 = 64, C3 = 128;
     sf::ConvL c1a, c1b, c2a, c2b, c3a, c3b, c4a, c4b, c5a, c5b;
     float *hw = nullptr, *hb = nullptr, *ghw = nullptr, *ghb = nullptr;
     float *mhw = nullptr, *vhw = nullptr, *mhb = nullptr, *vhb = nullptr;
@@ -406,26 +560,41 @@ static void conv_pack(sf_cnn* c, ConvL& L) {
                        L.wf, L.wt);
 }
 
+// 3x3 convolution launch (conv3x3_lds): 64-channel workgroup tiles when N allows, else 32
+template <int MODE>
+static void conv_launch(sf_cnn* c, const uint16_t* in, int in_cs, int in_co, int K, const uint16_t* w,
+                        const float* bias, int N, uint16_t* out, float* fout, int out_cs, int out_co, int H, int W,
+                        int B, int relu, int sgn, int accumulate) {
+    const bool t64 = N % 64 == 0;
+    const int tn = t64 ? 64 : 32;
+    const dim3 g((W + CONV_TP - 1) / CONV_TP, H, B * ((N + tn - 1) / tn));
+    if (t64)
+        hipLaunchKernelGGL((conv3x3_lds<MODE, 64>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias, N,
+                           out, fout, out_cs, out_co, H, W, relu, sgn, accumulate);
+    else
+        hipLaunchKernelGGL((conv3x3_lds<MODE, 32>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias, N,
+                           out, fout, out_cs, out_co, H, W, relu, sgn, accumulate);
+}
 // forward conv: in view (cs, co, K channels) -> out view, bias + ReLU
 static void conv_fwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs,
                      int out_co, int H, int W, int B) {
-    dim3 g((W + 63) / 64, H, B * ((L.N + 63) / 64));
-    hipLaunchKernelGGL(conv3x3_mfma<0>, g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, L.K, (const uint16_t*)L.wf,
-                       (const float*)L.b, L.N, out, (float*)nullptr, out_cs, out_co, H, W, 1, 1, 0);
+    conv_launch<0>(c, in, in_cs, in_co, L.K, (const uint16_t*)L.wf, (const float*)L.b, L.N, out, (float*)nullptr,
+                   out_cs, out_co, H, W, B, 1, 1, 0);
 }
 // backward of one conv: g (BF16 [npix][N]) -> weight / bias gradients and, if din != null, din
 static void conv_bwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, int in_co, const uint16_t* g,
                      float* din, int din_cs, int din_co, int H, int W, int B) {
-    const int rows = 8;
-    dim3 gw((B * H + rows - 1) / rows, 9, (L.K / 32) * (L.N / 32));
-    hipLaunchKernelGGL(conv3x3_wgrad, gw, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, L.K, g, L.N, L.N, L.gw, H, W,
-                       B, rows);
-    hipLaunchKernelGGL(bias_grad, dim3(256), dim3(256), 0, c->ctx->stream, g, L.N, L.N, (int64_t)B * H * W, L.gb);
-    if (din) {
-        dim3 gd((W + 63) / 64, H, B * ((L.K + 63) / 64));
-        hipLaunchKernelGGL(conv3x3_mfma<1>, gd, dim3(256), 0, c->ctx->stream, g, L.N, 0, L.N, (const uint16_t*)L.wt,
-                           (const float*)nullptr, L.K, (uint16_t*)nullptr, din, din_cs, din_co, H, W, 0, -1, 0);
-    }
+    // weight gradient: about 2048 workgroups over the (32 x 32 channel tile, pixel band) pairs
+    const int nchunks = B * H * ((W + WG_PX - 1) / WG_PX);
+    const int pairs = (L.K / 32) * (L.N / 32);
+    int bx = std::max(1, std::min(nchunks, (2048 + pairs - 1) / pairs));
+    const int cpb = (nchunks + bx - 1) / bx;
+    bx = (nchunks + cpb - 1) / cpb;
+    hipLaunchKernelGGL(conv3x3_wgrad_lds, dim3(bx, pairs), dim3(256), 0, c->ctx->stream, in, in_cs, in_co, L.K, g, L.N,
+                       L.N, L.gw, L.gb, H, W, B, cpb);
+    if (din)
+        conv_launch<1>(c, g, L.N, 0, L.N, (const uint16_t*)L.wt, (const float*)nullptr, L.K, (uint16_t*)nullptr, din,
+                       din_cs, din_co, H, W, B, 0, -1, 0);
 }
 static void relu_op(sf_cnn* c, const uint16_t* act, int a_cs, int a_co, const float* d, int d_cs, int d_co, int C,
                     int64_t npix) {
@@ -449,11 +618,11 @@ static void forward(sf_cnn* c, int B) {
                        C3 + C2, C3, C2, H2, W2, B, c->p2, C2, 0);
     conv_fwd(c, c->c3a, c->p2, C2, 0, c->e3a, C3, 0, H4, W4, B);
     conv_fwd(c, c->c3b, c->e3a, C3, 0, c->e3, C3, 0, H4, W4, B);
-    hipLaunchKernelGGL(upsample_fwd, dim3(grid1((int64_t)B * H2 * W2 * C3)), dim3(256), 0, st, (const uint16_t*)c->e3,
+    hipLaunchKernelGGL(upsample_fwd, dim3(grid1((int64_t)B * H2 * W2 * C3 / 8)), dim3(256), 0, st, (const uint16_t*)c->e3,
                        C3, C3, H4, W4, B, c->u2, C3 + C2, 0);
     conv_fwd(c, c->c4a, c->u2, C3 + C2, 0, c->d2a, C2, 0, H2, W2, B);
     conv_fwd(c, c->c4b, c->d2a, C2, 0, c->d2, C2, 0, H2, W2, B);
-    hipLaunchKernelGGL(upsample_fwd, dim3(grid1((int64_t)B * H * W * C2)), dim3(256), 0, st, (const uint16_t*)c->d2, C2,
+    hipLaunchKernelGGL(upsample_fwd, dim3(grid1((int64_t)B * H * W * C2 / 8)), dim3(256), 0, st, (const uint16_t*)c->d2, C2,
                        C2, H2, W2, B, c->u1, C2 + C1, 0);
     conv_fwd(c, c->c5a, c->u1, C2 + C1, 0, c->d1a, C1, 0, H, W, B);
     conv_fwd(c, c->c5b, c->d1a, C1, 0, c->d1, C1, 0, H, W, B);
@@ -466,7 +635,7 @@ static void backward(sf_cnn* c, int B) {
     const int H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
     const int64_t n1 = (int64_t)B * H * W, n2 = (int64_t)B * H2 * W2, n4 = (int64_t)B * H4 * W4;
     hipStream_t st = c->ctx->stream;
-    hipLaunchKernelGGL(head_bwd, dim3(256), dim3(256), 0, st, (const uint16_t*)c->d1, C1, n1, (const float*)c->hw,
+    hipLaunchKernelGGL(head_bwd, dim3(1024), dim3(256), 0, st, (const uint16_t*)c->d1, n1, (const float*)c->hw,
                        (const float*)c->dy, c->ghw, c->ghb, c->g_d1);
     relu_op(c, c->d1, C1, 0, c->g_d1, C1, 0, C1, n1);
     conv_bwd(c, c->c5b, c->d1a, C1, 0, c->gop, c->g_d1a, C1, 0, H, W, B);

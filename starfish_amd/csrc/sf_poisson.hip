@@ -29,6 +29,13 @@ enum { NB_SELF = 0, NB_FREE = 1, NB_FIXED = 2 };
 constexpr uint8_t CLS_NOT_FREE = 0xFF;
 
 constexpr int NK_RING = 8;
+// Forcing terms of the inexact Newton steps: a linear solve stops at eta times its starting
+// residual.  ETA_COLD for a solve that starts far from its solution (first residual at least half
+// the reference residual, as from phi = 0), ETA_WARM for one started from the previous step's phi.
+// Measured on the 1024x512 cylinder deck: warm solves take 0.82, 0.76, 0.74, 0.70, 0.67 and
+// 0.68 ms at eta = 1e-3, 3e-3, 1e-2, 3e-2, 5e-2 and 1e-1 (fewer PCG iterations, a few more Newton
+// updates); a cold solve with 1e-2 takes 117 instead of 70 ms.
+constexpr double ETA_COLD = 1e-3, ETA_WARM = 5e-2;
 struct NkState {
     double tol2;   // PCG tolerance (squared) of the current linear solve
     double r2;     // ||r||^2 of the Newton residual at the current linear solve's start (at stop: final)
@@ -65,6 +72,7 @@ struct PoissonState {
     bool host_newton = false;     // SF_HOST_NEWTON: keep every solve on the host-driven loop
     bool warm_hint = false;       // the last solve converged (the next one likely starts warm)
     NkState* nk = nullptr;        // [NK_RING] device-driven Newton-PCG state ring
+    double* nkc = nullptr;        // [NK_RING][2] (r.z, r.r) combined by pair i's nk_step_a
     NkState* host_nk = nullptr;   // pinned copy of the last state read
     double* host_scal = nullptr;  // pinned
     int* host_istate = nullptr;
@@ -146,6 +154,7 @@ void poisson_free(PoissonState* p) {
     if (p->host_istate) hipHostFree(p->host_istate);
     if (p->host_nk) hipHostFree(p->host_nk);
     if (p->nk) hipFree(p->nk);
+    if (p->nkc) hipFree(p->nkc);
     if (p->e0) hipEventDestroy(p->e0);
     if (p->e1) hipEventDestroy(p->e1);
     delete p;
@@ -291,10 +300,14 @@ __global__ void refine_residual(MeshP m, const uint8_t* __restrict__ cls, const 
 // iteration k - 1 is still available for beta.  Convergence (r.r <= tol^2) is detected by
 // every block of the step-1 kernel from the same sums; block 0 records it in istate, which
 // turns every later queued kernel into a no-op.
-constexpr int PT_W = 64, PT_H = 16;   // nodes per step-1 block (256 threads, 4 rows each)
+constexpr int PT_W = 64, PT_H = 16;   // nodes of a tile kernel's block
+constexpr int PT_THREADS = 512;                     // threads of a tile kernel's block
+constexpr int PT_RSTEP = PT_THREADS / PT_W;         // thread t owns column t % PT_W, rows t / PT_W + PT_RSTEP s
+constexpr int PT_RPT = PT_H / PT_RSTEP;             // rows per thread
+static_assert(PT_THREADS % PT_W == 0 && PT_H % PT_RSTEP == 0, "tile shape");
 
 // step 1: beta = rz_k / rz_{k-1} (0 at k = 0); p = z + beta p_old; q = J p; p.q partials
-__global__ void __launch_bounds__(256) pcg_spmv(MeshP m, const uint8_t* __restrict__ cls,
+__global__ void __launch_bounds__(PT_THREADS) pcg_spmv(MeshP m, const uint8_t* __restrict__ cls,
                                                 const float* __restrict__ D, const float* __restrict__ z,
                                                 const float* __restrict__ pold, float* __restrict__ pnew,
                                                 float* __restrict__ q, const double* __restrict__ pz_cur,
@@ -332,7 +345,7 @@ __global__ void __launch_bounds__(256) pcg_spmv(MeshP m, const uint8_t* __restri
     __syncthreads();
     const int li = threadIdx.x % PT_W;
     double acc = 0;
-    for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += 256 / PT_W) {
+    for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += PT_RSTEP) {
         const int i = i0 + 1 + li, j = j0 + 1 + lj;
         if (i >= m.ni || j >= m.nj) continue;
         size_t u = (size_t)j * m.ni + i;
@@ -364,18 +377,18 @@ __global__ void __launch_bounds__(256) pcg_spmv(MeshP m, const uint8_t* __restri
 // 7242 PCG iterations with Jacobi, 4048 with one sweep and 2623 with three, and the sweeps stay
 // inside LDS.
 constexpr int SGS_SWEEPS = 3;
-// Thread t owns column t % PT_W and rows t / PT_W + 4 s of the tile.
+// Thread t owns column t % PT_W and rows t / PT_W + PT_RSTEP s of the tile.
 struct TileRows {
-    float Dv[4], cE[4], cW[4], cN[4], cS[4];
-    uint8_t c[4];
-    bool in[4];
+    float Dv[PT_RPT], cE[PT_RPT], cW[PT_RPT], cN[PT_RPT], cS[PT_RPT];
+    uint8_t c[PT_RPT];
+    bool in[PT_RPT];
 };
 __device__ __forceinline__ void tile_rows(const MeshP& m, const uint8_t* __restrict__ cls, const float* __restrict__ D,
                                           int i0, int j0, TileRows& T) {
     const int li = threadIdx.x % PT_W;
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const int lj = threadIdx.x / PT_W + 4 * s;
+    for (int s = 0; s < PT_RPT; s++) {
+        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
         const int i = i0 + li, j = j0 + lj;
         T.in[s] = i < m.ni && j < m.nj;
         T.c[s] = CLS_NOT_FREE;
@@ -402,14 +415,14 @@ __device__ __forceinline__ void tile_rows(const MeshP& m, const uint8_t* __restr
 __device__ __forceinline__ void tile_sgs(const TileRows& T, int i0, int j0, const float* rs, float* zs) {
     const int li = threadIdx.x % PT_W;
 #pragma unroll
-    for (int s = 0; s < 4; s++) zs[(threadIdx.x / PT_W + 4 * s) * PT_W + li] = 0.0f;
+    for (int s = 0; s < PT_RPT; s++) zs[(threadIdx.x / PT_W + PT_RSTEP * s) * PT_W + li] = 0.0f;
     __syncthreads();
 #pragma unroll
     for (int h = 0; h < 2 * SGS_SWEEPS + 1; h++) {
         const int color = h & 1;   // red (i + j even), black, red, ...
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const int lj = threadIdx.x / PT_W + 4 * s;
+        for (int s = 0; s < PT_RPT; s++) {
+            const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
             if (((i0 + li + j0 + lj) & 1) != color || T.c[s] == CLS_NOT_FREE || !T.in[s]) continue;
             const int l = lj * PT_W + li;
             float acc = rs[l];
@@ -425,7 +438,7 @@ __device__ __forceinline__ void tile_sgs(const TileRows& T, int i0, int j0, cons
 
 // z = M^-1 res of the refinement residual and the (r.z, r.r) partials of iteration 0; clears the
 // PCG state word for the iterations queued behind it.
-__global__ void __launch_bounds__(256) pcg_precond(MeshP m, const uint8_t* __restrict__ cls,
+__global__ void __launch_bounds__(PT_THREADS) pcg_precond(MeshP m, const uint8_t* __restrict__ cls,
                                                    const float* __restrict__ D, const float* __restrict__ res,
                                                    float* __restrict__ z, double* __restrict__ pz0,
                                                    int* __restrict__ istate) {
@@ -439,18 +452,18 @@ __global__ void __launch_bounds__(256) pcg_precond(MeshP m, const uint8_t* __res
     const int li = threadIdx.x % PT_W;
     TileRows T;
     tile_rows(m, cls, D, i0, j0, T);
-    float rv[4];
+    float rv[PT_RPT];
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const int lj = threadIdx.x / PT_W + 4 * s;
+    for (int s = 0; s < PT_RPT; s++) {
+        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
         rv[s] = (T.in[s] && T.c[s] != CLS_NOT_FREE) ? res[(size_t)(j0 + lj) * m.ni + i0 + li] : 0.0f;
         rs[lj * PT_W + li] = rv[s];
     }
     tile_sgs(T, i0, j0, rs, zs);
     double a_rz = 0, a_rr = 0;
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const int lj = threadIdx.x / PT_W + 4 * s;
+    for (int s = 0; s < PT_RPT; s++) {
+        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
         if (!T.in[s]) continue;
         const float zv = zs[lj * PT_W + li];
         z[(size_t)(j0 + lj) * m.ni + i0 + li] = zv;
@@ -463,7 +476,7 @@ __global__ void __launch_bounds__(256) pcg_precond(MeshP m, const uint8_t* __res
 
 // step 2: alpha = rz_k / p.q; d += alpha p; res -= alpha q; z = M^-1 res; (r.z, r.r) partials
 // of iteration k + 1 into pz_next.
-__global__ void __launch_bounds__(256) pcg_update(MeshP m, const uint8_t* __restrict__ cls,
+__global__ void __launch_bounds__(PT_THREADS) pcg_update(MeshP m, const uint8_t* __restrict__ cls,
                                                   const float* __restrict__ D, const float* __restrict__ p,
                                                   const float* __restrict__ q, float* __restrict__ d,
                                                   float* __restrict__ res, float* __restrict__ z,
@@ -481,10 +494,10 @@ __global__ void __launch_bounds__(256) pcg_update(MeshP m, const uint8_t* __rest
     const int li = threadIdx.x % PT_W;
     TileRows T;
     tile_rows(m, cls, D, i0, j0, T);
-    float rv[4];
+    float rv[PT_RPT];
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const int lj = threadIdx.x / PT_W + 4 * s;
+    for (int s = 0; s < PT_RPT; s++) {
+        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
         rv[s] = 0.0f;
         if (T.in[s]) {
             const size_t u = (size_t)(j0 + lj) * m.ni + i0 + li;
@@ -499,8 +512,8 @@ __global__ void __launch_bounds__(256) pcg_update(MeshP m, const uint8_t* __rest
     tile_sgs(T, i0, j0, rs, zs);
     double a_rz = 0, a_rr = 0;
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const int lj = threadIdx.x / PT_W + 4 * s;
+    for (int s = 0; s < PT_RPT; s++) {
+        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
         if (!T.in[s]) continue;
         const float zv = zs[lj * PT_W + li];
         z[(size_t)(j0 + lj) * m.ni + i0 + li] = zv;
@@ -560,18 +573,24 @@ __device__ __forceinline__ int nk_decide(const NkState& S, const NkParams& np, c
     return NK_PCG;
 }
 
-__global__ void __launch_bounds__(256) nk_step_a(MeshP m, const uint8_t* __restrict__ cls,
+__global__ void __launch_bounds__(PT_THREADS) nk_step_a(MeshP m, const uint8_t* __restrict__ cls,
                                                  const float* __restrict__ D, const float* __restrict__ z,
                                                  float* __restrict__ p_even, float* __restrict__ p_odd,
                                                  float* __restrict__ q, double* __restrict__ x,
                                                  const float* __restrict__ d, const double* __restrict__ pz_even,
                                                  const double* __restrict__ pz_odd, int nbz,
                                                  double* __restrict__ ppq, const NkState* __restrict__ state,
-                                                 NkParams np, int i) {
+                                                 double* __restrict__ nkc, NkParams np, int i) {
     const NkState S = state[(i - 1) % NK_RING];
     if (S.done) return;
     double cur[2];
     combine_partials<2>((i - 1) & 1 ? pz_odd : pz_even, nbz, cur);
+    // every block combines the same partials in the same order; block 0 keeps the totals for
+    // nk_step_b of this pair and nk_step_a of the next one
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        nkc[2 * (i % NK_RING)] = cur[0];
+        nkc[2 * (i % NK_RING) + 1] = cur[1];
+    }
     int code;
     const int act = nk_decide(S, np, cur, code);
     if (act == NK_STOP) return;
@@ -580,7 +599,7 @@ __global__ void __launch_bounds__(256) nk_step_a(MeshP m, const uint8_t* __restr
     if (act == NK_UPDATE) {
         // x += limited d on the tile's free nodes
         const int li = threadIdx.x % PT_W;
-        for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += 256 / PT_W) {
+        for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += PT_RSTEP) {
             const int ii = bx * PT_W + li, jj = by * PT_H + lj;
             if (ii >= m.ni || jj >= m.nj) continue;
             const size_t u = (size_t)jj * m.ni + ii;
@@ -595,11 +614,7 @@ __global__ void __launch_bounds__(256) nk_step_a(MeshP m, const uint8_t* __restr
         return;
     }
     float beta = 0.0f;
-    if (S.k > 0) {
-        double prev[2];
-        combine_partials<2>(i & 1 ? pz_odd : pz_even, nbz, prev);   // pair i-2's partials
-        beta = (float)(cur[0] / prev[0]);
-    }
+    if (S.k > 0) beta = (float)(cur[0] / nkc[2 * ((i - 1) % NK_RING)]);   // r.z of pair i-2's partials
     const float* pold = (i - 1) & 1 ? p_odd : p_even;
     float* pnew = i & 1 ? p_odd : p_even;
     __shared__ float ps[(PT_H + 2) * (PT_W + 2)];
@@ -616,7 +631,7 @@ __global__ void __launch_bounds__(256) nk_step_a(MeshP m, const uint8_t* __restr
     __syncthreads();
     const int li = threadIdx.x % PT_W;
     double acc = 0;
-    for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += 256 / PT_W) {
+    for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += PT_RSTEP) {
         const int ii = i0 + 1 + li, jj = j0 + 1 + lj;
         if (ii >= m.ni || jj >= m.nj) continue;
         const size_t u = (size_t)jj * m.ni + ii;
@@ -641,7 +656,7 @@ __global__ void __launch_bounds__(256) nk_step_a(MeshP m, const uint8_t* __restr
 }
 
 // init = 1: the residual branch unconditionally (the state of pair 0, written to state[0]).
-__global__ void __launch_bounds__(256) nk_step_b(MeshP m, const uint8_t* __restrict__ cls,
+__global__ void __launch_bounds__(PT_THREADS) nk_step_b(MeshP m, const uint8_t* __restrict__ cls,
                                                  const double* __restrict__ W, const double* __restrict__ bfix,
                                                  const double* __restrict__ rho, const double* __restrict__ x,
                                                  double* __restrict__ r, float* __restrict__ D,
@@ -650,14 +665,18 @@ __global__ void __launch_bounds__(256) nk_step_b(MeshP m, const uint8_t* __restr
                                                  const float* __restrict__ p_even, const float* __restrict__ p_odd,
                                                  const float* __restrict__ q, const double* __restrict__ ppq, int nbq,
                                                  double* __restrict__ pz_even, double* __restrict__ pz_odd, int nbz,
-                                                 NkState* __restrict__ state, NkParams np, double inv_eps, double Cn0,
-                                                 double phi0, double inv_kTe, int i, int init) {
+                                                 NkState* __restrict__ state, const double* __restrict__ nkc,
+                                                 NkParams np, double inv_eps, double Cn0, double phi0, double inv_kTe,
+                                                 int i, int init) {
     NkState S{0.0, 0.0, 0, 0, 0, 0};
     int act = NK_UPDATE;   // init: compute the residual
     double cur[2] = {0.0, 0.0};
     if (!init) {
         S = state[(i - 1) % NK_RING];
-        if (!S.done) combine_partials<2>((i - 1) & 1 ? pz_odd : pz_even, nbz, cur);
+        if (!S.done) {   // the totals nk_step_a of this pair combined
+            cur[0] = nkc[2 * (i % NK_RING)];
+            cur[1] = nkc[2 * (i % NK_RING) + 1];
+        }
         int code;
         act = nk_decide(S, np, cur, code);
         if (act == NK_STOP) {
@@ -683,13 +702,13 @@ __global__ void __launch_bounds__(256) nk_step_b(MeshP m, const uint8_t* __restr
     const int nbx = (m.ni + PT_W - 1) / PT_W;
     const int i0 = (blockIdx.x % nbx) * PT_W, j0 = (blockIdx.x / nbx) * PT_H;
     const int li = threadIdx.x % PT_W;
-    double rr64[4] = {0.0, 0.0, 0.0, 0.0};
-    float rv[4];
+    double rr64[PT_RPT] = {};
+    float rv[PT_RPT];
     if (act == NK_UPDATE) {
         // FP64 Newton residual and Jacobian diagonal of the tile (the newton_residual kernel's rows)
 #pragma unroll
-        for (int s4 = 0; s4 < 4; s4++) {
-            const int lj = threadIdx.x / PT_W + 4 * s4;
+        for (int s4 = 0; s4 < PT_RPT; s4++) {
+            const int lj = threadIdx.x / PT_W + PT_RSTEP * s4;
             const int ii = i0 + li, jj = j0 + lj;
             rv[s4] = 0.0f;
             if (ii < m.ni && jj < m.nj) {
@@ -724,8 +743,8 @@ __global__ void __launch_bounds__(256) nk_step_b(MeshP m, const uint8_t* __restr
     tile_rows(m, cls, D, i0, j0, T);
     if (act == NK_PCG) {
 #pragma unroll
-        for (int s4 = 0; s4 < 4; s4++) {
-            const int lj = threadIdx.x / PT_W + 4 * s4;
+        for (int s4 = 0; s4 < PT_RPT; s4++) {
+            const int lj = threadIdx.x / PT_W + PT_RSTEP * s4;
             rv[s4] = 0.0f;
             if (T.in[s4]) {
                 const size_t u = (size_t)(j0 + lj) * m.ni + i0 + li;
@@ -741,8 +760,8 @@ __global__ void __launch_bounds__(256) nk_step_b(MeshP m, const uint8_t* __restr
     tile_sgs(T, i0, j0, rs, zs);
     double a_rz = 0, a_rr = 0;
 #pragma unroll
-    for (int s4 = 0; s4 < 4; s4++) {
-        const int lj = threadIdx.x / PT_W + 4 * s4;
+    for (int s4 = 0; s4 < PT_RPT; s4++) {
+        const int lj = threadIdx.x / PT_W + PT_RSTEP * s4;
         if (!T.in[s4]) continue;
         const float zv = zs[lj * PT_W + li];
         z[(size_t)(j0 + lj) * m.ni + i0 + li] = zv;
@@ -1025,12 +1044,14 @@ int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* dsc) {
         if (!e) e = hipHostMalloc((void**)&P->host_scal, S_NSCAL * 8 * 2);
         if (!e) e = hipHostMalloc((void**)&P->host_istate, 16);
         if (!e) e = A((void**)&P->nk, NK_RING * sizeof(NkState));
+        if (!e) e = A((void**)&P->nkc, NK_RING * 2 * sizeof(double));
         if (!e) e = hipHostMalloc((void**)&P->host_nk, sizeof(NkState));
         if (!e) e = hipEventCreate(&P->e0);
         if (!e) e = hipEventCreate(&P->e1);
         if (e) return sf_fail(ctx, SF_ERR_OOM, "poisson allocation failed: %s", hipGetErrorString(e));
         hipMemset(P->istate, 0, 16);
         hipMemset(P->nk, 0, NK_RING * sizeof(NkState));
+        hipMemset(P->nkc, 0, NK_RING * 2 * sizeof(double));
         P->host_newton = getenv("SF_HOST_NEWTON") != nullptr;
         hipMemset(P->pz, 0, (size_t)4 * std::max(P->nblk_elem, nb_tile) * 8);
         hipMemset(P->ppq, 0, (size_t)P->nblk_tile * 8);
@@ -1104,7 +1125,6 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     double rel = 0, rnorm = 0, ref = 1.0;
     bool converged = false;
     const bool debug = getenv("SF_DEBUG_SOLVE") != nullptr;
-    const char* eta_env0 = getenv("SF_NEWTON_ETA");
     const double lim0 = D.linear ? 0.0 : D.Te0;
     // A warm nonlinear solve (from near its solution) runs as device-driven Newton-PCG: the pairs
     // of step kernels are queued in batches and the host reads one state word per batch.  Returns
@@ -1114,20 +1134,20 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         const double ee = std::max(eta_d, 1e-5);
         NkParams np{(const double*)(P->scal + S_R0), rel_tol * rel_tol, ee * ee, lim0, max_it, nl_max};
         auto step_b = [&](int i, int init) {
-            hipLaunchKernelGGL(nk_step_b, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
+            hipLaunchKernelGGL(nk_step_b, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
                                (const double*)P->bfix, (const double*)ctx->d_rho, (const double*)P->x, P->r, P->D,
                                P->P, P->d, P->res, P->z, (const float*)P->p0, (const float*)P->p1,
-                               (const float*)P->q, (const double*)P->ppq, gt, pz0, pz1, gt, P->nk, np, inv_eps, Cn0,
-                               D.phi0, inv_kTe, i, init);
+                               (const float*)P->q, (const double*)P->ppq, gt, pz0, pz1, gt, P->nk, (const double*)P->nkc,
+                               np, inv_eps, Cn0, D.phi0, inv_kTe, i, init);
         };
         step_b(0, 1);
         int i = 1, batch = std::min(64, std::max(8, P->last_pairs + 2));
         while (true) {
             for (int b = 0; b < batch; b++, i++) {
-                hipLaunchKernelGGL(nk_step_a, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+                hipLaunchKernelGGL(nk_step_a, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls,
                                    (const float*)P->D, (const float*)P->z, P->p0, P->p1, P->q, P->x,
                                    (const float*)P->d, (const double*)pz0, (const double*)pz1, gt, P->ppq,
-                                   (const NkState*)P->nk, np, i);
+                                   (const NkState*)P->nk, P->nkc, np, i);
                 step_b(i, 0);
             }
             SF_HIP(ctx, hipGetLastError());
@@ -1158,7 +1178,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     // loop (it hands a cold start back untouched)
     bool device_done = false;
     if (device_ok && P->warm_hint) {
-        const int code = device_newton(eta_env0 ? atof(eta_env0) : 1e-2);
+        const int code = device_newton(ETA_WARM);
         if (code < 0) return code;
         device_done = code != NK_COLD;
     }
@@ -1168,13 +1188,8 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     double ref2 = P->host_scal[S_R0];
     ref = std::sqrt(ref2);
     if (!(ref > 0)) ref = 1.0;
-    // Forcing term of the inexact Newton steps: 1e-3 for a solve that starts far from its solution
-    // (first residual at least half the reference residual, as from phi = 0), 1e-2 for a solve
-    // warm-started from the previous step.  Measured on the cylinder deck: the warm solve takes
-    // 1.00 instead of 1.06 ms with 1e-2, while a cold solve with 1e-2 takes 117 instead of 70 ms.
-    const char* eta_env = getenv("SF_NEWTON_ETA");
-    double eta = eta_env ? atof(eta_env) : 1e-3;
-    bool eta_chosen = eta_env != nullptr;
+    double eta = ETA_COLD;
+    bool eta_chosen = false;
     // y is zero at the start of every Newton iteration (newton_update clears it), so the first
     // refinement residual r - J y is the Newton residual itself: one read serves both tests.
     SF_HIP(ctx, hipMemsetAsync(P->y, 0, nn * sizeof(double), st));
@@ -1207,7 +1222,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         if (nl_stop) { converged = true; break; }    // the reference's criterion met on the last step
         if (it == nl_max) break;
         if (!eta_chosen) {
-            eta = rel < 0.5 ? 1e-2 : 1e-3;
+            eta = rel < 0.5 ? ETA_WARM : ETA_COLD;
             eta_chosen = true;
         }
         if (it == 0 && rel < 0.5 && device_ok) {
@@ -1235,7 +1250,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             // inner FP32 tolerance: a relative reduction FP32 can deliver
             double inner_tol = std::max(lin_target, 1e-5 * rr);
             const double tol2 = inner_tol * inner_tol;
-            hipLaunchKernelGGL(pcg_precond, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const float*)P->D,
+            hipLaunchKernelGGL(pcg_precond, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls, (const float*)P->D,
                                (const float*)P->res, P->z, pz0, P->istate);
             // When this linear solve is the Newton step's last (it aims at the step's own target),
             // the update and the next Newton residual are queued behind the first batch of PCG
@@ -1253,10 +1268,10 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
                 for (int b = 0; b < nb; b++, k++) {
                     double* cur = (k & 1) ? pz1 : pz0;
                     double* nxt = (k & 1) ? pz0 : pz1;
-                    hipLaunchKernelGGL(pcg_spmv, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+                    hipLaunchKernelGGL(pcg_spmv, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls,
                                        (const float*)P->D, (const float*)P->z, (const float*)pa, pb, P->q,
                                        (const double*)cur, (const double*)nxt, gt, k, P->ppq, tol2, P->istate);
-                    hipLaunchKernelGGL(pcg_update, dim3(gt), dim3(256), 0, st, m, (const uint8_t*)P->cls,
+                    hipLaunchKernelGGL(pcg_update, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls,
                                        (const float*)P->D, (const float*)pb, (const float*)P->q, P->d, P->res,
                                        P->z, (const double*)P->ppq, gt, (const double*)cur, gt, nxt,
                                        (const int*)P->istate);

@@ -652,27 +652,29 @@ __global__ void __launch_bounds__(PUSH_THREADS, 4) push_tile_fast_xy(FastC fc, c
     const int total = L.seg_pre[TILE_H];
     const int sg0 = L.seg_start[0], sg1 = L.seg_start[1], sg2 = L.seg_start[2], sg3 = L.seg_start[3];
     const int pr1 = L.seg_pre[1], pr2 = L.seg_pre[2], pr3 = L.seg_pre[3];
-    auto slot_of = [&](int k) -> int64_t {
-        int b = sg0, pr = 0;
-        if (k >= pr1) { b = sg1; pr = pr1; }
-        if (k >= pr2) { b = sg2; pr = pr2; }
-        if (k >= pr3) { b = sg3; pr = pr3; }
-        return (int64_t)b + (k - pr);
+    // The slot and the loaded state are computed as values and assigned whole: selects or
+    // conditional stores over the addresses of locals or struct fields would put them in
+    // scratch memory, and its loads wait for every outstanding global store.
+    const int d1 = sg1 - pr1 - sg0, d2 = (sg2 - pr2) - (sg1 - pr1), d3 = (sg3 - pr3) - (sg2 - pr2);
+    auto slot_of = [sg0, d1, d2, d3, pr1, pr2, pr3](int k) -> int64_t {
+        // segment starts minus prefixes telescope: pr1 <= pr2 <= pr3
+        const int b = sg0 + ((k >= pr1 ? -1 : 0) & d1) + ((k >= pr2 ? -1 : 0) & d2) + ((k >= pr3 ? -1 : 0) & d3);
+        return (int64_t)b + k;
     };
     auto load = [&](PReg& r, int k) {
-        if (k < total) {
-            r.g = slot_of(k);
-            r.c = s.cell[r.g];
-            r.fx = s.fx[r.g];
-            r.fy = s.fy[r.g];
-            r.vx = s.vx[r.g];
-            r.vy = s.vy[r.g];
-            r.vz = s.vz[r.g];
-        } else {
-            r.g = 0;
-            r.c = -1;
-            r.fx = r.fy = r.vx = r.vy = r.vz = 0.0f;
+        const bool in = k < total;
+        const int64_t g = in ? slot_of(k) : 0;
+        int32_t c = -1;
+        float fx = 0.f, fy = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+        if (in) {
+            c = s.cell[g];
+            fx = s.fx[g];
+            fy = s.fy[g];
+            vx = s.vx[g];
+            vy = s.vy[g];
+            vz = s.vz[g];
         }
+        r = PReg{g, c, fx, fy, vx, vy, vz};
     };
     // per-lane mover sums: raw FMA bit patterns, the bias removed with the lane's count at the end
     long long sn = 0;

@@ -49,8 +49,11 @@ def _synthetic_sample(rng, H, W):
 @pytest.mark.parametrize("ni,nj", [(37, 21), (70, 33)])
 def test_layers_output_and_gradients_match_oracle(ctx, ni, nj):
     """Every convolution, pooling, upsampling and the head are checked against the oracle on the
-    device's own input activations (differences then come only from FP32 vs FP64 accumulation
-    order: norm-wise 1e-4); the backward pass is checked the same way from the device's forward
+    device's own input activations.  Differences then come only from FP32 vs FP64 accumulation
+    order: an output whose FP32 and FP64 sums straddle a BF16 rounding boundary rounds one BF16
+    step (2^-8 relative) apart.  So each differing value must be within one step, at most 1e-3 of
+    the values may differ, and norm-wise the error stays below 3e-4 (a few such values in these
+    small tensors give about 1e-4).  The backward pass is checked the same way from the device's forward
     activations (1e-2: a rounding difference of one BF16 gradient propagates through the
     remaining layers).  End to end the output stays within 2e-2 of the oracle's own forward
     pass: BF16 rounding flips amplify with depth, while a wrong index, tap or operand gives
@@ -83,7 +86,10 @@ def test_layers_output_and_gradients_match_oracle(ctx, ni, nj):
             ref = N.conv_layer(f64, name, x)
             e = _rel(a, ref)
             print("%-4s forward rel %.2e, BF16 values differing %.2e" % (name, e, np.mean(a != ref)), flush=True)
-            assert e < 1e-4, (name, e)
+            step = 2.0 ** -7 * np.maximum(np.abs(a), np.abs(ref)) + 1e-6 * np.abs(ref).max()
+            assert np.all(np.abs(a - ref) <= step), name
+            assert np.mean(a != ref) < 1e-3, name
+            assert e < 3e-4, (name, e)
         # pooling, upsampling and the concatenations
         e1, e2 = acts["c1b"][1], acts["c2b"][1]
         assert np.array_equal(acts["c2a"][0], N.bf16(N.pool(e1)))
