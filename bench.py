@@ -57,6 +57,10 @@ def parse():
                     help="C2-style runs (random cylinder radius / bias) that provide training snapshots")
     ap.add_argument("--learned-train-steps", type=int, default=600)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--workload", choices=["c2", "c3", "mcc", "dsmc"], default="c2",
+                    help="c2: the benchmark (default); c3: the RZ push at 2048x1024 with 1e8 Xe+; mcc: c3 with "
+                         "charge exchange every step; dsmc: 5e7 N2 with NTC collisions every step (these three print "
+                         "their own JSON line and use --steps / --warmup)")
     ap.add_argument("--cnn-only", action="store_true",
                     help="time only the learned guess's network (training steps at 1024x512, batch 4, on "
                          "synthetic samples) and print that JSON line")
@@ -106,6 +110,84 @@ def cnn_throughput(ctx, ni, nj, batch=4, steps=20):
         net.close()
 
 
+def other_workload(args):
+    """The other configurations' hot kernels on one GPU: c3 (RZ 2048x1024 nodes, 1e8 Xe+ preloaded,
+    push + deposit + sort every 4 steps + Boltzmann-electron solve), mcc (c3 plus charge exchange on
+    a 1e19 m^-3 Xe background every step) and dsmc (512x256 cells, 5e7 N2, free flight + sort +
+    NTC collisions every step).  Device time of the particle phase from HIP events, the MCC / DSMC
+    pass from the host clock around a synchronised call."""
+    import ctypes as C
+    import numpy as np
+    from starfish_amd import deck as D
+    from starfish_amd.plugin import Simulation
+    out = {"workload": args.workload, "steps": args.steps, "warmup": args.warmup, "n_gpus": 1}
+    if args.workload in ("c3", "mcc"):
+        deck = D.config_c3(particles=100_000_000)
+        sim = Simulation(deck, capacity=110_000_000, track_ids=args.workload == "mcc")
+    else:
+        mesh = D.MeshSpec("XY", 513, 257, (0.0, 0.0), (1e-4, 1e-4), {})
+        n = 50_000_000
+        spwt = 1e22 * 512 * 256 * 1e-8 / n
+        deck = D.Deck(mesh, materials=[D.MaterialSpec("N2", "kinetic_hip", 28.0, 0.0, spwt, 0.0)], solver=None,
+                      num_it=1, dt=1e-8, seed=0)
+        sim = Simulation(deck, capacity=n + 1_000_000, track_ids=True)
+    try:
+        mat = sim.materials[0]
+        coll = None
+        if args.workload in ("c3", "mcc"):
+            npart, drift, temp = deck.preload
+            mat.preload(npart, drift, temp, seed=3)
+            sim.initial_solve()
+            if args.workload == "mcc":
+                from starfish_amd.interactions import MccHip
+                coll = MccHip(sim, mat, "CEX", 5e-19, 131.0, seed=17)
+                coll.set_target(np.full(sim.nn, 1e19), temp=np.full(sim.nn, 300.0))
+        else:
+            from starfish_amd.interactions import DsmcHip
+            mat.preload(50_000_000, (500.0, 0.0, 0.0), 300.0, seed=5)
+            coll = DsmcHip(sim, mat, mat, sig_cr_max=1e-16, seed=3, sigma="bird463", diam=(4.17e-10, 4.17e-10),
+                           ref_temp=(273.0, 273.0), visc_temp_index=(0.74, 0.74))
+        sim.ctx.call("sf_set_timing", 1)
+        pushed, kern, coll_ms, ncoll, solve_ms = 0, np.zeros(4), 0.0, 0, []
+        for step in range(args.warmup + args.steps):
+            timed = step >= args.warmup
+            np_step = mat.getNp()
+            mat.update()
+            tm = np.zeros(4)
+            sim.ctx.call("sf_last_timings", tm.ctypes.data_as(C.POINTER(C.c_double)))
+            if coll is not None:
+                sim.ctx.call("sf_ctx_synchronize")
+                t0 = time.perf_counter()
+                nc = coll.perform(sim.it + 1, 0.0) if args.workload == "mcc" else coll.perform(sim.it + 1)
+                sim.ctx.call("sf_ctx_synchronize")
+                if timed:
+                    coll_ms += (time.perf_counter() - t0) * 1e3
+                    ncoll += nc
+            if deck.solver is not None:
+                sim.update_fields()
+                if timed:
+                    solve_ms.append(sim.solve_infos[-1].ms)
+            sim.it += 1
+            if timed:
+                pushed += np_step
+                kern += tm
+        phase_s = kern.sum() * 1e-3
+        out.update({"particles_per_step": pushed / args.steps, "particles_per_s": pushed / phase_s,
+                    "particle_phase_ms": {"push": kern[0] / args.steps, "scan": kern[1] / args.steps,
+                                          "scatter": kern[2] / args.steps, "normalize": kern[3] / args.steps},
+                    "push_GBs": 48.0 * pushed / (kern[0] * 1e-3) / 1e9})
+        if solve_ms:
+            out["solve_ms"] = float(np.mean(solve_ms))
+        if coll is not None:
+            out.update({"collision_pass_ms": coll_ms / args.steps, "collisions_per_step": ncoll / args.steps,
+                        "particles_per_s_collision_pass": pushed / (coll_ms * 1e-3)})
+        return out
+    finally:
+        if coll is not None:
+            coll.close()
+        sim.close()
+
+
 def learned_guess(sim, deck, args, rho0):
     """Trains the learned initial guess on snapshots of C2-style runs (cylinder radius
     U(0.02, 0.1) m, bias U(-200, 0) V; none is the benchmark's cylinder).  Reported: (1) the cold
@@ -122,7 +204,7 @@ def learned_guess(sim, deck, args, rho0):
     net = LearnedGuess(sim.ctx, max_batch=4, seed=1)
     try:
         generate_c2_snapshots(net, args.learned_configs, 24, 3, int(args.particles), m.ni, m.nj, seed=12345,
-                              skip=3)
+                              skip=3, initial=True)
         t_gen = time.perf_counter() - t0
         t1 = time.perf_counter()
         loss = 0.0
@@ -361,6 +443,9 @@ def main():
     from starfish_amd import deck as D
     from starfish_amd.plugin import Simulation
 
+    if args.workload != "c2":
+        print(json.dumps(other_workload(args)), flush=True)
+        return
     if args.cnn_only:
         from starfish_amd.plugin import Context
         ctx = Context(0)

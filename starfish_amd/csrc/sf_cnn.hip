@@ -41,6 +41,11 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
     memcpy(&r, &v, 16);
     return r;
 }
+// u where ok, else zero: a select on the values (a select between the addresses of two locals
+// would keep them in scratch memory)
+__device__ __forceinline__ uint4 sel4(bool ok, const uint4 u) {
+    return make_uint4(ok ? u.x : 0u, ok ? u.y : 0u, ok ? u.z : 0u, ok ? u.w : 0u);
+}
 __device__ __forceinline__ bf16x8 zero8() {
     uint4 v = make_uint4(0, 0, 0, 0);
     bf16x8 r;
@@ -66,14 +71,14 @@ struct FView {
     int cs, co;
 };
 
-// The operands are staged in LDS.  One workgroup: CONV_TP pixels of one row
-// x TN output channels (TN = 64: waves 2 x 2 over pixels x channels; TN = 32: 4 x 1).  For each
-// chunk of 32 input channels the 3 x (CONV_TP + 2) input pixels and the 9 x TN x 32 weights go to
-// LDS once, and every wave runs its 16 x 16 tiles over the 9 taps from there: an input fragment is
-// read once per tap for two channel tiles, a weight fragment once per tap for all the wave's pixel
-// tiles.  The next chunk's operands are loaded into registers while the current one is multiplied.
+// The operands are staged in LDS.  One workgroup: R rows of CONV_TP pixels x TN output channels
+// (TN = 64: waves 2 x 2 over pixels x channels; TN = 32: 4 x 1).  For each chunk of 32 input
+// channels the R + 2 input rows of CONV_TP + 2 pixels and the 9 x TN x 32 weights go to LDS once,
+// and every wave runs its 16 x 16 tiles of all R rows over the 9 taps from there: a weight fragment
+// is read once per tap for R x PT pixel tiles, an input fragment once for two channel tiles.  The
+// next chunk's operands are loaded into registers while the current one is multiplied.
 constexpr int CONV_TP = 128;
-template <int MODE, int TN>
+template <int MODE, int TN, int R>
 __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ in, int in_cs, int in_co, int K,
                                                    const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                                    int N, uint16_t* __restrict__ out, float* __restrict__ fout,
@@ -81,48 +86,44 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
                                                    int accumulate) {
     constexpr int WN = TN / 32;                 // waves along the channels (two 16-channel tiles each)
     constexpr int WP = 4 / WN;                  // waves along the pixels
-    constexpr int PT = CONV_TP / 16 / WP;       // 16-pixel tiles per wave
+    constexpr int PT = CONV_TP / 16 / WP;       // 16-pixel tiles per wave and row
     constexpr int IW = CONV_TP + 2;             // staged pixels per row
-    constexpr int NIN = 3 * IW * 4;             // 16-byte groups of the input stage (8 channels each)
+    constexpr int NIN = (R + 2) * IW * 4;       // 16-byte groups of the input stage (8 channels each)
     constexpr int NWT = 9 * TN * 4;             // 16-byte groups of the weight stage
     constexpr int RIN = (NIN + 255) / 256, RWT = (NWT + 255) / 256;
     __shared__ uint4 s_in[NIN];                 // [row][pixel][group]
     __shared__ uint4 s_w[NWT];                  // [tap][channel][group]
     const int ntn = (N + TN - 1) / TN;
-    const int x0 = blockIdx.x * CONV_TP, y = blockIdx.y;
+    const int x0 = blockIdx.x * CONV_TP, y0 = blockIdx.y * R;
     const int b = blockIdx.z / ntn, n0 = (blockIdx.z % ntn) * TN;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wp = wave % WP, wn = wave / WP;
     const int prow = lane & 15, kq = lane >> 4;
-    const uint4 zero = make_uint4(0, 0, 0, 0);
     uint4 rin[RIN], rwt[RWT];
-    auto fetch = [&](int k0) {
+    auto fetch = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
         for (int q = 0; q < RIN; q++) {
             const int e = threadIdx.x + 256 * q;
-            rin[q] = zero;
-            if (e < NIN) {
-                const int g = e & 3, px = (e >> 2) % IW, r = (e >> 2) / IW;
-                const int yy = y - 1 + r, xx = x0 - 1 + px;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
-                    rin[q] = *reinterpret_cast<const uint4*>(in + (((size_t)b * H + yy) * Wd + xx) * in_cs + in_co +
-                                                             k0 + g * 8);
-            }
+            const int g = e & 3, px = (e >> 2) % IW, r = (e >> 2) / IW;
+            const int yy = y0 - 1 + r, xx = x0 - 1 + px;
+            const bool ok = e < NIN && yy >= 0 && yy < H && xx >= 0 && xx < Wd;
+            const size_t o = ok ? (((size_t)b * H + yy) * Wd + xx) * in_cs + in_co + k0 + g * 8 : 0;
+            rin[q] = sel4(ok, *reinterpret_cast<const uint4*>(in + o));
         }
 #pragma unroll
         for (int q = 0; q < RWT; q++) {
             const int e = threadIdx.x + 256 * q;
-            rwt[q] = zero;
-            if (e < NWT) {
-                const int g = e & 3, n = (e >> 2) % TN, tap = (e >> 2) / TN;
-                if (n0 + n < N)
-                    rwt[q] = *reinterpret_cast<const uint4*>(w + ((size_t)tap * N + n0 + n) * K + k0 + g * 8);
-            }
+            const int g = e & 3, n = (e >> 2) % TN, tap = (e >> 2) / TN;
+            const bool ok = e < NWT && n0 + n < N;
+            const size_t o = ok ? ((size_t)tap * N + n0 + n) * K + k0 + g * 8 : 0;
+            rwt[q] = sel4(ok, *reinterpret_cast<const uint4*>(w + o));
         }
     };
-    f32x4 acc[PT][2];
+    f32x4 acc[R][PT][2];
 #pragma unroll
-    for (int p = 0; p < PT; p++) acc[p][0] = acc[p][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int p = 0; p < PT; p++) acc[r][p][0] = acc[r][p][1] = f32x4{0.f, 0.f, 0.f, 0.f};
     fetch(0);
     for (int k0 = 0; k0 < K; k0 += 32) {
         __syncthreads();   // the previous chunk's fragments are read
@@ -134,7 +135,9 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
             if (threadIdx.x + 256 * q < NWT) s_w[threadIdx.x + 256 * q] = rwt[q];
         __syncthreads();
         if (k0 + 32 < K) fetch(k0 + 32);
-#pragma unroll
+        // (taps not unrolled: the R x PT x 2 tiles of a tap are, and the accumulators stay in
+        // registers)
+#pragma unroll 1
         for (int tap = 0; tap < 9; tap++) {
             const int dy = sgn * (tap / 3 - 1), dx = sgn * (tap % 3 - 1);
             bf16x8 bw[2];
@@ -144,38 +147,45 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
                 memcpy(&bw[t], &v, 16);
             }
 #pragma unroll
-            for (int p = 0; p < PT; p++) {
-                const int px = (wp * PT + p) * 16 + prow;
-                const uint4 v = s_in[((1 + dy) * IW + px + 1 + dx) * 4 + kq];
-                bf16x8 a;
-                memcpy(&a, &v, 16);
-                acc[p][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[0], acc[p][0], 0, 0, 0);
-                acc[p][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[1], acc[p][1], 0, 0, 0);
-            }
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int p = 0; p < PT; p++) {
+                    const int px = (wp * PT + p) * 16 + prow;
+                    const uint4 v = s_in[((r + 1 + dy) * IW + px + 1 + dx) * 4 + kq];
+                    bf16x8 a;
+                    memcpy(&a, &v, 16);
+                    acc[r][p][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[0], acc[r][p][0], 0, 0, 0);
+                    acc[r][p][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[1], acc[r][p][1], 0, 0, 0);
+                }
         }
     }
     // D layout: lane holds rows (lane/16)*4 + i (pixels), column lane%16 (channel)
 #pragma unroll
-    for (int p = 0; p < PT; p++)
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + r;
+        if (y >= H) continue;
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-            const int n = n0 + wn * 32 + t * 16 + prow;
-            if (n >= N) continue;
+        for (int p = 0; p < PT; p++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int xo = x0 + (wp * PT + p) * 16 + (lane >> 4) * 4 + i;
-                if (xo >= Wd) continue;
-                const size_t o = (((size_t)b * H + y) * Wd + xo) * out_cs + out_co + n;
-                float v = acc[p][t][i];
-                if (MODE == 0) {
-                    v += bias[n];
-                    if (relu) v = fmaxf(v, 0.f);
-                    out[o] = f2bf(v);
-                } else {
-                    fout[o] = accumulate ? fout[o] + v : v;
+            for (int t = 0; t < 2; t++) {
+                const int n = n0 + wn * 32 + t * 16 + prow;
+                if (n >= N) continue;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int xo = x0 + (wp * PT + p) * 16 + (lane >> 4) * 4 + i;
+                    if (xo >= Wd) continue;
+                    const size_t o = (((size_t)b * H + y) * Wd + xo) * out_cs + out_co + n;
+                    float v = acc[r][p][t][i];
+                    if (MODE == 0) {
+                        v += bias[n];
+                        if (relu) v = fmaxf(v, 0.f);
+                        out[o] = f2bf(v);
+                    } else {
+                        fout[o] = accumulate ? fout[o] + v : v;
+                    }
                 }
             }
-        }
+    }
 }
 
 // Weight gradient with the operands staged in LDS.  One workgroup: 32 input x 32 output channels
@@ -187,17 +197,22 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
 // fragment read start on distinct banks.  The next chunk's pixels are loaded into registers while
 // the current one is multiplied.  The workgroups of the first input-channel tile also sum the
 // gradient over the pixels (the bias gradient) with one more MFMA per step against an all-ones
-// operand.  Requirements: K and N multiples of 32, strides and offsets multiples of 8.
+// operand.  Each workgroup stores its sums as a partial (WG_PART floats: [tap][k][n], then the 32
+// bias sums); wgrad_reduce adds the partials of a channel tile pair in a fixed order, which is
+// deterministic and avoids thousands of workgroups' atomics on the same addresses.
+// Requirements: K and N multiples of 32, strides and offsets multiples of 8.
 constexpr int WG_PX = 64;           // pixels per chunk (two MFMA reduction steps)
 constexpr int WG_Q = WG_PX / 4 + 2; // four-pixel entries per staged row (with padding)
+constexpr int WG_PART = 9 * 32 * 32 + 32;   // floats of a workgroup's partial
+constexpr int WG_BLOCKS = 768;              // target workgroups of a weight-gradient launch (3 per CU)
 __device__ __forceinline__ uint32_t bf_at(const uint4& v, int c) {
     const uint32_t w = c < 2 ? v.x : (c < 4 ? v.y : (c < 6 ? v.z : v.w));
     return (c & 1) ? (w >> 16) : (w & 0xFFFFu);
 }
 __global__ void __launch_bounds__(256) conv3x3_wgrad_lds(const uint16_t* __restrict__ in, int in_cs, int in_co,
                                                          int K, const uint16_t* __restrict__ g, int g_cs, int N,
-                                                         float* __restrict__ dw, float* __restrict__ db, int H,
-                                                         int Wd, int B, int chunks_per_block) {
+                                                         float* __restrict__ part, int H, int Wd, int B,
+                                                         int chunks_per_block) {
     __shared__ uint2 s_in[3][3][32][WG_Q];   // [dx + 1][dy + 1][input channel][pixel / 4]
     __shared__ uint2 s_g[32][WG_Q];          // [output channel][pixel / 4]
     const int nkt = K / 32;
@@ -209,11 +224,12 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_lds(const uint16_t* __restr
     const int nchunks = B * H * cpr;
     const int c_begin = blockIdx.x * chunks_per_block;
     const int c_end = min(c_begin + chunks_per_block, nchunks);
-    // staging items: threads 0..191 one input item (row r, pixel quad q, channel group gk: pixels
-    // 4q - 1 .. 4q + 4 of 8 channels), threads 192..255 one gradient item (quad q, group gn)
+    // staging items: threads 0..191 one input item (row r, pixel quad q, channel group gq: pixels
+    // 4q - 1 .. 4q + 4 of 8 channels), threads 192..255 one gradient item (quad q, group gq).  The
+    // quad is the fastest index, so the 16 lanes of an LDS write phase cover all banks.
     const bool is_in = t < 192;
     const int it = is_in ? t : t - 192;
-    const int r = it / 64, q = (it % 64) / 4, gq = it % 4;
+    const int r = it / 64, q = it % 16, gq = (it / 16) % 4;
     uint4 v[6];
     // (the same loads for both kinds of item, so that v stays in registers)
     const uint16_t* const src = is_in ? in : g;
@@ -227,8 +243,7 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_lds(const uint16_t* __restr
             const int xx = x0 + 4 * q + sdx + j;
             const bool ok = j < sj && yy >= 0 && yy < H && xx >= 0 && xx < Wd;
             const size_t o = ok ? (((size_t)b * H + yy) * Wd + xx) * scs + sco : 0;
-            const uint4 u = *reinterpret_cast<const uint4*>(src + o);
-            v[j] = ok ? u : make_uint4(0, 0, 0, 0);
+            v[j] = sel4(ok, *reinterpret_cast<const uint4*>(src + o));
         }
     };
     f32x4 acc[9];
@@ -282,16 +297,43 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_lds(const uint16_t* __restr
         }
     }
     // D rows (lane/16)*4+i = input channel, column lane%16 = output channel
+    float* const pb = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * WG_PART;
 #pragma unroll
     for (int tap = 0; tap < 9; tap++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int k = k0 + tk * 16 + kq * 4 + i;
-            const int n = n0 + tn * 16 + prow;
-            atomicAdd(&dw[((size_t)tap * N + n) * K + k], acc[tap][i]);
-        }
-    // every row of accb holds the column sums: lanes 0..15 add row 0
-    if (do_bias && kq == 0) atomicAdd(&db[n0 + tn * 16 + prow], accb[0]);
+        for (int i = 0; i < 4; i++) pb[(tap * 32 + tk * 16 + kq * 4 + i) * 32 + tn * 16 + prow] = acc[tap][i];
+    // every row of accb holds the column sums: lanes 0..15 store row 0
+    if (tk == 0 && kq == 0) pb[9 * 32 * 32 + tn * 16 + prow] = do_bias ? accb[0] : 0.f;
+}
+
+// Sum of the weight-gradient partials of channel tile pair y: a workgroup takes 32 entries e;
+// slice j of its threads adds the partials b = j, j + 8, ... in order, and the 8 slice sums are
+// added in order (a fixed order, so the result does not depend on timing).  Entry e < 9 * 32 * 32
+// ([tap][k][n] of the tile) goes to dw, the rest (the bias sums, first input-channel tile only)
+// to db.
+__global__ void __launch_bounds__(256) wgrad_reduce(const float* __restrict__ part, int nblk, int K, int N,
+                                                    float* __restrict__ dw, float* __restrict__ db) {
+    __shared__ float red[8][32];
+    const int nkt = K / 32;
+    const int k0 = (blockIdx.y % nkt) * 32, n0 = (blockIdx.y / nkt) * 32;
+    const int el = threadIdx.x & 31, slice = threadIdx.x >> 5;
+    const int e = blockIdx.x * 32 + el;
+    float s = 0.f;
+    if (e < WG_PART) {
+        const float* p = part + (size_t)blockIdx.y * nblk * WG_PART + e;
+        for (int b = slice; b < nblk; b += 8) s += p[(size_t)b * WG_PART];
+    }
+    red[slice][el] = s;
+    __syncthreads();
+    if (slice != 0 || e >= WG_PART) return;
+    s = red[0][el];
+    for (int j = 1; j < 8; j++) s += red[j][el];
+    if (e < 9 * 32 * 32) {
+        const int tap = e / 1024, k = (e / 32) % 32, n = e % 32;
+        dw[((size_t)tap * N + n0 + n) * K + k0 + k] += s;
+    } else if (k0 == 0) {
+        db[n0 + e - 9 * 32 * 32] += s;
+    }
 }
 
 
@@ -508,6 +550,7 @@ struct sf_cnn {
     int adam_t = 0;
     std::mt19937_64 rng{1};
     double* phi_save = nullptr;       // [ni*nj] stash of a context's potential
+    float* wpart = nullptr;           // weight-gradient partials: (WG_BLOCKS + 128) x WG_PART
     std::vector<void*> allocs;
 };
 
@@ -565,15 +608,16 @@ template <int MODE>
 static void conv_launch(sf_cnn* c, const uint16_t* in, int in_cs, int in_co, int K, const uint16_t* w,
                         const float* bias, int N, uint16_t* out, float* fout, int out_cs, int out_co, int H, int W,
                         int B, int relu, int sgn, int accumulate) {
+    // rows per workgroup: 2 with 64-channel tiles, 4 with 32 (the same accumulator count)
     const bool t64 = N % 64 == 0;
-    const int tn = t64 ? 64 : 32;
-    const dim3 g((W + CONV_TP - 1) / CONV_TP, H, B * ((N + tn - 1) / tn));
+    const int tn = t64 ? 64 : 32, rows = t64 ? 2 : 4;
+    const dim3 g((W + CONV_TP - 1) / CONV_TP, (H + rows - 1) / rows, B * ((N + tn - 1) / tn));
     if (t64)
-        hipLaunchKernelGGL((conv3x3_lds<MODE, 64>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias, N,
-                           out, fout, out_cs, out_co, H, W, relu, sgn, accumulate);
+        hipLaunchKernelGGL((conv3x3_lds<MODE, 64, 2>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias,
+                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate);
     else
-        hipLaunchKernelGGL((conv3x3_lds<MODE, 32>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias, N,
-                           out, fout, out_cs, out_co, H, W, relu, sgn, accumulate);
+        hipLaunchKernelGGL((conv3x3_lds<MODE, 32, 4>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias,
+                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate);
 }
 // forward conv: in view (cs, co, K channels) -> out view, bias + ReLU
 static void conv_fwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs,
@@ -584,14 +628,17 @@ static void conv_fwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, i
 // backward of one conv: g (BF16 [npix][N]) -> weight / bias gradients and, if din != null, din
 static void conv_bwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, int in_co, const uint16_t* g,
                      float* din, int din_cs, int din_co, int H, int W, int B) {
-    // weight gradient: about 2048 workgroups over the (32 x 32 channel tile, pixel band) pairs
+    // weight gradient: about WG_BLOCKS workgroups over the (32 x 32 channel tile, pixel band)
+    // pairs, their partials summed in workgroup order
     const int nchunks = B * H * ((W + WG_PX - 1) / WG_PX);
     const int pairs = (L.K / 32) * (L.N / 32);
-    int bx = std::max(1, std::min(nchunks, (2048 + pairs - 1) / pairs));
+    int bx = std::max(1, std::min(nchunks, (WG_BLOCKS + pairs - 1) / pairs));
     const int cpb = (nchunks + bx - 1) / bx;
     bx = (nchunks + cpb - 1) / cpb;
     hipLaunchKernelGGL(conv3x3_wgrad_lds, dim3(bx, pairs), dim3(256), 0, c->ctx->stream, in, in_cs, in_co, L.K, g, L.N,
-                       L.N, L.gw, L.gb, H, W, B, cpb);
+                       L.N, c->wpart, H, W, B, cpb);
+    hipLaunchKernelGGL(wgrad_reduce, dim3((WG_PART + 31) / 32, pairs), dim3(256), 0, c->ctx->stream,
+                       (const float*)c->wpart, bx, L.K, L.N, L.gw, L.gb);
     if (din)
         conv_launch<1>(c, g, L.N, 0, L.N, (const uint16_t*)L.wt, (const float*)nullptr, L.K, (uint16_t*)nullptr, din,
                        din_cs, din_co, H, W, B, 0, -1, 0);
@@ -792,6 +839,9 @@ int sf_cnn_create(sf_ctx* ctx, int ni, int nj, int max_batch, uint64_t seed, sf_
     rc |= conv_init(c, c->c4b, C2, C2, c->rng);
     rc |= conv_init(c, c->c5a, C2 + C1, C1, c->rng);
     rc |= conv_init(c, c->c5b, C1, C1, c->rng);
+    // (a launch has at most ceil(WG_BLOCKS / pairs) * pairs <= WG_BLOCKS + pairs - 1 workgroups;
+    // the network's layers have at most 16 channel tile pairs)
+    rc |= cnn_alloc(c, &c->wpart, (size_t)(WG_BLOCKS + 128) * WG_PART);
     rc |= cnn_alloc(c, &c->hw, C1);
     rc |= cnn_alloc(c, &c->hb, 1);
     rc |= cnn_alloc(c, &c->ghw, C1);

@@ -131,10 +131,12 @@ def solve_from(sim, start, net=None):
     return info, rc
 
 
-def generate_c2_snapshots(net, n_configs, steps, every, particles, ni, nj, seed=0, skip=0, log=None):
+def generate_c2_snapshots(net, n_configs, steps, every, particles, ni, nj, seed=0, skip=0, log=None,
+                          initial=False):
     """Runs C2-style simulations (O+ beam past a cylinder of radius U(0.02, 0.1) m biased at
     U(-200, 0) V) and adds a snapshot of (rho, boundary, phi) every `every` steps after `skip`
-    steps.  Returns the list of (radius, bias) used."""
+    steps, and with `initial` also the solved state of the preloaded plasma before the first step
+    (the cold solve's case).  Returns the list of (radius, bias) used."""
     from . import deck as D
     from .plugin import Simulation
     rng = np.random.default_rng(seed)
@@ -147,6 +149,8 @@ def generate_c2_snapshots(net, n_configs, steps, every, particles, ni, nj, seed=
             n, drift, temp = deck.preload
             sim.materials[0].preload(n, drift, temp, seed=int(rng.integers(1 << 30)))
             sim.initial_solve()
+            if initial:
+                net.snapshot(sim)
             for s in range(steps):
                 sim.step()
                 if s >= skip and (s - skip) % every == 0:
