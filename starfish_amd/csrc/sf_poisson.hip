@@ -249,14 +249,19 @@ __global__ void newton_residual(MeshP m, const uint8_t* __restrict__ cls, const 
 }
 
 // ||f + bfix - W b(0)||^2 for the fixed reference norm
+// ||r_ref||^2 partials (the residual at x = 0 on the free nodes), and x gathered from phi (0 on
+// the nodes that are not free) in the same pass
 __global__ void ref_norm_kernel(MeshP m, const uint8_t* __restrict__ cls, const double* __restrict__ W,
                                 const double* __restrict__ bfix, const double* __restrict__ rho, double inv_eps,
                                 double Cn0, double phi0, double inv_kTe, int linear,
-                                double* __restrict__ partial) {
+                                double* __restrict__ partial, const double* __restrict__ phi,
+                                double* __restrict__ x) {
     const int64_t nn = (int64_t)m.ni * m.nj;
     double acc = 0;
     for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x) {
-        if (cls[u] == CLS_NOT_FREE) continue;
+        const bool free_node = cls[u] != CLS_NOT_FREE;
+        x[u] = free_node ? phi[u] : 0.0;
+        if (!free_node) continue;
         double b0 = linear ? 0.0 : Cn0 * exp((0.0 - phi0) * inv_kTe);
         double v = W[u] * rho[u] * inv_eps + bfix[u] - W[u] * b0;
         acc += v * v;
@@ -300,8 +305,8 @@ __global__ void refine_residual(MeshP m, const uint8_t* __restrict__ cls, const 
 // iteration k - 1 is still available for beta.  Convergence (r.r <= tol^2) is detected by
 // every block of the step-1 kernel from the same sums; block 0 records it in istate, which
 // turns every later queued kernel into a no-op.
-constexpr int PT_W = 64, PT_H = 16;   // nodes of a tile kernel's block
-constexpr int PT_THREADS = 512;                     // threads of a tile kernel's block
+constexpr int PT_W = 128, PT_H = 16;  // nodes of a tile kernel's block
+constexpr int PT_THREADS = 1024;                    // threads of a tile kernel's block
 constexpr int PT_RSTEP = PT_THREADS / PT_W;         // thread t owns column t % PT_W, rows t / PT_W + PT_RSTEP s
 constexpr int PT_RPT = PT_H / PT_RSTEP;             // rows per thread
 static_assert(PT_THREADS % PT_W == 0 && PT_H % PT_RSTEP == 0, "tile shape");
@@ -844,11 +849,6 @@ __global__ void expand_phi(int64_t nn, const int32_t* __restrict__ src, const do
         phi[u] = s < 0 ? fixval[u] : x[s] + fixval[u];
     }
 }
-__global__ void gather_free(int64_t nn, const uint8_t* __restrict__ cls, const double* __restrict__ phi,
-                            double* __restrict__ x) {
-    for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nn; u += (int64_t)gridDim.x * blockDim.x)
-        x[u] = cls[u] == CLS_NOT_FREE ? 0.0 : phi[u];
-}
 
 // E = -grad(phi): Solver.evaluateGradient with the assembled Gi/Gj rows, which on a uniform
 // mesh are central differences inside and one-sided differences on the mesh faces.
@@ -1115,11 +1115,9 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     const double nl_tol = D.linear ? 0.0 : D.nl_tol;
     bool nl_stop = false;
     SF_HIP(ctx, hipEventRecord(P->e0, st));
-    hipLaunchKernelGGL(gather_free, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, (const double*)ctx->d_phi,
-                       P->x);
     hipLaunchKernelGGL(ref_norm_kernel, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
                        (const double*)P->bfix, (const double*)ctx->d_rho, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
-                       P->partial);
+                       P->partial, (const double*)ctx->d_phi, P->x);
     finalize<1>(P, st, P->partial, ge, S_R0);
     int newton = 0, pcg_total = 0;
     double rel = 0, rnorm = 0, ref = 1.0;
@@ -1154,6 +1152,10 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             SF_HIP(ctx, hipMemcpyAsync(P->host_nk, P->nk + (i - 1) % NK_RING, sizeof(NkState),
                                        hipMemcpyDeviceToHost, st));
             SF_HIP(ctx, hipMemcpyAsync(P->host_scal, P->scal, S_NSCAL * 8, hipMemcpyDeviceToHost, st));
+            // phi from x, queued before the host reads the state: when the batch ended the solve the
+            // potential is already written (otherwise the next batch's expansion overwrites it)
+            hipLaunchKernelGGL(expand_phi, dim3(ge), dim3(256), 0, st, nn, (const int32_t*)P->src,
+                               (const double*)P->fixval, (const double*)P->x, ctx->d_phi);
             SF_HIP(ctx, hipStreamSynchronize(st));
             if (P->host_nk->done) break;
             if (i > 4 * (max_it + 1) * (nl_max + 1)) return sf_fail(ctx, SF_ERR_STATE, "device Newton-PCG did not stop");
@@ -1338,8 +1340,9 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
     }
     }   // !device_done
     P->warm_hint = converged;
-    hipLaunchKernelGGL(expand_phi, dim3(ge), dim3(256), 0, st, nn, (const int32_t*)P->src, (const double*)P->fixval,
-                       (const double*)P->x, ctx->d_phi);
+    if (!device_done)   // (the device loop expanded phi with its last batch)
+        hipLaunchKernelGGL(expand_phi, dim3(ge), dim3(256), 0, st, nn, (const int32_t*)P->src,
+                           (const double*)P->fixval, (const double*)P->x, ctx->d_phi);
     SF_HIP(ctx, hipGetLastError());
     SF_HIP(ctx, hipEventRecord(P->e1, st));
     SF_HIP(ctx, hipEventSynchronize(P->e1));
@@ -1580,11 +1583,9 @@ int poisson_rel_residual(sf_ctx* ctx, double* rel) {
     const double inv_eps = 1.0 / P->eps;
     const double Cn0 = D.linear ? 0.0 : P->C * D.n0;
     const double inv_kTe = D.linear ? 0.0 : 1.0 / D.Te0;
-    hipLaunchKernelGGL(gather_free, dim3(ge), dim3(256), 0, st, nn, (const uint8_t*)P->cls, (const double*)ctx->d_phi,
-                       P->x);
     hipLaunchKernelGGL(ref_norm_kernel, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
                        (const double*)P->bfix, (const double*)ctx->d_rho, inv_eps, Cn0, D.phi0, inv_kTe, D.linear,
-                       P->partial);
+                       P->partial, (const double*)ctx->d_phi, P->x);
     finalize<1>(P, st, P->partial, ge, S_R0);
     hipLaunchKernelGGL(newton_residual, dim3(ge), dim3(256), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
                        (const double*)P->bfix, (const double*)ctx->d_rho, (const double*)P->x, P->r, P->D, P->P,

@@ -1033,13 +1033,29 @@ __device__ __forceinline__ int64_t wave_rank(bool valid, int key, int32_t* curso
     return valid ? b + __popcll(my_group & ((1ull << lane) - 1ull)) : -1;
 }
 
-// sort without a push (after particles were loaded): key = current cell
+// sort without a push (after particles were loaded, before DSMC): key = current cell.  In a
+// cell-ordered array the lanes of a wave hold few cells: up to 4 groups of lanes with one cell
+// each add their count with one atomic by their lowest lane; the lanes left (an unordered array)
+// add their own.
 __global__ void __launch_bounds__(256) count_current(MeshP m, PState s, int64_t np, int32_t* __restrict__ keys,
                                                      int32_t* __restrict__ counts) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        int32_t c = s.cell[p];
-        keys[p] = c;
-        if (c >= 0) atomicAdd(&counts[cell_j(c) * m.nci + cell_i(c)], 1);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nloop = (np + stride - 1) / stride * stride;   // the lanes of a wave iterate together
+    const int lane = threadIdx.x & 63;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nloop; p += stride) {
+        const bool valid = p < np;
+        const int32_t c = valid ? s.cell[p] : -1;
+        if (valid) keys[p] = c;
+        const int lk = c >= 0 ? cell_j(c) * m.nci + cell_i(c) : -1;
+        unsigned long long rem = __ballot(lk >= 0);
+        for (int g = 0; g < 4 && rem; g++) {
+            const int leader = __ffsll((long long)rem) - 1;
+            const int key = __shfl(lk, leader);
+            const unsigned long long grp = __ballot(lk == key);
+            if (lane == leader) atomicAdd(&counts[key], __popcll(grp));
+            rem &= ~grp;
+        }
+        if ((rem >> lane) & 1ull) atomicAdd(&counts[lk], 1);
     }
 }
 
