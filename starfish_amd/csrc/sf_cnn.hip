@@ -28,11 +28,29 @@ __device__ __host__ __forceinline__ uint16_t f2bf(float f) {
     u += 0x7FFFu + ((u >> 16) & 1u);                                     // round to nearest even
     return (uint16_t)(u >> 16);
 }
+// two floats to packed BF16 (round to nearest even, as f2bf for finite values) with the
+// hardware conversion
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    const __bf16 x = (__bf16)a, y = (__bf16)b;
+    uint16_t ux, uy;
+    memcpy(&ux, &x, 2);
+    memcpy(&uy, &y, 2);
+    return (uint32_t)ux | ((uint32_t)uy << 16);
+}
 __device__ __host__ __forceinline__ float bf2f(uint16_t h) {
     uint32_t u = (uint32_t)h << 16;
     float f;
     memcpy(&f, &u, 4);
     return f;
+}
+
+// ReLU backward of 4 channels: g where the BF16 activation (4 values packed in a) is positive, else 0
+__device__ __forceinline__ float4 relu_mask4(const float4 g, const uint2 a) {
+    return make_float4(bf2f((uint16_t)(a.x & 0xFFFFu)) > 0.f ? g.x : 0.f, bf2f((uint16_t)(a.x >> 16)) > 0.f ? g.y : 0.f,
+                       bf2f((uint16_t)(a.y & 0xFFFFu)) > 0.f ? g.z : 0.f, bf2f((uint16_t)(a.y >> 16)) > 0.f ? g.w : 0.f);
+}
+__device__ __forceinline__ uint2 pack_bf16x4(const float4 e) {
+    return make_uint2(pack_bf16x2(e.x, e.y), pack_bf16x2(e.z, e.w));
 }
 
 __device__ __forceinline__ bf16x8 load8(const uint16_t* p) {
@@ -58,6 +76,9 @@ __device__ __forceinline__ bf16x8 zero8() {
 //   forward:       out[p][n] = sum_tap sum_k in[p + d(tap)][k] * w[tap][n][k]   (+ bias, ReLU)
 //   backward data: din[p][k'] = sum_tap sum_n' g[p - d(tap)][n'] * w[tap][n'][k'] with the weight
 //                  stored transposed as wt[tap][k'][n'] -> the same kernel with sgn = -1.
+// MODE 0: BF16 output with bias and optional ReLU; MODE 1: FP32 output (optionally accumulated);
+// MODE 2: the ReLU backward of the previous layer fused: BF16 output act > 0 ? din : 0, with act
+// that layer's BF16 activation.
 // Requirements (checked on the host): K multiple of 32, N multiple of 16, channel strides and
 // offsets multiples of 8 (16-byte operand loads).
 // ------------------------------------------------------------------------------------------
@@ -76,14 +97,18 @@ struct FView {
 // channels the R + 2 input rows of CONV_TP + 2 pixels and the 9 x TN x 32 weights go to LDS once,
 // and every wave runs its 16 x 16 tiles of all R rows over the 9 taps from there: a weight fragment
 // is read once per tap for R x PT pixel tiles, an input fragment once for two channel tiles.  The
-// next chunk's operands are loaded into registers while the current one is multiplied.
+// next chunk's operands are loaded into registers while the current one is multiplied.  The
+// weights are the MFMA's A operand (rows = output channels) and the pixels its B operand, so a
+// lane's result holds 4 consecutive channels of one pixel and leaves in one 8-byte (BF16) or
+// 16-byte (FP32) store.
 constexpr int CONV_TP = 128;
 template <int MODE, int TN, int R>
 __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ in, int in_cs, int in_co, int K,
                                                    const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                                    int N, uint16_t* __restrict__ out, float* __restrict__ fout,
                                                    int out_cs, int out_co, int H, int Wd, int relu, int sgn,
-                                                   int accumulate) {
+                                                   int accumulate, const uint16_t* __restrict__ act, int act_cs,
+                                                   int act_co) {
     constexpr int WN = TN / 32;                 // waves along the channels (two 16-channel tiles each)
     constexpr int WP = 4 / WN;                  // waves along the pixels
     constexpr int PT = CONV_TP / 16 / WP;       // 16-pixel tiles per wave and row
@@ -154,37 +179,49 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
                     const uint4 v = s_in[((r + 1 + dy) * IW + px + 1 + dx) * 4 + kq];
                     bf16x8 a;
                     memcpy(&a, &v, 16);
-                    acc[r][p][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[0], acc[r][p][0], 0, 0, 0);
-                    acc[r][p][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[1], acc[r][p][1], 0, 0, 0);
+                    acc[r][p][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[0], a, acc[r][p][0], 0, 0, 0);
+                    acc[r][p][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[1], a, acc[r][p][1], 0, 0, 0);
                 }
         }
     }
-    // D layout: lane holds rows (lane/16)*4 + i (pixels), column lane%16 (channel)
+    // D layout: lane holds rows (lane/16)*4 + i (channels), column lane%16 (pixel)
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int y = y0 + r;
         if (y >= H) continue;
 #pragma unroll
-        for (int p = 0; p < PT; p++)
+        for (int p = 0; p < PT; p++) {
+            const int xo = x0 + (wp * PT + p) * 16 + prow;
+            if (xo >= Wd) continue;
 #pragma unroll
             for (int t = 0; t < 2; t++) {
-                const int n = n0 + wn * 32 + t * 16 + prow;
+                const int n = n0 + wn * 32 + t * 16 + kq * 4;   // this lane's 4 channels n .. n + 3
                 if (n >= N) continue;
+                const size_t o = (((size_t)b * H + y) * Wd + xo) * out_cs + out_co + n;
+                const f32x4 v = acc[r][p][t];
+                if (MODE == 0) {
+                    float e[4];
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int xo = x0 + (wp * PT + p) * 16 + (lane >> 4) * 4 + i;
-                    if (xo >= Wd) continue;
-                    const size_t o = (((size_t)b * H + y) * Wd + xo) * out_cs + out_co + n;
-                    float v = acc[r][p][t][i];
-                    if (MODE == 0) {
-                        v += bias[n];
-                        if (relu) v = fmaxf(v, 0.f);
-                        out[o] = f2bf(v);
+                    for (int i = 0; i < 4; i++) {
+                        e[i] = v[i] + bias[n + i];
+                        if (relu) e[i] = fmaxf(e[i], 0.f);
+                    }
+                    *reinterpret_cast<uint2*>(out + o) = make_uint2(pack_bf16x2(e[0], e[1]), pack_bf16x2(e[2], e[3]));
+                } else if (MODE == 2) {
+                    const uint2 a = *reinterpret_cast<const uint2*>(
+                        act + (((size_t)b * H + y) * Wd + xo) * act_cs + act_co + n);
+                    *reinterpret_cast<uint2*>(out + o) = pack_bf16x4(relu_mask4(make_float4(v[0], v[1], v[2], v[3]), a));
+                } else {
+                    float4* f = reinterpret_cast<float4*>(fout + o);
+                    if (accumulate) {
+                        const float4 g = *f;
+                        *f = make_float4(g.x + v[0], g.y + v[1], g.z + v[2], g.w + v[3]);
                     } else {
-                        fout[o] = accumulate ? fout[o] + v : v;
+                        *f = make_float4(v[0], v[1], v[2], v[3]);
                     }
                 }
             }
+        }
     }
 }
 
@@ -337,16 +374,6 @@ __global__ void __launch_bounds__(256) wgrad_reduce(const float* __restrict__ pa
 }
 
 
-// ReLU backward into a BF16 operand: g = (act > 0) ? d : 0
-__global__ void relu_bwd(const uint16_t* __restrict__ act, int a_cs, int a_co, const float* __restrict__ d, int d_cs,
-                         int d_co, int C, int64_t npix, uint16_t* __restrict__ g) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npix * C; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t p = i / C;
-        int c = (int)(i % C);
-        float a = bf2f(act[p * a_cs + a_co + c]);
-        g[p * C + c] = f2bf(a > 0.f ? d[p * d_cs + d_co + c] : 0.f);
-    }
-}
 
 // 2x2 average pooling (forward BF16, backward FP32 accumulate)
 __global__ void avgpool_fwd(const uint16_t* __restrict__ in, int in_cs, int in_co, int C, int H, int Wd, int B,
@@ -363,16 +390,24 @@ __global__ void avgpool_fwd(const uint16_t* __restrict__ in, int in_cs, int in_c
         out[(((size_t)b * Ho + yo) * Wo + xo) * out_cs + out_co + c] = f2bf(0.25f * s);
     }
 }
-__global__ void avgpool_bwd(const float* __restrict__ dout, int d_cs, int d_co, int C, int H, int Wd, int B,
-                            float* __restrict__ din, int i_cs, int i_co) {
-    const int Ho = H / 2, Wo = Wd / 2;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * H * Wd * C;
+// Backward of the 2x2 average pooling fused with the ReLU backward of the pooled activation: the
+// gradient of a full-resolution pixel is its skip-path gradient (the view gskip) plus a quarter of
+// its parent's (dout, [pixel][C]), masked by the activation (view act) and rounded to BF16 into
+// g [pixel][C].  4 channels per thread; C, strides and offsets multiples of 4.
+__global__ void avgpool_relu_bwd(const float* __restrict__ dout, int C, int H, int Wd, int B,
+                                 const float* __restrict__ gskip, int s_cs, int s_co, const uint16_t* __restrict__ act,
+                                 int a_cs, int a_co, uint16_t* __restrict__ g) {
+    const int Ho = H / 2, Wo = Wd / 2, G = C / 4;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * H * Wd * G;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int c = (int)(i % C);
-        int64_t q = i / C;
-        int x = (int)(q % Wd), y = (int)((q / Wd) % H), b = (int)(q / ((int64_t)Wd * H));
-        float v = 0.25f * dout[(((size_t)b * Ho + y / 2) * Wo + x / 2) * d_cs + d_co + c];
-        din[(((size_t)b * H + y) * Wd + x) * i_cs + i_co + c] += v;
+        const int c = 4 * (int)(i % G);
+        const int64_t q = i / G;
+        const int x = (int)(q % Wd), y = (int)((q / Wd) % H), b = (int)(q / ((int64_t)Wd * H));
+        const float4 d = *reinterpret_cast<const float4*>(dout + (((size_t)b * Ho + y / 2) * Wo + x / 2) * C + c);
+        const float4 sk = *reinterpret_cast<const float4*>(gskip + q * s_cs + s_co + c);
+        const uint2 a = *reinterpret_cast<const uint2*>(act + q * a_cs + a_co + c);
+        const float4 e = make_float4(sk.x + 0.25f * d.x, sk.y + 0.25f * d.y, sk.z + 0.25f * d.z, sk.w + 0.25f * d.w);
+        *reinterpret_cast<uint2*>(g + q * C + c) = pack_bf16x4(relu_mask4(e, a));
     }
 }
 // nearest 2x upsampling (forward BF16 into a view, 8 channels per 16-byte access; backward FP32
@@ -389,18 +424,27 @@ __global__ void upsample_fwd(const uint16_t* __restrict__ in, int in_cs, int C, 
             *reinterpret_cast<const uint4*>(in + (((size_t)b * Hi + y / 2) * Wi + x / 2) * in_cs + 8 * g);
     }
 }
-__global__ void upsample_bwd(const float* __restrict__ dout, int d_cs, int d_co, int C, int Hi, int Wi, int B,
-                             float* __restrict__ din) {
-    const int H = 2 * Hi, Wd = 2 * Wi;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * Hi * Wi * C;
+// Backward of the nearest 2x upsampling fused with the ReLU backward of the upsampled activation:
+// the sum of the four children's gradients (view dout, d_cs channels per pixel), masked by the
+// activation (act, [pixel][C] at the low resolution) and rounded to BF16 into g [pixel][C].  4
+// channels per thread; C and d_cs multiples of 4.
+__global__ void upsample_relu_bwd(const float* __restrict__ dout, int d_cs, int C, int Hi, int Wi, int B,
+                                  const uint16_t* __restrict__ act, uint16_t* __restrict__ g) {
+    const int H = 2 * Hi, Wd = 2 * Wi, G = C / 4;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * Hi * Wi * G;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int c = (int)(i % C);
-        int64_t q = i / C;
-        int x = (int)(q % Wi), y = (int)((q / Wi) % Hi), b = (int)(q / ((int64_t)Wi * Hi));
-        float s = 0;
+        const int c = 4 * (int)(i % G);
+        const int64_t q = i / G;
+        const int x = (int)(q % Wi), y = (int)((q / Wi) % Hi), b = (int)(q / ((int64_t)Wi * Hi));
+        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int a = 0; a < 2; a++)
-            for (int e = 0; e < 2; e++) s += dout[(((size_t)b * H + 2 * y + a) * Wd + 2 * x + e) * d_cs + d_co + c];
-        din[(((size_t)b * Hi + y) * Wi + x) * C + c] = s;
+            for (int e = 0; e < 2; e++) {
+                const float4 v =
+                    *reinterpret_cast<const float4*>(dout + (((size_t)b * H + 2 * y + a) * Wd + 2 * x + e) * d_cs + c);
+                s = make_float4(s.x + v.x, s.y + v.y, s.z + v.z, s.w + v.w);
+            }
+        const uint2 av = *reinterpret_cast<const uint2*>(act + q * C + c);
+        *reinterpret_cast<uint2*>(g + q * C + c) = pack_bf16x4(relu_mask4(s, av));
     }
 }
 // 1x1 head C -> 1 in FP32 on top of a base channel of the input: y = base + sum_c w[c] act[c] + b
@@ -433,13 +477,13 @@ __global__ void mse_loss(const float* __restrict__ y, const float* __restrict__ 
     }
 }
 // head backward (C = HEAD_C channels): dw[c] += sum_p dy[p] act[p][c]; db += sum dy;
-// dact[p][c] = dy[p] w[c] (FP32).  Each thread keeps its channel sums in registers; the block
+// gact[p][c] = act[p][c] > 0 ? dy[p] w[c] : 0 in BF16 (the ReLU backward of act fused).  Each thread keeps its channel sums in registers; the block
 // adds them with shuffles and one atomic per channel.
 constexpr int HEAD_C = 32;
 __global__ void __launch_bounds__(256) head_bwd(const uint16_t* __restrict__ act, int64_t npix,
                                                 const float* __restrict__ w, const float* __restrict__ dy,
                                                 float* __restrict__ dw, float* __restrict__ db,
-                                                float* __restrict__ dact) {
+                                                uint16_t* __restrict__ gact) {
     __shared__ float red[HEAD_C + 1][4];
     float sw[HEAD_C], sdb = 0.f, wv[HEAD_C];
 #pragma unroll
@@ -450,7 +494,7 @@ __global__ void __launch_bounds__(256) head_bwd(const uint16_t* __restrict__ act
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
         const float d = dy[p];
         const uint4* a4 = reinterpret_cast<const uint4*>(act + p * HEAD_C);
-        float4* o4 = reinterpret_cast<float4*>(dact + p * HEAD_C);
+        uint2* o4 = reinterpret_cast<uint2*>(gact + p * HEAD_C);
 #pragma unroll
         for (int q = 0; q < HEAD_C / 8; q++) {
             const uint4 u = a4[q];
@@ -460,8 +504,11 @@ __global__ void __launch_bounds__(256) head_bwd(const uint16_t* __restrict__ act
                 sw[8 * q + 2 * e] += d * bf2f((uint16_t)(uw[e] & 0xFFFFu));
                 sw[8 * q + 2 * e + 1] += d * bf2f((uint16_t)(uw[e] >> 16));
             }
-            o4[2 * q] = make_float4(d * wv[8 * q], d * wv[8 * q + 1], d * wv[8 * q + 2], d * wv[8 * q + 3]);
-            o4[2 * q + 1] = make_float4(d * wv[8 * q + 4], d * wv[8 * q + 5], d * wv[8 * q + 6], d * wv[8 * q + 7]);
+            o4[2 * q] = pack_bf16x4(relu_mask4(
+                make_float4(d * wv[8 * q], d * wv[8 * q + 1], d * wv[8 * q + 2], d * wv[8 * q + 3]), make_uint2(u.x, u.y)));
+            o4[2 * q + 1] = pack_bf16x4(relu_mask4(
+                make_float4(d * wv[8 * q + 4], d * wv[8 * q + 5], d * wv[8 * q + 6], d * wv[8 * q + 7]),
+                make_uint2(u.z, u.w)));
         }
         sdb += d;
     }
@@ -534,12 +581,11 @@ struct sf_cnn {
     // activations (BF16 NHWC)
     uint16_t *x0 = nullptr, *e1a = nullptr, *u1 = nullptr, *p1 = nullptr, *e2a = nullptr, *u2 = nullptr,
              *p2 = nullptr, *e3a = nullptr, *e3 = nullptr, *d2a = nullptr, *d2 = nullptr, *d1a = nullptr,
-             *d1 = nullptr, *gop = nullptr;
+             *d1 = nullptr;
+    uint16_t *gop = nullptr, *gop2 = nullptr;   // BF16 operand gradients [npix][channels] (backward)
     float* y = nullptr;
-    // gradients (FP32)
-    float *g_e1a = nullptr, *g_u1 = nullptr, *g_p1 = nullptr, *g_e2a = nullptr, *g_u2 = nullptr, *g_p2 = nullptr,
-          *g_e3a = nullptr, *g_e3 = nullptr, *g_d2a = nullptr, *g_d2 = nullptr, *g_d1a = nullptr, *g_d1 = nullptr,
-          *dy = nullptr, *loss = nullptr;
+    // FP32 gradients where two paths add
+    float *g_u1 = nullptr, *g_p1 = nullptr, *g_u2 = nullptr, *g_p2 = nullptr, *dy = nullptr, *loss = nullptr;
     // dataset
     std::vector<uint16_t*> s_in;      // [H][W][4] BF16: rho, mask, phi_b, 0
     std::vector<float*> s_t;          // [H][W] target phi / phi_ref
@@ -607,17 +653,18 @@ static void conv_pack(sf_cnn* c, ConvL& L) {
 template <int MODE>
 static void conv_launch(sf_cnn* c, const uint16_t* in, int in_cs, int in_co, int K, const uint16_t* w,
                         const float* bias, int N, uint16_t* out, float* fout, int out_cs, int out_co, int H, int W,
-                        int B, int relu, int sgn, int accumulate) {
+                        int B, int relu, int sgn, int accumulate, const uint16_t* act = nullptr, int act_cs = 0,
+                        int act_co = 0) {
     // rows per workgroup: 2 with 64-channel tiles, 4 with 32 (the same accumulator count)
     const bool t64 = N % 64 == 0;
     const int tn = t64 ? 64 : 32, rows = t64 ? 2 : 4;
     const dim3 g((W + CONV_TP - 1) / CONV_TP, (H + rows - 1) / rows, B * ((N + tn - 1) / tn));
     if (t64)
         hipLaunchKernelGGL((conv3x3_lds<MODE, 64, 2>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias,
-                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate);
+                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate, act, act_cs, act_co);
     else
         hipLaunchKernelGGL((conv3x3_lds<MODE, 32, 4>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias,
-                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate);
+                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate, act, act_cs, act_co);
 }
 // forward conv: in view (cs, co, K channels) -> out view, bias + ReLU
 static void conv_fwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs,
@@ -625,9 +672,12 @@ static void conv_fwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, i
     conv_launch<0>(c, in, in_cs, in_co, L.K, (const uint16_t*)L.wf, (const float*)L.b, L.N, out, (float*)nullptr,
                    out_cs, out_co, H, W, B, 1, 1, 0);
 }
-// backward of one conv: g (BF16 [npix][N]) -> weight / bias gradients and, if din != null, din
+// backward of one conv: g (BF16 [npix][N]) -> weight / bias gradients and, if din != null, the FP32
+// input gradient into the din view; if gout != null, the input gradient through the ReLU of the
+// layer that produced the input (activation view act) as BF16 [npix][K] into gout
 static void conv_bwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, int in_co, const uint16_t* g,
-                     float* din, int din_cs, int din_co, int H, int W, int B) {
+                     float* din, int din_cs, int din_co, int H, int W, int B, uint16_t* gout = nullptr,
+                     const uint16_t* act = nullptr, int act_cs = 0, int act_co = 0) {
     // weight gradient: about WG_BLOCKS workgroups over the (32 x 32 channel tile, pixel band)
     // pairs, their partials summed in workgroup order
     const int nchunks = B * H * ((W + WG_PX - 1) / WG_PX);
@@ -642,11 +692,9 @@ static void conv_bwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, i
     if (din)
         conv_launch<1>(c, g, L.N, 0, L.N, (const uint16_t*)L.wt, (const float*)nullptr, L.K, (uint16_t*)nullptr, din,
                        din_cs, din_co, H, W, B, 0, -1, 0);
-}
-static void relu_op(sf_cnn* c, const uint16_t* act, int a_cs, int a_co, const float* d, int d_cs, int d_co, int C,
-                    int64_t npix) {
-    hipLaunchKernelGGL(relu_bwd, dim3(grid1(npix * C)), dim3(256), 0, c->ctx->stream, act, a_cs, a_co, d, d_cs, d_co,
-                       C, npix, c->gop);
+    if (gout)
+        conv_launch<2>(c, g, L.N, 0, L.N, (const uint16_t*)L.wt, (const float*)nullptr, L.K, gout, (float*)nullptr,
+                       L.K, 0, H, W, B, 0, -1, 0, act, act_cs, act_co);
 }
 
 static void forward(sf_cnn* c, int B) {
@@ -677,41 +725,37 @@ static void forward(sf_cnn* c, int B) {
                        (int64_t)B * H * W, (const float*)c->hw, (const float*)c->hb, (const uint16_t*)c->x0, 32, 3, c->y);
 }
 
+// The BF16 operand gradients alternate between gop and gop2 (a layer's backward reads one and
+// writes the next into the other); every ReLU backward is fused into the kernel that produces the
+// gradient of the activation.  FP32 gradients remain only where two paths add: the inputs of the
+// decoder convolutions (upsampled part plus skip part) and the pooled activations.
 static void backward(sf_cnn* c, int B) {
     const int H = c->H, W = c->W, C1 = c->C1, C2 = c->C2, C3 = c->C3;
     const int H2 = H / 2, W2 = W / 2, H4 = H / 4, W4 = W / 4;
     const int64_t n1 = (int64_t)B * H * W, n2 = (int64_t)B * H2 * W2, n4 = (int64_t)B * H4 * W4;
     hipStream_t st = c->ctx->stream;
+    uint16_t* ga = c->gop;
+    uint16_t* gb = c->gop2;
     hipLaunchKernelGGL(head_bwd, dim3(1024), dim3(256), 0, st, (const uint16_t*)c->d1, n1, (const float*)c->hw,
-                       (const float*)c->dy, c->ghw, c->ghb, c->g_d1);
-    relu_op(c, c->d1, C1, 0, c->g_d1, C1, 0, C1, n1);
-    conv_bwd(c, c->c5b, c->d1a, C1, 0, c->gop, c->g_d1a, C1, 0, H, W, B);
-    relu_op(c, c->d1a, C1, 0, c->g_d1a, C1, 0, C1, n1);
-    conv_bwd(c, c->c5a, c->u1, C2 + C1, 0, c->gop, c->g_u1, C2 + C1, 0, H, W, B);
-    hipLaunchKernelGGL(upsample_bwd, dim3(grid1(n2 * C2)), dim3(256), 0, st, (const float*)c->g_u1, C2 + C1, 0, C2,
-                       H2, W2, B, c->g_d2);
-    relu_op(c, c->d2, C2, 0, c->g_d2, C2, 0, C2, n2);
-    conv_bwd(c, c->c4b, c->d2a, C2, 0, c->gop, c->g_d2a, C2, 0, H2, W2, B);
-    relu_op(c, c->d2a, C2, 0, c->g_d2a, C2, 0, C2, n2);
-    conv_bwd(c, c->c4a, c->u2, C3 + C2, 0, c->gop, c->g_u2, C3 + C2, 0, H2, W2, B);
-    hipLaunchKernelGGL(upsample_bwd, dim3(grid1(n4 * C3)), dim3(256), 0, st, (const float*)c->g_u2, C3 + C2, 0, C3,
-                       H4, W4, B, c->g_e3);
-    relu_op(c, c->e3, C3, 0, c->g_e3, C3, 0, C3, n4);
-    conv_bwd(c, c->c3b, c->e3a, C3, 0, c->gop, c->g_e3a, C3, 0, H4, W4, B);
-    relu_op(c, c->e3a, C3, 0, c->g_e3a, C3, 0, C3, n4);
-    conv_bwd(c, c->c3a, c->p2, C2, 0, c->gop, c->g_p2, C2, 0, H4, W4, B);
-    hipLaunchKernelGGL(avgpool_bwd, dim3(grid1(n2 * C2)), dim3(256), 0, st, (const float*)c->g_p2, C2, 0, C2, H2, W2,
-                       B, c->g_u2, C3 + C2, C3);
-    relu_op(c, c->u2, C3 + C2, C3, c->g_u2, C3 + C2, C3, C2, n2);
-    conv_bwd(c, c->c2b, c->e2a, C2, 0, c->gop, c->g_e2a, C2, 0, H2, W2, B);
-    relu_op(c, c->e2a, C2, 0, c->g_e2a, C2, 0, C2, n2);
-    conv_bwd(c, c->c2a, c->p1, C1, 0, c->gop, c->g_p1, C1, 0, H2, W2, B);
-    hipLaunchKernelGGL(avgpool_bwd, dim3(grid1(n1 * C1)), dim3(256), 0, st, (const float*)c->g_p1, C1, 0, C1, H, W, B,
-                       c->g_u1, C2 + C1, C2);
-    relu_op(c, c->u1, C2 + C1, C2, c->g_u1, C2 + C1, C2, C1, n1);
-    conv_bwd(c, c->c1b, c->e1a, C1, 0, c->gop, c->g_e1a, C1, 0, H, W, B);
-    relu_op(c, c->e1a, C1, 0, c->g_e1a, C1, 0, C1, n1);
-    conv_bwd(c, c->c1a, c->x0, 32, 0, c->gop, nullptr, 0, 0, H, W, B);
+                       (const float*)c->dy, c->ghw, c->ghb, ga);
+    conv_bwd(c, c->c5b, c->d1a, C1, 0, ga, nullptr, 0, 0, H, W, B, gb, c->d1a, C1, 0);
+    conv_bwd(c, c->c5a, c->u1, C2 + C1, 0, gb, c->g_u1, C2 + C1, 0, H, W, B);
+    hipLaunchKernelGGL(upsample_relu_bwd, dim3(grid1(n2 * C2 / 4)), dim3(256), 0, st, (const float*)c->g_u1, C2 + C1,
+                       C2, H2, W2, B, (const uint16_t*)c->d2, ga);
+    conv_bwd(c, c->c4b, c->d2a, C2, 0, ga, nullptr, 0, 0, H2, W2, B, gb, c->d2a, C2, 0);
+    conv_bwd(c, c->c4a, c->u2, C3 + C2, 0, gb, c->g_u2, C3 + C2, 0, H2, W2, B);
+    hipLaunchKernelGGL(upsample_relu_bwd, dim3(grid1(n4 * C3 / 4)), dim3(256), 0, st, (const float*)c->g_u2, C3 + C2,
+                       C3, H4, W4, B, (const uint16_t*)c->e3, ga);
+    conv_bwd(c, c->c3b, c->e3a, C3, 0, ga, nullptr, 0, 0, H4, W4, B, gb, c->e3a, C3, 0);
+    conv_bwd(c, c->c3a, c->p2, C2, 0, gb, c->g_p2, C2, 0, H4, W4, B);
+    hipLaunchKernelGGL(avgpool_relu_bwd, dim3(grid1(n2 * C2 / 4)), dim3(256), 0, st, (const float*)c->g_p2, C2, H2, W2,
+                       B, (const float*)c->g_u2, C3 + C2, C3, (const uint16_t*)c->u2, C3 + C2, C3, ga);
+    conv_bwd(c, c->c2b, c->e2a, C2, 0, ga, nullptr, 0, 0, H2, W2, B, gb, c->e2a, C2, 0);
+    conv_bwd(c, c->c2a, c->p1, C1, 0, gb, c->g_p1, C1, 0, H2, W2, B);
+    hipLaunchKernelGGL(avgpool_relu_bwd, dim3(grid1(n1 * C1 / 4)), dim3(256), 0, st, (const float*)c->g_p1, C1, H, W, B,
+                       (const float*)c->g_u1, C2 + C1, C2, (const uint16_t*)c->u1, C2 + C1, C2, ga);
+    conv_bwd(c, c->c1b, c->e1a, C1, 0, ga, nullptr, 0, 0, H, W, B, gb, c->e1a, C1, 0);
+    conv_bwd(c, c->c1a, c->x0, 32, 0, gb, nullptr, 0, 0, H, W, B);
 }
 
 static std::vector<std::pair<float*, size_t>> grads(sf_cnn* c) {
@@ -864,19 +908,12 @@ int sf_cnn_create(sf_ctx* ctx, int ni, int nj, int max_batch, uint64_t seed, sf_
     rc |= cnn_alloc(c, &c->d1a, n1 * C1);
     rc |= cnn_alloc(c, &c->d1, n1 * C1);
     rc |= cnn_alloc(c, &c->gop, n1 * (C2 + C1));
+    rc |= cnn_alloc(c, &c->gop2, n1 * C1);
     rc |= cnn_alloc(c, &c->y, n1);
-    rc |= cnn_alloc(c, &c->g_e1a, n1 * C1);
     rc |= cnn_alloc(c, &c->g_u1, n1 * (C2 + C1));
     rc |= cnn_alloc(c, &c->g_p1, n2 * C1);
-    rc |= cnn_alloc(c, &c->g_e2a, n2 * C2);
     rc |= cnn_alloc(c, &c->g_u2, n2 * (C3 + C2));
     rc |= cnn_alloc(c, &c->g_p2, n4 * C2);
-    rc |= cnn_alloc(c, &c->g_e3a, n4 * C3);
-    rc |= cnn_alloc(c, &c->g_e3, n4 * C3);
-    rc |= cnn_alloc(c, &c->g_d2a, n2 * C2);
-    rc |= cnn_alloc(c, &c->g_d2, n2 * C2);
-    rc |= cnn_alloc(c, &c->g_d1a, n1 * C1);
-    rc |= cnn_alloc(c, &c->g_d1, n1 * C1);
     rc |= cnn_alloc(c, &c->dy, n1);
     rc |= cnn_alloc(c, &c->loss, 4);
     rc |= cnn_alloc(c, &c->bt, n1);

@@ -67,6 +67,7 @@ struct PoissonState {
     double* scal = nullptr;       // device scalars
     int* istate = nullptr;        // [0] done (1 converged, 2 not finite), [1] iterations at done
     int nblk_tile = 0, nblk_elem = 0;
+    bool wide_tiles = true;       // TileWide (else TileNarrow) for the tile kernels
     int last_its = 7;             // PCG iterations of the last linear solve (sizes the next batch)
     int last_pairs = 24;          // step pairs of the last device-driven solve (sizes its first batch)
     bool host_newton = false;     // SF_HOST_NEWTON: keep every solve on the host-driven loop
@@ -305,14 +306,26 @@ __global__ void refine_residual(MeshP m, const uint8_t* __restrict__ cls, const 
 // iteration k - 1 is still available for beta.  Convergence (r.r <= tol^2) is detected by
 // every block of the step-1 kernel from the same sums; block 0 records it in istate, which
 // turns every later queued kernel into a no-op.
-constexpr int PT_W = 128, PT_H = 16;  // nodes of a tile kernel's block
-constexpr int PT_THREADS = 1024;                    // threads of a tile kernel's block
-constexpr int PT_RSTEP = PT_THREADS / PT_W;         // thread t owns column t % PT_W, rows t / PT_W + PT_RSTEP s
-constexpr int PT_RPT = PT_H / PT_RSTEP;             // rows per thread
-static_assert(PT_THREADS % PT_W == 0 && PT_H % PT_RSTEP == 0, "tile shape");
+// The tile kernels are templates over the tile shape: W x H nodes per block of NT threads; thread
+// t owns column t % W and rows t / W + RSTEP s (s < RPT) of the tile.  MINW: the waves per SIMD
+// nk_step_b is compiled for (its register budget).
+template <int W_, int H_, int NT_, int MINW_>
+struct TileShape {
+    static constexpr int W = W_, H = H_, NT = NT_, MINW = MINW_;
+    static constexpr int RSTEP = NT_ / W_;
+    static constexpr int RPT = H_ / RSTEP;
+    static_assert(NT_ % W_ == 0 && H_ % RSTEP == 0, "tile shape");
+};
+// 128 x 16 tiles of 1024 threads: the stronger block preconditioner, for meshes whose tiles fit in
+// one workgroup per CU.  64 x 16 tiles of 512 threads on larger meshes, where several workgroups
+// per CU hide each other's barriers (choose_tile_shape); there nk_step_b is held to 64 registers so
+// that four of them fit a CU (69 registers allow three).
+using TileWide = TileShape<128, 16, 1024, 4>;
+using TileNarrow = TileShape<64, 16, 512, 8>;
 
 // step 1: beta = rz_k / rz_{k-1} (0 at k = 0); p = z + beta p_old; q = J p; p.q partials
-__global__ void __launch_bounds__(PT_THREADS) pcg_spmv(MeshP m, const uint8_t* __restrict__ cls,
+template <class TS>
+__global__ void __launch_bounds__(TS::NT) pcg_spmv(MeshP m, const uint8_t* __restrict__ cls,
                                                 const float* __restrict__ D, const float* __restrict__ z,
                                                 const float* __restrict__ pold, float* __restrict__ pnew,
                                                 float* __restrict__ q, const double* __restrict__ pz_cur,
@@ -334,12 +347,12 @@ __global__ void __launch_bounds__(PT_THREADS) pcg_spmv(MeshP m, const uint8_t* _
         combine_partials<2>(pz_prev, nbz, prev);
         beta = (float)(cur[0] / prev[0]);
     }
-    __shared__ float ps[(PT_H + 2) * (PT_W + 2)];
-    const int nbx = (m.ni + PT_W - 1) / PT_W;
+    __shared__ float ps[(TS::H + 2) * (TS::W + 2)];
+    const int nbx = (m.ni + TS::W - 1) / TS::W;
     const int bx = blockIdx.x % nbx, by = blockIdx.x / nbx;
-    const int i0 = bx * PT_W - 1, j0 = by * PT_H - 1;
-    for (int t = threadIdx.x; t < (PT_H + 2) * (PT_W + 2); t += blockDim.x) {
-        int i = i0 + t % (PT_W + 2), j = j0 + t / (PT_W + 2);
+    const int i0 = bx * TS::W - 1, j0 = by * TS::H - 1;
+    for (int t = threadIdx.x; t < (TS::H + 2) * (TS::W + 2); t += blockDim.x) {
+        int i = i0 + t % (TS::W + 2), j = j0 + t / (TS::W + 2);
         float v = 0.0f;
         if (i >= 0 && j >= 0 && i < m.ni && j < m.nj) {
             size_t u = (size_t)j * m.ni + i;
@@ -348,14 +361,14 @@ __global__ void __launch_bounds__(PT_THREADS) pcg_spmv(MeshP m, const uint8_t* _
         ps[t] = v;
     }
     __syncthreads();
-    const int li = threadIdx.x % PT_W;
+    const int li = threadIdx.x % TS::W;
     double acc = 0;
-    for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += PT_RSTEP) {
+    for (int lj = threadIdx.x / TS::W; lj < TS::H; lj += TS::RSTEP) {
         const int i = i0 + 1 + li, j = j0 + 1 + lj;
         if (i >= m.ni || j >= m.nj) continue;
         size_t u = (size_t)j * m.ni + i;
         uint8_t c = cls[u];
-        int l = (lj + 1) * (PT_W + 2) + (li + 1);
+        int l = (lj + 1) * (TS::W + 2) + (li + 1);
         float pv = ps[l];
         float qv = 0.0f;
         if (c != CLS_NOT_FREE) {
@@ -363,8 +376,8 @@ __global__ void __launch_bounds__(PT_THREADS) pcg_spmv(MeshP m, const uint8_t* _
             qv = D[u] * pv;
             if (nbc(c, 0) == NB_FREE) qv -= (float)s.cE * ps[l + 1];
             if (nbc(c, 1) == NB_FREE) qv -= (float)s.cW * ps[l - 1];
-            if (nbc(c, 2) == NB_FREE) qv -= (float)s.cN * ps[l + PT_W + 2];
-            if (nbc(c, 3) == NB_FREE) qv -= (float)s.cS * ps[l - PT_W - 2];
+            if (nbc(c, 2) == NB_FREE) qv -= (float)s.cN * ps[l + TS::W + 2];
+            if (nbc(c, 3) == NB_FREE) qv -= (float)s.cS * ps[l - TS::W - 2];
         }
         pnew[u] = pv;
         q[u] = qv;
@@ -375,25 +388,27 @@ __global__ void __launch_bounds__(PT_THREADS) pcg_spmv(MeshP m, const uint8_t* _
 }
 
 // Preconditioner M: SGS_SWEEPS symmetric Gauss-Seidel sweeps of J in red-black order (red,
-// black, red, ..., red) on each PT_W x PT_H tile, with the values outside the tile taken as zero.
+// black, red, ..., red) on each TS::W x TS::H tile, with the values outside the tile taken as zero.
 // That is block Jacobi over the tiles with a symmetric inner solve, so M is symmetric positive
 // definite like the Jacobi preconditioner it replaces (the reference's LinearSolverPCG uses
 // M = diag(A), LinearSolverPCG.java:48-49).  On the 1024x512 cylinder deck a cold solve takes
 // 7242 PCG iterations with Jacobi, 4048 with one sweep and 2623 with three, and the sweeps stay
 // inside LDS.
 constexpr int SGS_SWEEPS = 3;
-// Thread t owns column t % PT_W and rows t / PT_W + PT_RSTEP s of the tile.
+// Thread t owns column t % TS::W and rows t / TS::W + TS::RSTEP s of the tile.
+template <class TS>
 struct TileRows {
-    float Dv[PT_RPT], cE[PT_RPT], cW[PT_RPT], cN[PT_RPT], cS[PT_RPT];
-    uint8_t c[PT_RPT];
-    bool in[PT_RPT];
+    float Dv[TS::RPT], cE[TS::RPT], cW[TS::RPT], cN[TS::RPT], cS[TS::RPT];
+    uint8_t c[TS::RPT];
+    bool in[TS::RPT];
 };
+template <class TS>
 __device__ __forceinline__ void tile_rows(const MeshP& m, const uint8_t* __restrict__ cls, const float* __restrict__ D,
-                                          int i0, int j0, TileRows& T) {
-    const int li = threadIdx.x % PT_W;
+                                          int i0, int j0, TileRows<TS>& T) {
+    const int li = threadIdx.x % TS::W;
 #pragma unroll
-    for (int s = 0; s < PT_RPT; s++) {
-        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
+    for (int s = 0; s < TS::RPT; s++) {
+        const int lj = threadIdx.x / TS::W + TS::RSTEP * s;
         const int i = i0 + li, j = j0 + lj;
         T.in[s] = i < m.ni && j < m.nj;
         T.c[s] = CLS_NOT_FREE;
@@ -407,9 +422,9 @@ __device__ __forceinline__ void tile_rows(const MeshP& m, const uint8_t* __restr
                 const Stencil st = stencil(m, i, j);
                 T.Dv[s] = D[u];
                 // only free neighbours inside the tile couple
-                T.cE[s] = (nbc(c, 0) == NB_FREE && li + 1 < PT_W) ? (float)st.cE : 0.0f;
+                T.cE[s] = (nbc(c, 0) == NB_FREE && li + 1 < TS::W) ? (float)st.cE : 0.0f;
                 T.cW[s] = (nbc(c, 1) == NB_FREE && li > 0) ? (float)st.cW : 0.0f;
-                T.cN[s] = (nbc(c, 2) == NB_FREE && lj + 1 < PT_H) ? (float)st.cN : 0.0f;
+                T.cN[s] = (nbc(c, 2) == NB_FREE && lj + 1 < TS::H) ? (float)st.cN : 0.0f;
                 T.cS[s] = (nbc(c, 3) == NB_FREE && lj > 0) ? (float)st.cS : 0.0f;
             }
         }
@@ -417,24 +432,25 @@ __device__ __forceinline__ void tile_rows(const MeshP& m, const uint8_t* __restr
 }
 // z = M^-1 r on the tile: rs holds r (0 at nodes that are not free or outside the mesh),
 // zs receives z.
-__device__ __forceinline__ void tile_sgs(const TileRows& T, int i0, int j0, const float* rs, float* zs) {
-    const int li = threadIdx.x % PT_W;
+template <class TS>
+__device__ __forceinline__ void tile_sgs(const TileRows<TS>& T, int i0, int j0, const float* rs, float* zs) {
+    const int li = threadIdx.x % TS::W;
 #pragma unroll
-    for (int s = 0; s < PT_RPT; s++) zs[(threadIdx.x / PT_W + PT_RSTEP * s) * PT_W + li] = 0.0f;
+    for (int s = 0; s < TS::RPT; s++) zs[(threadIdx.x / TS::W + TS::RSTEP * s) * TS::W + li] = 0.0f;
     __syncthreads();
 #pragma unroll
     for (int h = 0; h < 2 * SGS_SWEEPS + 1; h++) {
         const int color = h & 1;   // red (i + j even), black, red, ...
 #pragma unroll
-        for (int s = 0; s < PT_RPT; s++) {
-            const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
+        for (int s = 0; s < TS::RPT; s++) {
+            const int lj = threadIdx.x / TS::W + TS::RSTEP * s;
             if (((i0 + li + j0 + lj) & 1) != color || T.c[s] == CLS_NOT_FREE || !T.in[s]) continue;
-            const int l = lj * PT_W + li;
+            const int l = lj * TS::W + li;
             float acc = rs[l];
             if (T.cE[s] != 0.0f) acc += T.cE[s] * zs[l + 1];
             if (T.cW[s] != 0.0f) acc += T.cW[s] * zs[l - 1];
-            if (T.cN[s] != 0.0f) acc += T.cN[s] * zs[l + PT_W];
-            if (T.cS[s] != 0.0f) acc += T.cS[s] * zs[l - PT_W];
+            if (T.cN[s] != 0.0f) acc += T.cN[s] * zs[l + TS::W];
+            if (T.cS[s] != 0.0f) acc += T.cS[s] * zs[l - TS::W];
             zs[l] = acc / T.Dv[s];
         }
         __syncthreads();
@@ -443,7 +459,8 @@ __device__ __forceinline__ void tile_sgs(const TileRows& T, int i0, int j0, cons
 
 // z = M^-1 res of the refinement residual and the (r.z, r.r) partials of iteration 0; clears the
 // PCG state word for the iterations queued behind it.
-__global__ void __launch_bounds__(PT_THREADS) pcg_precond(MeshP m, const uint8_t* __restrict__ cls,
+template <class TS>
+__global__ void __launch_bounds__(TS::NT) pcg_precond(MeshP m, const uint8_t* __restrict__ cls,
                                                    const float* __restrict__ D, const float* __restrict__ res,
                                                    float* __restrict__ z, double* __restrict__ pz0,
                                                    int* __restrict__ istate) {
@@ -451,26 +468,26 @@ __global__ void __launch_bounds__(PT_THREADS) pcg_precond(MeshP m, const uint8_t
         istate[0] = 0;
         istate[1] = 0;
     }
-    __shared__ float rs[PT_H * PT_W], zs[PT_H * PT_W];
-    const int nbx = (m.ni + PT_W - 1) / PT_W;
-    const int i0 = (blockIdx.x % nbx) * PT_W, j0 = (blockIdx.x / nbx) * PT_H;
-    const int li = threadIdx.x % PT_W;
-    TileRows T;
-    tile_rows(m, cls, D, i0, j0, T);
-    float rv[PT_RPT];
+    __shared__ float rs[TS::H * TS::W], zs[TS::H * TS::W];
+    const int nbx = (m.ni + TS::W - 1) / TS::W;
+    const int i0 = (blockIdx.x % nbx) * TS::W, j0 = (blockIdx.x / nbx) * TS::H;
+    const int li = threadIdx.x % TS::W;
+    TileRows<TS> T;
+    tile_rows<TS>(m, cls, D, i0, j0, T);
+    float rv[TS::RPT];
 #pragma unroll
-    for (int s = 0; s < PT_RPT; s++) {
-        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
+    for (int s = 0; s < TS::RPT; s++) {
+        const int lj = threadIdx.x / TS::W + TS::RSTEP * s;
         rv[s] = (T.in[s] && T.c[s] != CLS_NOT_FREE) ? res[(size_t)(j0 + lj) * m.ni + i0 + li] : 0.0f;
-        rs[lj * PT_W + li] = rv[s];
+        rs[lj * TS::W + li] = rv[s];
     }
-    tile_sgs(T, i0, j0, rs, zs);
+    tile_sgs<TS>(T, i0, j0, rs, zs);
     double a_rz = 0, a_rr = 0;
 #pragma unroll
-    for (int s = 0; s < PT_RPT; s++) {
-        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
+    for (int s = 0; s < TS::RPT; s++) {
+        const int lj = threadIdx.x / TS::W + TS::RSTEP * s;
         if (!T.in[s]) continue;
-        const float zv = zs[lj * PT_W + li];
+        const float zv = zs[lj * TS::W + li];
         z[(size_t)(j0 + lj) * m.ni + i0 + li] = zv;
         a_rz += (double)rv[s] * (double)zv;
         a_rr += (double)rv[s] * (double)rv[s];
@@ -481,7 +498,8 @@ __global__ void __launch_bounds__(PT_THREADS) pcg_precond(MeshP m, const uint8_t
 
 // step 2: alpha = rz_k / p.q; d += alpha p; res -= alpha q; z = M^-1 res; (r.z, r.r) partials
 // of iteration k + 1 into pz_next.
-__global__ void __launch_bounds__(PT_THREADS) pcg_update(MeshP m, const uint8_t* __restrict__ cls,
+template <class TS>
+__global__ void __launch_bounds__(TS::NT) pcg_update(MeshP m, const uint8_t* __restrict__ cls,
                                                   const float* __restrict__ D, const float* __restrict__ p,
                                                   const float* __restrict__ q, float* __restrict__ d,
                                                   float* __restrict__ res, float* __restrict__ z,
@@ -493,16 +511,16 @@ __global__ void __launch_bounds__(PT_THREADS) pcg_update(MeshP m, const uint8_t*
     combine_partials<1>(ppq, nbq, pq);
     combine_partials<2>(pz_cur, nbz, cur);
     const float alpha = (float)(cur[0] / pq[0]);
-    __shared__ float rs[PT_H * PT_W], zs[PT_H * PT_W];
-    const int nbx = (m.ni + PT_W - 1) / PT_W;
-    const int i0 = (blockIdx.x % nbx) * PT_W, j0 = (blockIdx.x / nbx) * PT_H;
-    const int li = threadIdx.x % PT_W;
-    TileRows T;
-    tile_rows(m, cls, D, i0, j0, T);
-    float rv[PT_RPT];
+    __shared__ float rs[TS::H * TS::W], zs[TS::H * TS::W];
+    const int nbx = (m.ni + TS::W - 1) / TS::W;
+    const int i0 = (blockIdx.x % nbx) * TS::W, j0 = (blockIdx.x / nbx) * TS::H;
+    const int li = threadIdx.x % TS::W;
+    TileRows<TS> T;
+    tile_rows<TS>(m, cls, D, i0, j0, T);
+    float rv[TS::RPT];
 #pragma unroll
-    for (int s = 0; s < PT_RPT; s++) {
-        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
+    for (int s = 0; s < TS::RPT; s++) {
+        const int lj = threadIdx.x / TS::W + TS::RSTEP * s;
         rv[s] = 0.0f;
         if (T.in[s]) {
             const size_t u = (size_t)(j0 + lj) * m.ni + i0 + li;
@@ -512,15 +530,15 @@ __global__ void __launch_bounds__(PT_THREADS) pcg_update(MeshP m, const uint8_t*
             res[u] = rv[s];
             if (T.c[s] == CLS_NOT_FREE) rv[s] = 0.0f;
         }
-        rs[lj * PT_W + li] = rv[s];
+        rs[lj * TS::W + li] = rv[s];
     }
-    tile_sgs(T, i0, j0, rs, zs);
+    tile_sgs<TS>(T, i0, j0, rs, zs);
     double a_rz = 0, a_rr = 0;
 #pragma unroll
-    for (int s = 0; s < PT_RPT; s++) {
-        const int lj = threadIdx.x / PT_W + PT_RSTEP * s;
+    for (int s = 0; s < TS::RPT; s++) {
+        const int lj = threadIdx.x / TS::W + TS::RSTEP * s;
         if (!T.in[s]) continue;
-        const float zv = zs[lj * PT_W + li];
+        const float zv = zs[lj * TS::W + li];
         z[(size_t)(j0 + lj) * m.ni + i0 + li] = zv;
         a_rz += (double)rv[s] * (double)zv;
         a_rr += (double)rv[s] * (double)rv[s];
@@ -578,7 +596,8 @@ __device__ __forceinline__ int nk_decide(const NkState& S, const NkParams& np, c
     return NK_PCG;
 }
 
-__global__ void __launch_bounds__(PT_THREADS) nk_step_a(MeshP m, const uint8_t* __restrict__ cls,
+template <class TS>
+__global__ void __launch_bounds__(TS::NT) nk_step_a(MeshP m, const uint8_t* __restrict__ cls,
                                                  const float* __restrict__ D, const float* __restrict__ z,
                                                  float* __restrict__ p_even, float* __restrict__ p_odd,
                                                  float* __restrict__ q, double* __restrict__ x,
@@ -599,13 +618,13 @@ __global__ void __launch_bounds__(PT_THREADS) nk_step_a(MeshP m, const uint8_t* 
     int code;
     const int act = nk_decide(S, np, cur, code);
     if (act == NK_STOP) return;
-    const int nbx = (m.ni + PT_W - 1) / PT_W;
+    const int nbx = (m.ni + TS::W - 1) / TS::W;
     const int bx = blockIdx.x % nbx, by = blockIdx.x / nbx;
     if (act == NK_UPDATE) {
         // x += limited d on the tile's free nodes
-        const int li = threadIdx.x % PT_W;
-        for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += PT_RSTEP) {
-            const int ii = bx * PT_W + li, jj = by * PT_H + lj;
+        const int li = threadIdx.x % TS::W;
+        for (int lj = threadIdx.x / TS::W; lj < TS::H; lj += TS::RSTEP) {
+            const int ii = bx * TS::W + li, jj = by * TS::H + lj;
             if (ii >= m.ni || jj >= m.nj) continue;
             const size_t u = (size_t)jj * m.ni + ii;
             if (cls[u] == CLS_NOT_FREE) continue;
@@ -622,10 +641,10 @@ __global__ void __launch_bounds__(PT_THREADS) nk_step_a(MeshP m, const uint8_t* 
     if (S.k > 0) beta = (float)(cur[0] / nkc[2 * ((i - 1) % NK_RING)]);   // r.z of pair i-2's partials
     const float* pold = (i - 1) & 1 ? p_odd : p_even;
     float* pnew = i & 1 ? p_odd : p_even;
-    __shared__ float ps[(PT_H + 2) * (PT_W + 2)];
-    const int i0 = bx * PT_W - 1, j0 = by * PT_H - 1;
-    for (int t = threadIdx.x; t < (PT_H + 2) * (PT_W + 2); t += blockDim.x) {
-        const int ii = i0 + t % (PT_W + 2), jj = j0 + t / (PT_W + 2);
+    __shared__ float ps[(TS::H + 2) * (TS::W + 2)];
+    const int i0 = bx * TS::W - 1, j0 = by * TS::H - 1;
+    for (int t = threadIdx.x; t < (TS::H + 2) * (TS::W + 2); t += blockDim.x) {
+        const int ii = i0 + t % (TS::W + 2), jj = j0 + t / (TS::W + 2);
         float v = 0.0f;
         if (ii >= 0 && jj >= 0 && ii < m.ni && jj < m.nj) {
             const size_t u = (size_t)jj * m.ni + ii;
@@ -634,14 +653,14 @@ __global__ void __launch_bounds__(PT_THREADS) nk_step_a(MeshP m, const uint8_t* 
         ps[t] = v;
     }
     __syncthreads();
-    const int li = threadIdx.x % PT_W;
+    const int li = threadIdx.x % TS::W;
     double acc = 0;
-    for (int lj = threadIdx.x / PT_W; lj < PT_H; lj += PT_RSTEP) {
+    for (int lj = threadIdx.x / TS::W; lj < TS::H; lj += TS::RSTEP) {
         const int ii = i0 + 1 + li, jj = j0 + 1 + lj;
         if (ii >= m.ni || jj >= m.nj) continue;
         const size_t u = (size_t)jj * m.ni + ii;
         const uint8_t c = cls[u];
-        const int l = (lj + 1) * (PT_W + 2) + (li + 1);
+        const int l = (lj + 1) * (TS::W + 2) + (li + 1);
         const float pv = ps[l];
         float qv = 0.0f;
         if (c != CLS_NOT_FREE) {
@@ -649,8 +668,8 @@ __global__ void __launch_bounds__(PT_THREADS) nk_step_a(MeshP m, const uint8_t* 
             qv = D[u] * pv;
             if (nbc(c, 0) == NB_FREE) qv -= (float)st.cE * ps[l + 1];
             if (nbc(c, 1) == NB_FREE) qv -= (float)st.cW * ps[l - 1];
-            if (nbc(c, 2) == NB_FREE) qv -= (float)st.cN * ps[l + PT_W + 2];
-            if (nbc(c, 3) == NB_FREE) qv -= (float)st.cS * ps[l - PT_W - 2];
+            if (nbc(c, 2) == NB_FREE) qv -= (float)st.cN * ps[l + TS::W + 2];
+            if (nbc(c, 3) == NB_FREE) qv -= (float)st.cS * ps[l - TS::W - 2];
         }
         pnew[u] = pv;
         q[u] = qv;
@@ -661,7 +680,8 @@ __global__ void __launch_bounds__(PT_THREADS) nk_step_a(MeshP m, const uint8_t* 
 }
 
 // init = 1: the residual branch unconditionally (the state of pair 0, written to state[0]).
-__global__ void __launch_bounds__(PT_THREADS) nk_step_b(MeshP m, const uint8_t* __restrict__ cls,
+template <class TS>
+__global__ void __launch_bounds__(TS::NT, TS::MINW) nk_step_b(MeshP m, const uint8_t* __restrict__ cls,
                                                  const double* __restrict__ W, const double* __restrict__ bfix,
                                                  const double* __restrict__ rho, const double* __restrict__ x,
                                                  double* __restrict__ r, float* __restrict__ D,
@@ -703,17 +723,17 @@ __global__ void __launch_bounds__(PT_THREADS) nk_step_b(MeshP m, const uint8_t* 
         alpha = (float)(cur[0] / pq[0]);
     }
     const float* p = i & 1 ? p_odd : p_even;
-    __shared__ float rs[PT_H * PT_W], zs[PT_H * PT_W];
-    const int nbx = (m.ni + PT_W - 1) / PT_W;
-    const int i0 = (blockIdx.x % nbx) * PT_W, j0 = (blockIdx.x / nbx) * PT_H;
-    const int li = threadIdx.x % PT_W;
-    double rr64[PT_RPT] = {};
-    float rv[PT_RPT];
+    __shared__ float rs[TS::H * TS::W], zs[TS::H * TS::W];
+    const int nbx = (m.ni + TS::W - 1) / TS::W;
+    const int i0 = (blockIdx.x % nbx) * TS::W, j0 = (blockIdx.x / nbx) * TS::H;
+    const int li = threadIdx.x % TS::W;
+    double rr64[TS::RPT] = {};
+    float rv[TS::RPT];
     if (act == NK_UPDATE) {
         // FP64 Newton residual and Jacobian diagonal of the tile (the newton_residual kernel's rows)
 #pragma unroll
-        for (int s4 = 0; s4 < PT_RPT; s4++) {
-            const int lj = threadIdx.x / PT_W + PT_RSTEP * s4;
+        for (int s4 = 0; s4 < TS::RPT; s4++) {
+            const int lj = threadIdx.x / TS::W + TS::RSTEP * s4;
             const int ii = i0 + li, jj = j0 + lj;
             rv[s4] = 0.0f;
             if (ii < m.ni && jj < m.nj) {
@@ -740,16 +760,16 @@ __global__ void __launch_bounds__(PT_THREADS) nk_step_b(MeshP m, const uint8_t* 
                     rr64[s4] = ru;
                 }
             }
-            rs[lj * PT_W + li] = rv[s4];
+            rs[lj * TS::W + li] = rv[s4];
         }
         __syncthreads();   // the tile's D values, read by tile_rows below
     }
-    TileRows T;
-    tile_rows(m, cls, D, i0, j0, T);
+    TileRows<TS> T;
+    tile_rows<TS>(m, cls, D, i0, j0, T);
     if (act == NK_PCG) {
 #pragma unroll
-        for (int s4 = 0; s4 < PT_RPT; s4++) {
-            const int lj = threadIdx.x / PT_W + PT_RSTEP * s4;
+        for (int s4 = 0; s4 < TS::RPT; s4++) {
+            const int lj = threadIdx.x / TS::W + TS::RSTEP * s4;
             rv[s4] = 0.0f;
             if (T.in[s4]) {
                 const size_t u = (size_t)(j0 + lj) * m.ni + i0 + li;
@@ -759,16 +779,16 @@ __global__ void __launch_bounds__(PT_THREADS) nk_step_b(MeshP m, const uint8_t* 
                 res[u] = rv[s4];
                 if (T.c[s4] == CLS_NOT_FREE) rv[s4] = 0.0f;
             }
-            rs[lj * PT_W + li] = rv[s4];
+            rs[lj * TS::W + li] = rv[s4];
         }
     }
-    tile_sgs(T, i0, j0, rs, zs);
+    tile_sgs<TS>(T, i0, j0, rs, zs);
     double a_rz = 0, a_rr = 0;
 #pragma unroll
-    for (int s4 = 0; s4 < PT_RPT; s4++) {
-        const int lj = threadIdx.x / PT_W + PT_RSTEP * s4;
+    for (int s4 = 0; s4 < TS::RPT; s4++) {
+        const int lj = threadIdx.x / TS::W + TS::RSTEP * s4;
         if (!T.in[s4]) continue;
-        const float zv = zs[lj * PT_W + li];
+        const float zv = zs[lj * TS::W + li];
         z[(size_t)(j0 + lj) * m.ni + i0 + li] = zv;
         if (act == NK_UPDATE) {
             a_rz += rr64[s4] * (double)zv;
@@ -795,6 +815,26 @@ __global__ void __launch_bounds__(PT_THREADS) nk_step_b(MeshP m, const uint8_t* 
         }
         state[i % NK_RING] = o;
     }
+}
+
+template <class TS>
+static int tile_count(int ni, int nj) { return ((ni + TS::W - 1) / TS::W) * ((nj + TS::H - 1) / TS::H); }
+// Wide tiles while the mesh needs at most one of their workgroups per CU.  nk_step_b's 69 registers
+// leave room for one 16-wave workgroup per CU, so more wide tiles than CUs run in rounds with no
+// second workgroup to overlap a round's barriers and tail.  Measured warm solves: 1024 x 512 (256
+// wide tiles) 0.62 ms with wide tiles, 0.65 with narrow ones; 2048 x 1024 (1024 wide tiles) 21.9 ms
+// with wide tiles, 17.6 with narrow ones held to 64 registers (17.9 with wide ones held to 64).
+static bool choose_tile_shape(int ni, int nj) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+        ncu = 256;
+    return tile_count<TileWide>(ni, nj) <= ncu;
+}
+template <class F>
+static void with_tile_shape(bool wide, F&& f) {
+    if (wide) f(TileWide{});
+    else f(TileNarrow{});
 }
 
 // y += d (FP64 accumulation of the FP32 correction)
@@ -1015,7 +1055,8 @@ int poisson_setup(sf_ctx* ctx, const sf_poisson_desc* dsc) {
         ctx->poisson = P;
         auto A = [&](void** p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 8); };
         hipError_t e = hipSuccess;
-        int nb_tile = ((ni + PT_W - 1) / PT_W) * ((nj + PT_H - 1) / PT_H);
+        P->wide_tiles = choose_tile_shape(ni, nj);
+        const int nb_tile = P->wide_tiles ? tile_count<TileWide>(ni, nj) : tile_count<TileNarrow>(ni, nj);
         P->nblk_tile = nb_tile;
         P->nblk_elem = grid_elem((int64_t)nn);
         size_t npart = (size_t)std::max(nb_tile, P->nblk_elem) * 4 + 16;
@@ -1132,20 +1173,26 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         const double ee = std::max(eta_d, 1e-5);
         NkParams np{(const double*)(P->scal + S_R0), rel_tol * rel_tol, ee * ee, lim0, max_it, nl_max};
         auto step_b = [&](int i, int init) {
-            hipLaunchKernelGGL(nk_step_b, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls, (const double*)P->W,
-                               (const double*)P->bfix, (const double*)ctx->d_rho, (const double*)P->x, P->r, P->D,
-                               P->P, P->d, P->res, P->z, (const float*)P->p0, (const float*)P->p1,
-                               (const float*)P->q, (const double*)P->ppq, gt, pz0, pz1, gt, P->nk, (const double*)P->nkc,
-                               np, inv_eps, Cn0, D.phi0, inv_kTe, i, init);
+            with_tile_shape(P->wide_tiles, [&](auto ts) {
+                using TS = decltype(ts);
+                hipLaunchKernelGGL(nk_step_b<TS>, dim3(gt), dim3(TS::NT), 0, st, m, (const uint8_t*)P->cls,
+                                   (const double*)P->W, (const double*)P->bfix, (const double*)ctx->d_rho,
+                                   (const double*)P->x, P->r, P->D, P->P, P->d, P->res, P->z, (const float*)P->p0,
+                                   (const float*)P->p1, (const float*)P->q, (const double*)P->ppq, gt, pz0, pz1, gt,
+                                   P->nk, (const double*)P->nkc, np, inv_eps, Cn0, D.phi0, inv_kTe, i, init);
+            });
         };
         step_b(0, 1);
         int i = 1, batch = std::min(64, std::max(8, P->last_pairs + 2));
         while (true) {
             for (int b = 0; b < batch; b++, i++) {
-                hipLaunchKernelGGL(nk_step_a, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls,
-                                   (const float*)P->D, (const float*)P->z, P->p0, P->p1, P->q, P->x,
-                                   (const float*)P->d, (const double*)pz0, (const double*)pz1, gt, P->ppq,
-                                   (const NkState*)P->nk, P->nkc, np, i);
+                with_tile_shape(P->wide_tiles, [&](auto ts) {
+                    using TS = decltype(ts);
+                    hipLaunchKernelGGL(nk_step_a<TS>, dim3(gt), dim3(TS::NT), 0, st, m, (const uint8_t*)P->cls,
+                                       (const float*)P->D, (const float*)P->z, P->p0, P->p1, P->q, P->x,
+                                       (const float*)P->d, (const double*)pz0, (const double*)pz1, gt, P->ppq,
+                                       (const NkState*)P->nk, P->nkc, np, i);
+                });
                 step_b(i, 0);
             }
             SF_HIP(ctx, hipGetLastError());
@@ -1252,8 +1299,11 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
             // inner FP32 tolerance: a relative reduction FP32 can deliver
             double inner_tol = std::max(lin_target, 1e-5 * rr);
             const double tol2 = inner_tol * inner_tol;
-            hipLaunchKernelGGL(pcg_precond, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls, (const float*)P->D,
-                               (const float*)P->res, P->z, pz0, P->istate);
+            with_tile_shape(P->wide_tiles, [&](auto ts) {
+                using TS = decltype(ts);
+                hipLaunchKernelGGL(pcg_precond<TS>, dim3(gt), dim3(TS::NT), 0, st, m, (const uint8_t*)P->cls,
+                                   (const float*)P->D, (const float*)P->res, P->z, pz0, P->istate);
+            });
             // When this linear solve is the Newton step's last (it aims at the step's own target),
             // the update and the next Newton residual are queued behind the first batch of PCG
             // iterations.  They run only if that batch converged (the kernels read istate), and
@@ -1270,13 +1320,16 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
                 for (int b = 0; b < nb; b++, k++) {
                     double* cur = (k & 1) ? pz1 : pz0;
                     double* nxt = (k & 1) ? pz0 : pz1;
-                    hipLaunchKernelGGL(pcg_spmv, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls,
-                                       (const float*)P->D, (const float*)P->z, (const float*)pa, pb, P->q,
-                                       (const double*)cur, (const double*)nxt, gt, k, P->ppq, tol2, P->istate);
-                    hipLaunchKernelGGL(pcg_update, dim3(gt), dim3(PT_THREADS), 0, st, m, (const uint8_t*)P->cls,
-                                       (const float*)P->D, (const float*)pb, (const float*)P->q, P->d, P->res,
-                                       P->z, (const double*)P->ppq, gt, (const double*)cur, gt, nxt,
-                                       (const int*)P->istate);
+                    with_tile_shape(P->wide_tiles, [&](auto ts) {
+                        using TS = decltype(ts);
+                        hipLaunchKernelGGL(pcg_spmv<TS>, dim3(gt), dim3(TS::NT), 0, st, m, (const uint8_t*)P->cls,
+                                           (const float*)P->D, (const float*)P->z, (const float*)pa, pb, P->q,
+                                           (const double*)cur, (const double*)nxt, gt, k, P->ppq, tol2, P->istate);
+                        hipLaunchKernelGGL(pcg_update<TS>, dim3(gt), dim3(TS::NT), 0, st, m, (const uint8_t*)P->cls,
+                                           (const float*)P->D, (const float*)pb, (const float*)P->q, P->d, P->res,
+                                           P->z, (const double*)P->ppq, gt, (const double*)cur, gt, nxt,
+                                           (const int*)P->istate);
+                    });
                     std::swap(pa, pb);
                 }
                 const bool queued = ahead && first_batch;

@@ -207,3 +207,52 @@ def test_boltzmann_electron_density_matches_oracle():
         assert np.allclose(e_given.field("t"), 1.5 * QE / 1.3806503e-23, rtol=1e-15)
     finally:
         sim.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ni,nj", [(1024, 512), (1024, 1024)])
+def test_linear_potential_is_the_exact_solution_at_full_size(ni, nj):
+    """A closed form at full size: LEFT at 0 V, RIGHT at V, open TOP and BOTTOM (zero-gradient face
+    rows), so the linear phi* = V (x - x0) / Lx satisfies every face row and the interior stencil
+    exactly; with rho = e n0 exp((phi* - phi0) / kTe) the Boltzmann term cancels the source, so
+    phi* is the exact discrete solution.  1024 x 512 runs on the 128 x 16-node tiles, 1024 x 1024
+    (512 such tiles, more than the CUs) on the 64 x 16-node ones.  A cold solve from phi = 0 (the
+    host-checked Newton loop) and a warm solve from phi* plus a smooth 10 mV bump (the
+    device-driven Newton-PCG) must both return phi*.
+    Bound: the Debye length sqrt(eps kTe / (e n0)) = 1.05e-3 m is ten cells, so the Jacobian
+    L + diag(e n0 exp((phi - phi0) / kTe) / (eps kTe)) has its smallest eigenvalue above
+    p_min = 3.3e5 m^-2 (exp(-1) at phi = -2 V) and its largest below 8 / dx^2 = 8e8; with the
+    relative residual at 1e-8 (relative to the residual at phi = 0, which is J (phi* - 0) by the
+    mean value theorem), ||phi - phi*||_2 <= cond * 1e-8 ||phi*||_2 = 2.4e-5 ||phi*||_2."""
+    from starfish_amd.plugin import Simulation, _dp
+    import ctypes as C
+    V, dx, n0, kTe = -2.0, 1e-4, 1e14, 2.0
+    Lx = (ni - 1) * dx
+    mesh = D.MeshSpec("XY", ni, nj, (0.0, 0.0), (dx, dx), {"LEFT": ("DIRICHLET", 0.0), "RIGHT": ("DIRICHLET", V)})
+    deck = D.Deck(mesh, materials=[D.MaterialSpec("Xe+", "kinetic_hip", 131.0, 1.0, 1e4, 0.0)],
+                  solver=D.SolverSpec("poisson_hip", "GS", False, n0, kTe, 0.0), num_it=1, dt=1e-8, seed=1)
+    n = ni * nj
+    X = (np.arange(n) % ni) * dx
+    Y = (np.arange(n) // ni) * dx
+    exact = V * X / Lx
+    rho = QE * n0 * np.exp(exact / kTe)
+    sim = Simulation(deck, capacity=16)
+    try:
+        sim.ctx.call("sf_rho_set", _dp(rho))
+        info = sim.solver.last
+        rc = sim.ctx.call("sf_poisson_solve", None, C.byref(info))
+        err = np.linalg.norm(sim.phi() - exact) / np.linalg.norm(exact)
+        print(ni, nj, "cold: newton", info.newton_iterations, "pcg", info.pcg_iterations, "rel res",
+              info.rel_residual, "ms %.1f" % info.ms, "err %.2e" % err, flush=True)
+        assert rc == 0 and info.rel_residual <= 1e-8
+        assert err < 2.4e-5
+        bump = 0.01 * np.sin(np.pi * X / Lx) * np.sin(np.pi * Y / ((nj - 1) * dx))
+        sim.ctx.call("sf_phi_set", _dp(exact + bump))
+        rc = sim.ctx.call("sf_poisson_solve", None, C.byref(info))
+        err = np.linalg.norm(sim.phi() - exact) / np.linalg.norm(exact)
+        print(ni, nj, "warm: newton", info.newton_iterations, "pcg", info.pcg_iterations, "rel res",
+              info.rel_residual, "ms %.2f" % info.ms, "err %.2e" % err, flush=True)
+        assert rc == 0 and info.rel_residual <= 1e-8
+        assert err < 2.4e-5
+    finally:
+        sim.close()
