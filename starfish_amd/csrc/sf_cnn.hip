@@ -226,17 +226,23 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
 }
 
 // Weight gradient with the operands staged in LDS.  One workgroup: 32 input x 32 output channels
-// for all 9 taps over a band of 64-pixel row chunks; wave w owns the 16 x 16 sub-tile
-// (w & 1, w >> 1) of every tap.  Each chunk's three input rows (with the three horizontal shifts)
-// and its gradient row are staged transposed to [channel][pixel], so an operand fragment
-// (8 consecutive pixels of one channel) is one 16-byte LDS read, and the gradient fragment serves
-// the 9 taps.  Rows hold 16 four-pixel entries plus 2 of padding, so the 16 channels of a
-// fragment read start on distinct banks.  The next chunk's pixels are loaded into registers while
-// the current one is multiplied.  The workgroups of the first input-channel tile also sum the
-// gradient over the pixels (the bias gradient) with one more MFMA per step against an all-ones
-// operand.  Each workgroup stores its sums as a partial (WG_PART floats: [tap][k][n], then the 32
-// bias sums); wgrad_reduce adds the partials of a channel tile pair in a fixed order, which is
-// deterministic and avoids thousands of workgroups' atomics on the same addresses.
+// for all 9 taps over a run of chunks, a chunk being 64 pixels of one row; the chunks of a run go
+// down the rows of a 64-pixel column band, so the three input rows a chunk needs (y - 1, y, y + 1)
+// stay in a ring of three LDS rows and each chunk stages one new input row (three at the start of
+// a run or a band) and its gradient row.  Rows are staged transposed to [channel][pixel], each in
+// three copies shifted by one pixel (dx = -1, 0, 1), so an operand fragment (8 consecutive pixels
+// of one channel) is one 16-byte LDS read; a row holds 16 four-pixel entries plus 2 of padding, so
+// the 16 channels of a fragment read start on distinct banks.  Wave w takes the pixels
+// 32 (w & 1) .. + 31 of every chunk (one MFMA reduction step) and input channels 16 (w >> 1) .. + 15
+// against all 32 output channels: 9 input and 2 gradient fragments per 18 MFMAs.  The next
+// chunk's rows are loaded into registers while the current one is multiplied.  The waves of the
+// first input-channel tile also sum the gradient over the pixels (the bias gradient) with one
+// more MFMA per output tile against an all-ones operand.  At the end the two pixel halves are
+// added through LDS and the workgroup stores its sums as a partial (WG_PART floats: [tap][k][n],
+// then the 32 bias sums); wgrad_reduce adds the partials of a channel tile pair in a fixed order,
+// which is deterministic and avoids thousands of workgroups' atomics on the same addresses.
+// The kernel is held to 168 registers (three waves per SIMD: three workgroups per CU, as many as
+// its 46 KB of LDS allow); with 174 it ran two and took 243 instead of 141 us per launch.
 // Requirements: K and N multiples of 32, strides and offsets multiples of 8.
 constexpr int WG_PX = 64;           // pixels per chunk (two MFMA reduction steps)
 constexpr int WG_Q = WG_PX / 4 + 2; // four-pixel entries per staged row (with padding)
@@ -246,35 +252,41 @@ __device__ __forceinline__ uint32_t bf_at(const uint4& v, int c) {
     const uint32_t w = c < 2 ? v.x : (c < 4 ? v.y : (c < 6 ? v.z : v.w));
     return (c & 1) ? (w >> 16) : (w & 0xFFFFu);
 }
-__global__ void __launch_bounds__(256) conv3x3_wgrad_lds(const uint16_t* __restrict__ in, int in_cs, int in_co,
+__global__ void __launch_bounds__(256, 3) conv3x3_wgrad_lds(const uint16_t* __restrict__ in, int in_cs, int in_co,
                                                          int K, const uint16_t* __restrict__ g, int g_cs, int N,
                                                          float* __restrict__ part, int H, int Wd, int B,
                                                          int chunks_per_block) {
-    __shared__ uint2 s_in[3][3][32][WG_Q];   // [dx + 1][dy + 1][input channel][pixel / 4]
+    __shared__ uint2 s_in[3][3][32][WG_Q];   // [dx + 1][row % 3][input channel][pixel / 4]
     __shared__ uint2 s_g[32][WG_Q];          // [output channel][pixel / 4]
     const int nkt = K / 32;
     const int k0 = (blockIdx.y % nkt) * 32, n0 = (blockIdx.y / nkt) * 32;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int tk = wave & 1, tn = wave >> 1;
+    const int ks = wave & 1, tk = wave >> 1;   // pixel half (reduction step), input-channel half
     const int prow = lane & 15, kq = lane >> 4;
     const int cpr = (Wd + WG_PX - 1) / WG_PX;
     const int nchunks = B * H * cpr;
     const int c_begin = blockIdx.x * chunks_per_block;
     const int c_end = min(c_begin + chunks_per_block, nchunks);
-    // staging items: threads 0..191 one input item (row r, pixel quad q, channel group gq: pixels
-    // 4q - 1 .. 4q + 4 of 8 channels), threads 192..255 one gradient item (quad q, group gq).  The
-    // quad is the fastest index, so the 16 lanes of an LDS write phase cover all banks.
-    const bool is_in = t < 192;
-    const int it = is_in ? t : t - 192;
-    const int r = it / 64, q = it % 16, gq = (it / 16) % 4;
+    // chunk c: row y = c % H of column band (c / H) % cpr of image c / (H cpr)
+    // staging items: waves 0..2 an input row each (row y - 1 + r; in a run's steady state only
+    // r = 2, the new row y + 1), wave 3 the gradient row y; a thread's item is the pixel quad q
+    // (pixels 4q - 1 .. 4q + 4 of 8 channels for the input, 4q .. 4q + 3 for the gradient) of
+    // channel group gq.  The quad is the fastest index, so the 16 lanes of an LDS write phase cover
+    // all banks.
+    const bool is_in = wave < 3;
+    const int r = wave, q = lane & 15, gq = lane >> 4;
     uint4 v[6];
     // (the same loads for both kinds of item, so that v stays in registers)
     const uint16_t* const src = is_in ? in : g;
     const int scs = is_in ? in_cs : g_cs, sco = is_in ? in_co + k0 + gq * 8 : n0 + gq * 8;
-    const int sdy = is_in ? r - 1 : 0, sdx = is_in ? -1 : 0, sj = is_in ? 6 : 4;
+    const int sdx = is_in ? -1 : 0, sj = is_in ? 6 : 4;
+    // rows of chunk c this thread stages: the run's first chunk and the first row of a band
+    // stage all three input rows
+    auto stages = [&](int c) { return !is_in || r == 2 || c == c_begin || c % H == 0; };
     auto fetch = [&](int c) {
-        const int row = c / cpr, x0 = (c % cpr) * WG_PX;
-        const int b = row / H, yy = row % H + sdy;
+        const int y = c % H, band = c / H;
+        const int b = band / cpr, x0 = (band % cpr) * WG_PX;
+        const int yy = is_in ? y - 1 + r : y;
 #pragma unroll
         for (int j = 0; j < 6; j++) {
             const int xx = x0 + 4 * q + sdx + j;
@@ -283,10 +295,10 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_lds(const uint16_t* __restr
             v[j] = sel4(ok, *reinterpret_cast<const uint4*>(src + o));
         }
     };
-    f32x4 acc[9];
+    f32x4 acc[9][2];
 #pragma unroll
-    for (int tap = 0; tap < 9; tap++) acc[tap] = f32x4{0.f, 0.f, 0.f, 0.f};
-    f32x4 accb = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; tap++) acc[tap][0] = acc[tap][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    f32x4 accb[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
     const bool do_bias = k0 == 0 && tk == 0;   // uniform per wave
     bf16x8 ones;
     {
@@ -294,53 +306,83 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_lds(const uint16_t* __restr
         const uint4 u = make_uint4(o2, o2, o2, o2);
         memcpy(&ones, &u, 16);
     }
-    if (c_begin < c_end) fetch(c_begin);
+    if (c_begin < c_end && stages(c_begin)) fetch(c_begin);
     for (int c = c_begin; c < c_end; c++) {
+        const int y = c % H;
         __syncthreads();   // the previous chunk's fragments are read
-        if (is_in) {
+        if (stages(c)) {
+            if (is_in) {
+                const int slot = (y - 1 + r + 3) % 3;
 #pragma unroll
-            for (int d = 0; d < 3; d++)   // copy d holds pixels 4q + d - 1 + i: v[d + i]
+                for (int d = 0; d < 3; d++)   // copy d holds pixels 4q + d - 1 + i: v[d + i]
+#pragma unroll
+                    for (int ch = 0; ch < 8; ch++)
+                        s_in[d][slot][gq * 8 + ch][q] = make_uint2(bf_at(v[d], ch) | (bf_at(v[d + 1], ch) << 16),
+                                                                   bf_at(v[d + 2], ch) | (bf_at(v[d + 3], ch) << 16));
+            } else {
 #pragma unroll
                 for (int ch = 0; ch < 8; ch++)
-                    s_in[d][r][gq * 8 + ch][q] = make_uint2(bf_at(v[d], ch) | (bf_at(v[d + 1], ch) << 16),
-                                                            bf_at(v[d + 2], ch) | (bf_at(v[d + 3], ch) << 16));
-        } else {
-#pragma unroll
-            for (int ch = 0; ch < 8; ch++)
-                s_g[gq * 8 + ch][q] = make_uint2(bf_at(v[0], ch) | (bf_at(v[1], ch) << 16),
-                                                 bf_at(v[2], ch) | (bf_at(v[3], ch) << 16));
+                    s_g[gq * 8 + ch][q] = make_uint2(bf_at(v[0], ch) | (bf_at(v[1], ch) << 16),
+                                                     bf_at(v[2], ch) | (bf_at(v[3], ch) << 16));
+            }
         }
         __syncthreads();
-        if (c + 1 < c_end) fetch(c + 1);
+        if (c + 1 < c_end && stages(c + 1)) fetch(c + 1);
+        bf16x8 bg[2];
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-            bf16x8 bg;
-            {
-                const uint2 lo = s_g[tn * 16 + prow][8 * s + 2 * kq], hi = s_g[tn * 16 + prow][8 * s + 2 * kq + 1];
-                const uint4 u = make_uint4(lo.x, lo.y, hi.x, hi.y);
-                memcpy(&bg, &u, 16);
-            }
-            if (do_bias) accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ones, bg, accb, 0, 0, 0);
+        for (int n = 0; n < 2; n++) {
+            const uint2 lo = s_g[n * 16 + prow][8 * ks + 2 * kq], hi = s_g[n * 16 + prow][8 * ks + 2 * kq + 1];
+            const uint4 u = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            memcpy(&bg[n], &u, 16);
+            if (do_bias) accb[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ones, bg[n], accb[n], 0, 0, 0);
+        }
 #pragma unroll
-            for (int tap = 0; tap < 9; tap++) {
-                const int dy = tap / 3, dx = tap % 3;   // copy indices (offset + 1)
-                const uint2 lo = s_in[dx][dy][tk * 16 + prow][8 * s + 2 * kq];
-                const uint2 hi = s_in[dx][dy][tk * 16 + prow][8 * s + 2 * kq + 1];
-                const uint4 u = make_uint4(lo.x, lo.y, hi.x, hi.y);
-                bf16x8 a;
-                memcpy(&a, &u, 16);
-                acc[tap] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bg, acc[tap], 0, 0, 0);
-            }
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;   // copy indices (offset + 1)
+            const int slot = (y - 1 + dy + 3) % 3;
+            const uint2 lo = s_in[dx][slot][tk * 16 + prow][8 * ks + 2 * kq];
+            const uint2 hi = s_in[dx][slot][tk * 16 + prow][8 * ks + 2 * kq + 1];
+            const uint4 u = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            bf16x8 a;
+            memcpy(&a, &u, 16);
+            acc[tap][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bg[0], acc[tap][0], 0, 0, 0);
+            acc[tap][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bg[1], acc[tap][1], 0, 0, 0);
         }
     }
+    // the second pixel half's sums go through LDS (the input stage, 9 x 2 x 4 floats per lane and
+    // wave, then the bias sums) to the first half's waves, which add them and store the partial
+    __syncthreads();
+    float* const red = reinterpret_cast<float*>(&s_in[0][0][0][0]);   // 2 x 80 x 64 floats
+    static_assert(sizeof(s_in) >= 2 * 80 * 64 * sizeof(float), "reduction space");
+    if (ks == 1) {
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+            for (int n = 0; n < 2; n++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) red[((tk * 80) + (tap * 2 + n) * 4 + i) * 64 + lane] = acc[tap][n][i];
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) red[((tk * 80) + 72 + n * 4 + i) * 64 + lane] = accb[n][i];
+    }
+    __syncthreads();
+    if (ks == 1) return;
     // D rows (lane/16)*4+i = input channel, column lane%16 = output channel
     float* const pb = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * WG_PART;
 #pragma unroll
     for (int tap = 0; tap < 9; tap++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) pb[(tap * 32 + tk * 16 + kq * 4 + i) * 32 + tn * 16 + prow] = acc[tap][i];
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                pb[(tap * 32 + tk * 16 + kq * 4 + i) * 32 + n * 16 + prow] =
+                    acc[tap][n][i] + red[((tk * 80) + (tap * 2 + n) * 4 + i) * 64 + lane];
     // every row of accb holds the column sums: lanes 0..15 store row 0
-    if (tk == 0 && kq == 0) pb[9 * 32 * 32 + tn * 16 + prow] = do_bias ? accb[0] : 0.f;
+    if (tk == 0 && kq == 0)
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+            pb[9 * 32 * 32 + n * 16 + prow] = do_bias ? accb[n][0] + red[(72 + n * 4) * 64 + lane] : 0.f;
 }
 
 // Sum of the weight-gradient partials of channel tile pair y: a workgroup takes 32 entries e;
