@@ -418,18 +418,28 @@ __global__ void __launch_bounds__(256) wgrad_reduce(const float* __restrict__ pa
 
 
 // 2x2 average pooling (forward BF16, backward FP32 accumulate)
+// (8 channels per thread: C, strides and offsets multiples of 8)
 __global__ void avgpool_fwd(const uint16_t* __restrict__ in, int in_cs, int in_co, int C, int H, int Wd, int B,
                             uint16_t* __restrict__ out, int out_cs, int out_co) {
-    const int Ho = H / 2, Wo = Wd / 2;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * Ho * Wo * C;
+    const int Ho = H / 2, Wo = Wd / 2, G = C / 8;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)B * Ho * Wo * G;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int c = (int)(i % C);
-        int64_t q = i / C;
-        int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), b = (int)(q / ((int64_t)Wo * Ho));
-        float s = 0;
+        const int c = 8 * (int)(i % G);
+        const int64_t q = i / G;
+        const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), b = (int)(q / ((int64_t)Wo * Ho));
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int a = 0; a < 2; a++)
-            for (int e = 0; e < 2; e++) s += bf2f(in[(((size_t)b * H + 2 * yo + a) * Wd + 2 * xo + e) * in_cs + in_co + c]);
-        out[(((size_t)b * Ho + yo) * Wo + xo) * out_cs + out_co + c] = f2bf(0.25f * s);
+            for (int e = 0; e < 2; e++) {
+                const uint4 v = *reinterpret_cast<const uint4*>(
+                    in + (((size_t)b * H + 2 * yo + a) * Wd + 2 * xo + e) * in_cs + in_co + c);
+#pragma unroll
+                for (int k = 0; k < 8; k++) s[k] += bf2f((uint16_t)bf_at(v, k));
+            }
+        uint32_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = (uint32_t)f2bf(0.25f * s[2 * k]) | ((uint32_t)f2bf(0.25f * s[2 * k + 1]) << 16);
+        *reinterpret_cast<uint4*>(out + (((size_t)b * Ho + yo) * Wo + xo) * out_cs + out_co + c) =
+            make_uint4(o[0], o[1], o[2], o[3]);
     }
 }
 // Backward of the 2x2 average pooling fused with the ReLU backward of the pooled activation: the
@@ -490,12 +500,23 @@ __global__ void upsample_relu_bwd(const float* __restrict__ dout, int d_cs, int 
     }
 }
 // 1x1 head C -> 1 in FP32 on top of a base channel of the input: y = base + sum_c w[c] act[c] + b
-__global__ void head_fwd(const uint16_t* __restrict__ act, int C, int64_t npix, const float* __restrict__ w,
-                         const float* __restrict__ bias, const uint16_t* __restrict__ base, int base_cs,
-                         int base_co, float* __restrict__ y) {
+constexpr int HEAD_C = 32;   // channels into the 1x1 head
+__global__ void __launch_bounds__(256) head_fwd(const uint16_t* __restrict__ act, int64_t npix,
+                                                const float* __restrict__ w, const float* __restrict__ bias,
+                                                const uint16_t* __restrict__ base, int base_cs, int base_co,
+                                                float* __restrict__ y) {
+    float wv[HEAD_C];
+#pragma unroll
+    for (int c = 0; c < HEAD_C; c++) wv[c] = w[c];
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
         float s = bias[0] + bf2f(base[p * base_cs + base_co]);
-        for (int c = 0; c < C; c++) s += w[c] * bf2f(act[p * C + c]);
+        const uint4* a4 = reinterpret_cast<const uint4*>(act + p * HEAD_C);
+#pragma unroll
+        for (int q = 0; q < HEAD_C / 8; q++) {
+            const uint4 u = a4[q];
+#pragma unroll
+            for (int k = 0; k < 8; k++) s += wv[8 * q + k] * bf2f((uint16_t)bf_at(u, k));
+        }
         y[p] = s;
     }
 }
@@ -515,13 +536,13 @@ __global__ void mse_loss(const float* __restrict__ y, const float* __restrict__ 
     if (threadIdx.x == 0) {
         float s = 0.f;
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
-        atomicAdd(loss, s * inv_count);   // one atomic per block
+        atomicAdd(loss, s * inv_count);   // one atomic per block (launched with at most 256 blocks)
     }
 }
 // head backward (C = HEAD_C channels): dw[c] += sum_p dy[p] act[p][c]; db += sum dy;
-// gact[p][c] = act[p][c] > 0 ? dy[p] w[c] : 0 in BF16 (the ReLU backward of act fused).  Each thread keeps its channel sums in registers; the block
-// adds them with shuffles and one atomic per channel.
-constexpr int HEAD_C = 32;
+// gact[p][c] = act[p][c] > 0 ? dy[p] w[c] : 0 in BF16 (the ReLU backward of act fused).  Each
+// thread keeps its channel sums in registers; the block adds them with shuffles and one atomic per
+// channel.
 __global__ void __launch_bounds__(256) head_bwd(const uint16_t* __restrict__ act, int64_t npix,
                                                 const float* __restrict__ w, const float* __restrict__ dy,
                                                 float* __restrict__ dw, float* __restrict__ db,
@@ -747,11 +768,11 @@ static void forward(sf_cnn* c, int B) {
         conv_pack(c, *L);
     conv_fwd(c, c->c1a, c->x0, 32, 0, c->e1a, C1, 0, H, W, B);
     conv_fwd(c, c->c1b, c->e1a, C1, 0, c->u1, C2 + C1, C2, H, W, B);                  // e1 -> u1[C2:]
-    hipLaunchKernelGGL(avgpool_fwd, dim3(grid1((int64_t)B * H2 * W2 * C1)), dim3(256), 0, st, (const uint16_t*)c->u1,
+    hipLaunchKernelGGL(avgpool_fwd, dim3(grid1((int64_t)B * H2 * W2 * C1 / 8)), dim3(256), 0, st, (const uint16_t*)c->u1,
                        C2 + C1, C2, C1, H, W, B, c->p1, C1, 0);
     conv_fwd(c, c->c2a, c->p1, C1, 0, c->e2a, C2, 0, H2, W2, B);
     conv_fwd(c, c->c2b, c->e2a, C2, 0, c->u2, C3 + C2, C3, H2, W2, B);                // e2 -> u2[C3:]
-    hipLaunchKernelGGL(avgpool_fwd, dim3(grid1((int64_t)B * H4 * W4 * C2)), dim3(256), 0, st, (const uint16_t*)c->u2,
+    hipLaunchKernelGGL(avgpool_fwd, dim3(grid1((int64_t)B * H4 * W4 * C2 / 8)), dim3(256), 0, st, (const uint16_t*)c->u2,
                        C3 + C2, C3, C2, H2, W2, B, c->p2, C2, 0);
     conv_fwd(c, c->c3a, c->p2, C2, 0, c->e3a, C3, 0, H4, W4, B);
     conv_fwd(c, c->c3b, c->e3a, C3, 0, c->e3, C3, 0, H4, W4, B);
@@ -763,7 +784,7 @@ static void forward(sf_cnn* c, int B) {
                        C2, H2, W2, B, c->u1, C2 + C1, 0);
     conv_fwd(c, c->c5a, c->u1, C2 + C1, 0, c->d1a, C1, 0, H, W, B);
     conv_fwd(c, c->c5b, c->d1a, C1, 0, c->d1, C1, 0, H, W, B);
-    hipLaunchKernelGGL(head_fwd, dim3(grid1((int64_t)B * H * W)), dim3(256), 0, st, (const uint16_t*)c->d1, C1,
+    hipLaunchKernelGGL(head_fwd, dim3(grid1((int64_t)B * H * W)), dim3(256), 0, st, (const uint16_t*)c->d1,
                        (int64_t)B * H * W, (const float*)c->hw, (const float*)c->hb, (const uint16_t*)c->x0, 32, 3, c->y);
 }
 
@@ -1103,7 +1124,7 @@ int sf_cnn_train(sf_cnn* c, int steps, int batch, double lr, double* loss_out) {
         for (auto& g : G) SF_HIP(ctx, hipMemsetAsync(g.first, 0, g.second * 4, st));
         SF_HIP(ctx, hipMemsetAsync(c->loss, 0, 4, st));
         forward(c, batch);
-        hipLaunchKernelGGL(mse_loss, dim3(grid1((int64_t)batch * np)), dim3(256), 0, st, (const float*)c->y,
+        hipLaunchKernelGGL(mse_loss, dim3(std::min(grid1((int64_t)batch * np), 256)), dim3(256), 0, st, (const float*)c->y,
                            (const float*)c->bt, (const uint8_t*)c->bmask, (int64_t)batch * np, 1.0f / (float)counted,
                            c->dy, c->loss);
         backward(c, batch);
@@ -1143,7 +1164,7 @@ int sf_cnn_loss_and_grads(sf_cnn* c, int sample, double* loss_out, float* y_out,
     for (auto& g : G) SF_HIP(ctx, hipMemsetAsync(g.first, 0, g.second * 4, st));
     SF_HIP(ctx, hipMemsetAsync(c->loss, 0, 4, st));
     forward(c, 1);
-    hipLaunchKernelGGL(mse_loss, dim3(grid1(np)), dim3(256), 0, st, (const float*)c->y, (const float*)c->bt,
+    hipLaunchKernelGGL(mse_loss, dim3(std::min(grid1(np), 256)), dim3(256), 0, st, (const float*)c->y, (const float*)c->bt,
                        (const uint8_t*)c->bmask, np, 1.0f / (float)np, c->dy, c->loss);
     backward(c, 1);
     SF_HIP(ctx, hipGetLastError());
