@@ -103,12 +103,12 @@ struct FView {
 // 16-byte (FP32) store.
 constexpr int CONV_TP = 128;
 template <int MODE, int TN, int R>
-__global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ in, int in_cs, int in_co, int K,
+__global__ void __launch_bounds__(256, 2) conv3x3_lds(const uint16_t* __restrict__ in, int in_cs, int in_co, int K,
                                                    const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                                    int N, uint16_t* __restrict__ out, float* __restrict__ fout,
                                                    int out_cs, int out_co, int H, int Wd, int relu, int sgn,
                                                    int accumulate, const uint16_t* __restrict__ act, int act_cs,
-                                                   int act_co) {
+                                                   int act_co, int nrt) {
     constexpr int WN = TN / 32;                 // waves along the channels (two 16-channel tiles each)
     constexpr int WP = 4 / WN;                  // waves along the pixels
     constexpr int PT = CONV_TP / 16 / WP;       // 16-pixel tiles per wave and row
@@ -119,13 +119,18 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
     __shared__ uint4 s_in[NIN];                 // [row][pixel][group]
     __shared__ uint4 s_w[NWT];                  // [tap][channel][group]
     const int ntn = (N + TN - 1) / TN;
-    const int x0 = blockIdx.x * CONV_TP, y0 = blockIdx.y * R;
+    // the workgroup walks nrt tiles of R rows down its column; step it is chunk it % nkc of row
+    // tile it / nkc, so the next tile's first chunk is loaded while the current tile is multiplied
+    const int x0 = blockIdx.x * CONV_TP, y_first = blockIdx.y * nrt * R;
+    const int nkc = K / 32;
+    const int steps = min(nrt, (H - y_first + R - 1) / R) * nkc;
     const int b = blockIdx.z / ntn, n0 = (blockIdx.z % ntn) * TN;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wp = wave % WP, wn = wave / WP;
     const int prow = lane & 15, kq = lane >> 4;
     uint4 rin[RIN], rwt[RWT];
-    auto fetch = [&](int k0) __attribute__((always_inline)) {
+    auto fetch = [&](int it) __attribute__((always_inline)) {
+        const int y0 = y_first + (it / nkc) * R, k0 = (it % nkc) * 32;
 #pragma unroll
         for (int q = 0; q < RIN; q++) {
             const int e = threadIdx.x + 256 * q;
@@ -149,8 +154,8 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
     for (int r = 0; r < R; r++)
 #pragma unroll
         for (int p = 0; p < PT; p++) acc[r][p][0] = acc[r][p][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-    fetch(0);
-    for (int k0 = 0; k0 < K; k0 += 32) {
+    if (steps > 0) fetch(0);
+    for (int it = 0; it < steps; it++) {
         __syncthreads();   // the previous chunk's fragments are read
 #pragma unroll
         for (int q = 0; q < RIN; q++)
@@ -159,7 +164,7 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
         for (int q = 0; q < RWT; q++)
             if (threadIdx.x + 256 * q < NWT) s_w[threadIdx.x + 256 * q] = rwt[q];
         __syncthreads();
-        if (k0 + 32 < K) fetch(k0 + 32);
+        if (it + 1 < steps) fetch(it + 1);
         // (taps not unrolled: the R x PT x 2 tiles of a tap are, and the accumulators stay in
         // registers)
 #pragma unroll 1
@@ -183,45 +188,52 @@ __global__ void __launch_bounds__(256) conv3x3_lds(const uint16_t* __restrict__ 
                     acc[r][p][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[1], a, acc[r][p][1], 0, 0, 0);
                 }
         }
-    }
-    // D layout: lane holds rows (lane/16)*4 + i (channels), column lane%16 (pixel)
+        if (it % nkc != nkc - 1) continue;
+        // the row tile's last chunk: store its outputs and clear the accumulators
+        const int y0 = y_first + (it / nkc) * R;
+        // D layout: lane holds rows (lane/16)*4 + i (channels), column lane%16 (pixel)
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int y = y0 + r;
-        if (y >= H) continue;
+        for (int r = 0; r < R; r++) {
+            const int y = y0 + r;
+            if (y >= H) continue;
 #pragma unroll
-        for (int p = 0; p < PT; p++) {
-            const int xo = x0 + (wp * PT + p) * 16 + prow;
-            if (xo >= Wd) continue;
+            for (int p = 0; p < PT; p++) {
+                const int xo = x0 + (wp * PT + p) * 16 + prow;
+                if (xo >= Wd) continue;
 #pragma unroll
-            for (int t = 0; t < 2; t++) {
-                const int n = n0 + wn * 32 + t * 16 + kq * 4;   // this lane's 4 channels n .. n + 3
-                if (n >= N) continue;
-                const size_t o = (((size_t)b * H + y) * Wd + xo) * out_cs + out_co + n;
-                const f32x4 v = acc[r][p][t];
-                if (MODE == 0) {
-                    float e[4];
+                for (int t = 0; t < 2; t++) {
+                    const int n = n0 + wn * 32 + t * 16 + kq * 4;   // this lane's 4 channels n .. n + 3
+                    if (n >= N) continue;
+                    const size_t o = (((size_t)b * H + y) * Wd + xo) * out_cs + out_co + n;
+                    const f32x4 v = acc[r][p][t];
+                    if (MODE == 0) {
+                        float e[4];
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        e[i] = v[i] + bias[n + i];
-                        if (relu) e[i] = fmaxf(e[i], 0.f);
-                    }
-                    *reinterpret_cast<uint2*>(out + o) = make_uint2(pack_bf16x2(e[0], e[1]), pack_bf16x2(e[2], e[3]));
-                } else if (MODE == 2) {
-                    const uint2 a = *reinterpret_cast<const uint2*>(
-                        act + (((size_t)b * H + y) * Wd + xo) * act_cs + act_co + n);
-                    *reinterpret_cast<uint2*>(out + o) = pack_bf16x4(relu_mask4(make_float4(v[0], v[1], v[2], v[3]), a));
-                } else {
-                    float4* f = reinterpret_cast<float4*>(fout + o);
-                    if (accumulate) {
-                        const float4 g = *f;
-                        *f = make_float4(g.x + v[0], g.y + v[1], g.z + v[2], g.w + v[3]);
+                        for (int i = 0; i < 4; i++) {
+                            e[i] = v[i] + bias[n + i];
+                            if (relu) e[i] = fmaxf(e[i], 0.f);
+                        }
+                        *reinterpret_cast<uint2*>(out + o) = make_uint2(pack_bf16x2(e[0], e[1]), pack_bf16x2(e[2], e[3]));
+                    } else if (MODE == 2) {
+                        const uint2 a = *reinterpret_cast<const uint2*>(
+                            act + (((size_t)b * H + y) * Wd + xo) * act_cs + act_co + n);
+                        *reinterpret_cast<uint2*>(out + o) = pack_bf16x4(relu_mask4(make_float4(v[0], v[1], v[2], v[3]), a));
                     } else {
-                        *f = make_float4(v[0], v[1], v[2], v[3]);
+                        float4* f = reinterpret_cast<float4*>(fout + o);
+                        if (accumulate) {
+                            const float4 g = *f;
+                            *f = make_float4(g.x + v[0], g.y + v[1], g.z + v[2], g.w + v[3]);
+                        } else {
+                            *f = make_float4(v[0], v[1], v[2], v[3]);
+                        }
                     }
                 }
             }
         }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int p = 0; p < PT; p++) acc[r][p][0] = acc[r][p][1] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
 }
 
@@ -721,13 +733,16 @@ static void conv_launch(sf_cnn* c, const uint16_t* in, int in_cs, int in_co, int
     // rows per workgroup: 2 with 64-channel tiles, 4 with 32 (the same accumulator count)
     const bool t64 = N % 64 == 0;
     const int tn = t64 ? 64 : 32, rows = t64 ? 2 : 4;
-    const dim3 g((W + CONV_TP - 1) / CONV_TP, (H + rows - 1) / rows, B * ((N + tn - 1) / tn));
+    // row tiles per workgroup: up to 4 while the launch keeps at least 1024 workgroups
+    const int64_t tiles = (int64_t)((W + CONV_TP - 1) / CONV_TP) * ((H + rows - 1) / rows) * B * ((N + tn - 1) / tn);
+    const int nrt = (int)std::max<int64_t>(1, std::min<int64_t>(4, tiles / 1024));
+    const dim3 g((W + CONV_TP - 1) / CONV_TP, (H + rows * nrt - 1) / (rows * nrt), B * ((N + tn - 1) / tn));
     if (t64)
         hipLaunchKernelGGL((conv3x3_lds<MODE, 64, 2>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias,
-                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate, act, act_cs, act_co);
+                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate, act, act_cs, act_co, nrt);
     else
         hipLaunchKernelGGL((conv3x3_lds<MODE, 32, 4>), g, dim3(256), 0, c->ctx->stream, in, in_cs, in_co, K, w, bias,
-                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate, act, act_cs, act_co);
+                           N, out, fout, out_cs, out_co, H, W, relu, sgn, accumulate, act, act_cs, act_co, nrt);
 }
 // forward conv: in view (cs, co, K channels) -> out view, bias + ReLU
 static void conv_fwd(sf_cnn* c, const ConvL& L, const uint16_t* in, int in_cs, int in_co, uint16_t* out, int out_cs,
