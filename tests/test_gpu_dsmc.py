@@ -34,8 +34,11 @@ def _vel_by_id(gp):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["vhs", "vss", "two_species"])
+@pytest.mark.parametrize("case", ["vhs", "vss", "two_species", "dense_cells"])
 def test_dsmc_matches_oracle(case):
+    """dense_cells adds a cluster of 4000 particles over 2 x 2 cells (about 1000 per cell): cells
+    longer than 512 particles go through the kernel copy with 2048-entry lists, the others through
+    the 512-entry copy, in the same pass."""
     from starfish_amd.interactions import DsmcHip
     from starfish_amd.plugin import Simulation
     two = case == "two_species"
@@ -44,12 +47,21 @@ def test_dsmc_matches_oracle(case):
     o = oracle_for(deck, geo)
     n = 20000
     vss = (1.4, 1.4) if case == "vss" else (1.0, 1.0)
-    sim = Simulation(deck, capacity=2 * n, track_ids=True)
+    sim = Simulation(deck, capacity=2 * n + 4000, track_ids=True)
     dsmc = None
     try:
         mats = sim.materials
         for k, mat in enumerate(mats):
             x, y, v = _particles(deck, n // (k + 1), 7 + k, [200.0 * (1 - 2 * k), 0, 0])
+            if case == "dense_cells":
+                rng = np.random.default_rng(11)
+                m = deck.mesh
+                cx = rng.uniform(10, 12, 4000).astype(np.float32).astype(np.float64)
+                cy = rng.uniform(6, 8, 4000).astype(np.float32).astype(np.float64)
+                cv = (rng.normal(size=(4000, 3)) * 350.0).astype(np.float32).astype(np.float64)
+                x = np.concatenate([x, m.x0[0] + cx * m.dh[0]])
+                y = np.concatenate([y, m.x0[1] + cy * m.dh[1]])
+                v = np.concatenate([v, cv])
             ids = np.arange(len(x), dtype=np.int32)
             mat.add_particles(x, y, v[:, 0], v[:, 1], v[:, 2], ids=ids, rewind=False)
             o.load_particles(k, np.stack([x, y, np.zeros(len(x))], 1), v, ids=ids, rewind=False)
