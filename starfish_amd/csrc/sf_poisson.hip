@@ -34,8 +34,12 @@ constexpr int NK_RING = 8;
 // the reference residual, as from phi = 0), ETA_WARM for one started from the previous step's phi.
 // Measured on the 1024x512 cylinder deck: warm solves take 0.82, 0.76, 0.74, 0.70, 0.67 and
 // 0.68 ms at eta = 1e-3, 3e-3, 1e-2, 3e-2, 5e-2 and 1e-1 (fewer PCG iterations, a few more Newton
-// updates); a cold solve with 1e-2 takes 117 instead of 70 ms.
-constexpr double ETA_COLD = 1e-3, ETA_WARM = 5e-2;
+// updates); a cold solve with 1e-2 takes 117 instead of 70 ms.  On meshes too large for one
+// wide PCG tile per CU (narrow tiles, choose_tile_shape) a PCG iteration costs more and the
+// linear solves converge more slowly, and looser inner solves pay: on the 2048 x 1024 RZ deck warm
+// solves take 18.1, 17.6, 16.2 and 19.0 ms at eta = 2e-2, 5e-2, 1e-1 and 2e-1 (the 1024 x 512
+// deck: 0.63 ms at 5e-2, 0.66 at 1e-1, 0.70 at 2e-1).
+constexpr double ETA_COLD = 1e-3, ETA_WARM = 5e-2, ETA_WARM_LARGE = 1e-1;
 struct NkState {
     double tol2;   // PCG tolerance (squared) of the current linear solve
     double r2;     // ||r||^2 of the Newton residual at the current linear solve's start (at stop: final)
@@ -1222,12 +1226,13 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         converged = fin.done == NK_CONVERGED;
         return fin.done;
     };
+    const double eta_warm = P->wide_tiles ? ETA_WARM : ETA_WARM_LARGE;
     int rc = 0;
     // the last solve converged and this one starts from its potential: go straight to the device
     // loop (it hands a cold start back untouched)
     bool device_done = false;
     if (device_ok && P->warm_hint) {
-        const int code = device_newton(ETA_WARM);
+        const int code = device_newton(eta_warm);
         if (code < 0) return code;
         device_done = code != NK_COLD;
     }
@@ -1271,7 +1276,7 @@ int poisson_solve(sf_ctx* ctx, sf_solve_info* info) {
         if (nl_stop) { converged = true; break; }    // the reference's criterion met on the last step
         if (it == nl_max) break;
         if (!eta_chosen) {
-            eta = rel < 0.5 ? ETA_WARM : ETA_COLD;
+            eta = rel < 0.5 ? eta_warm : ETA_COLD;
             eta_chosen = true;
         }
         if (it == 0 && rel < 0.5 && device_ok) {
