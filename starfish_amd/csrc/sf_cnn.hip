@@ -601,16 +601,47 @@ __global__ void __launch_bounds__(256) head_bwd(const uint16_t* __restrict__ act
     }
 }
 // Adam (Kingma & Ba 2015) on FP32 master weights
-__global__ void adam_step(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                          float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
-                          float c2) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float gi = g[i];
-        float mi = b1 * m[i] + (1 - b1) * gi;
-        float vi = b2 * v[i] + (1 - b2) * gi * gi;
-        m[i] = mi;
-        v[i] = vi;
-        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+// The parameter tensors as one index space for the optimizer: segment k holds elements
+// [begin, begin + n) with its parameter, gradient and moment arrays.  One launch zeroes every
+// gradient and one runs Adam on every parameter (instead of one memset and one Adam launch per
+// tensor); each element finds its segment by binary search over the begins.
+struct AdamSeg {
+    float *p, *g, *m, *v;
+    int64_t begin, n;
+};
+constexpr int ADAM_MAXSEG = 32;
+__device__ __forceinline__ int adam_seg_of(const AdamSeg* s, int nseg, int64_t i) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s[mid].begin <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__global__ void __launch_bounds__(256) zero_grads(const AdamSeg* __restrict__ segs, int nseg, int64_t total) {
+    __shared__ AdamSeg s[ADAM_MAXSEG];
+    if (threadIdx.x < nseg) s[threadIdx.x] = segs[threadIdx.x];
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int k = adam_seg_of(s, nseg, i);
+        s[k].g[i - s[k].begin] = 0.f;
+    }
+}
+__global__ void __launch_bounds__(256) adam_segs(const AdamSeg* __restrict__ segs, int nseg, int64_t total, float lr,
+                                                 float b1, float b2, float eps, float c1, float c2) {
+    __shared__ AdamSeg s[ADAM_MAXSEG];
+    if (threadIdx.x < nseg) s[threadIdx.x] = segs[threadIdx.x];
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int k = adam_seg_of(s, nseg, i);
+        const int64_t j = i - s[k].begin;
+        const float gi = s[k].g[j];
+        const float mi = b1 * s[k].m[j] + (1 - b1) * gi;
+        const float vi = b2 * s[k].v[j] + (1 - b2) * gi * gi;
+        s[k].m[j] = mi;
+        s[k].v[j] = vi;
+        s[k].p[j] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     }
 }
 // master FP32 weights [tap][n][k] (k < Kreal used, padded to K) -> BF16 forward [tap][n][K] and
@@ -653,6 +684,7 @@ struct sf_cnn {
     sf::ConvL c1a, c1b, c2a, c2b, c3a, c3b, c4a, c4b, c5a, c5b;
     float *hw = nullptr, *hb = nullptr, *ghw = nullptr, *ghb = nullptr;
     float *mhw = nullptr, *vhw = nullptr, *mhb = nullptr, *vhb = nullptr;
+    sf::AdamSeg* segs = nullptr;      // the optimizer's segment table (device)
     // activations (BF16 NHWC)
     uint16_t *x0 = nullptr, *e1a = nullptr, *u1 = nullptr, *p1 = nullptr, *e2a = nullptr, *u2 = nullptr,
              *p2 = nullptr, *e3a = nullptr, *e3 = nullptr, *d2a = nullptr, *d2 = nullptr, *d1a = nullptr,
@@ -972,6 +1004,7 @@ int sf_cnn_create(sf_ctx* ctx, int ni, int nj, int max_batch, uint64_t seed, sf_
     rc |= cnn_alloc(c, &c->vhw, C1);
     rc |= cnn_alloc(c, &c->mhb, 1);
     rc |= cnn_alloc(c, &c->vhb, 1);
+    rc |= cnn_alloc(c, &c->segs, ADAM_MAXSEG);
     rc |= cnn_alloc(c, &c->x0, n1 * 32);
     rc |= cnn_alloc(c, &c->e1a, n1 * C1);
     rc |= cnn_alloc(c, &c->u1, n1 * (C2 + C1));
@@ -1127,6 +1160,15 @@ int sf_cnn_train(sf_cnn* c, int steps, int batch, double lr, double* loss_out) {
     }
     mv.push_back({c->mhw, c->vhw});
     mv.push_back({c->mhb, c->vhb});
+    std::vector<AdamSeg> segs;
+    int64_t total = 0;
+    for (size_t q = 0; q < P.size(); q++) {
+        segs.push_back(AdamSeg{P[q].first, G[q].first, mv[q].first, mv[q].second, total, (int64_t)P[q].second});
+        total += (int64_t)P[q].second;
+    }
+    if (segs.size() > (size_t)ADAM_MAXSEG) return sf_fail(ctx, SF_ERR_STATE, "sf_cnn_train: too many tensors");
+    SF_HIP(ctx, hipMemcpy(c->segs, segs.data(), segs.size() * sizeof(AdamSeg), hipMemcpyHostToDevice));
+    const int nseg = (int)segs.size();
     for (int s = 0; s < steps; s++) {
         int64_t counted = 0;
         for (int b = 0; b < batch; b++) {
@@ -1136,7 +1178,7 @@ int sf_cnn_train(sf_cnn* c, int steps, int batch, double lr, double* loss_out) {
                                c->bt + b * np, c->bmask + b * np);
             counted += np;
         }
-        for (auto& g : G) SF_HIP(ctx, hipMemsetAsync(g.first, 0, g.second * 4, st));
+        hipLaunchKernelGGL(zero_grads, dim3(grid1(total)), dim3(256), 0, st, (const AdamSeg*)c->segs, nseg, total);
         SF_HIP(ctx, hipMemsetAsync(c->loss, 0, 4, st));
         forward(c, batch);
         hipLaunchKernelGGL(mse_loss, dim3(std::min(grid1((int64_t)batch * np), 256)), dim3(256), 0, st, (const float*)c->y,
@@ -1146,10 +1188,8 @@ int sf_cnn_train(sf_cnn* c, int steps, int batch, double lr, double* loss_out) {
         c->adam_t++;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.f - powf(b1, (float)c->adam_t), c2 = 1.f - powf(b2, (float)c->adam_t);
-        for (size_t q = 0; q < P.size(); q++)
-            hipLaunchKernelGGL(adam_step, dim3(grid1((int64_t)P[q].second)), dim3(256), 0, st, P[q].first,
-                               (const float*)G[q].first, mv[q].first, mv[q].second, (int64_t)P[q].second, (float)lr,
-                               b1, b2, 1e-8f, c1, c2);
+        hipLaunchKernelGGL(adam_segs, dim3(grid1(total)), dim3(256), 0, st, (const AdamSeg*)c->segs, nseg, total,
+                           (float)lr, b1, b2, 1e-8f, c1, c2);
         SF_HIP(ctx, hipGetLastError());
         if (s >= steps - std::max(1, steps / 10)) {
             float l = 0;
