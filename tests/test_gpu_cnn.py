@@ -128,6 +128,37 @@ def test_layers_output_and_gradients_match_oracle(ctx, ni, nj):
 
 
 @pytest.mark.gpu
+def test_row_tile_walk_matches_oracle(ctx):
+    """On a 1024 x 2048 grid a 32-channel convolution has 4096 tiles of 4 rows x 128 pixels, so each
+    workgroup walks four row tiles (the next tile's operands loaded during the current one's
+    MFMAs, the outputs stored and the accumulators cleared between tiles).  c5b's output is
+    checked against the oracle on the device's own input, with the criteria of the layer test."""
+    import cnn_oracle as N
+    from starfish_amd.learned import LearnedGuess
+    rng = np.random.default_rng(5)
+    net = LearnedGuess(ctx, ni=1024, nj=2048, max_batch=1)
+    try:
+        H, W = net.H, net.W
+        flat = N.init_params(seed=5).astype(np.float32)
+        net.set_params(flat)
+        x4, t, m = _synthetic_sample(rng, H, W)
+        net.add_sample(x4, t, m)
+        net.loss_and_grads(0)
+        k_in, k_out = N.DEVICE_TENSORS.index("d1a"), N.DEVICE_TENSORS.index("d1")
+        x = np.asarray(net.activation(k_in), np.float64).reshape(H, W, N.C1)
+        a = np.asarray(net.activation(k_out), np.float64).reshape(H, W, N.C1)
+        ref = N.conv_layer(flat.astype(np.float64), "c5b", x)
+        e = _rel(a, ref)
+        print("c5b at %dx%d: forward rel %.2e, BF16 values differing %.2e" % (W, H, e, np.mean(a != ref)), flush=True)
+        step = 2.0 ** -7 * np.maximum(np.abs(a), np.abs(ref)) + 1e-6 * np.abs(ref).max()
+        assert np.all(np.abs(a - ref) <= step)
+        assert np.mean(a != ref) < 1e-3
+        assert e < 3e-4
+    finally:
+        net.close()
+
+
+@pytest.mark.gpu
 def test_training_reduces_the_loss(ctx):
     """A smooth target that depends on the input through a local stencil: Adam on the device
     lowers the masked MSE of the initial network by at least 10x within 300 steps."""
